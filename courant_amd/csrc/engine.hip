@@ -1,0 +1,1512 @@
+// MI355X (gfx950) time-stepping engine behind the courant.h ABI.
+//
+// Per step (order contract of solver/src/sim.c:867-933):
+//   k_update_h : bulk H curl-E update fused with the CPML psi_H recursions
+//                (sim.c:579-606, 680-754), one thread per (j,k) column of an
+//                i-segment, FP32;
+//   k_update_e : bulk E update fused with CPML psi_E (sim.c:608-637, 756-832)
+//                and the post-phase per-edge physics in reference order: Debye
+//                correction (dispersive.c:93-100), SIBC overwrite
+//                (sibc.c:256-269), lumped and modal sources (sim.c:900-909,
+//                modeport.c:130-135) and the Debye current update
+//                (dispersive.c:102-112).  Sparse edge sets are located with
+//                per-wavefront (base, 64-bit mask) tables: a lane's slot is
+//                base + popcount(mask below the lane), so the sparse state is
+//                read coalesced, in the reference's own (comp, i, j, k) order;
+//   k_gather   : port / probe line and loop integrals, one wavefront per
+//                sample, FP64 sums reduced with fixed-order shuffles, written
+//                to a per-step sample series;
+//   k_dft      : every chunk of steps, the FP64 running DFT of all samples
+//                (sim.c:840-861, export.c:121-149), one thread per
+//                (accumulator, frequency), phasors resynchronised every 1024
+//                steps and rotated in between exactly as the reference does.
+// Fields are component-major FP32 arrays with the reference index
+// i*si + j*sj + k, so every index list of the plan applies unchanged.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "courant_amd.h"
+#include "sim.hpp"
+
+using namespace cem;
+
+namespace cem {
+
+constexpr int kBlock = 256;        // 4 wavefronts of 64
+constexpr int kIseg = 8;           // i-planes per thread in the stencil kernels
+constexpr int kSeriesCap = 512;    // steps recorded between DFT flushes
+
+#define HIP_OK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            char m_[200];                                                             \
+            std::snprintf(m_, sizeof m_, "HIP error %s at %s:%d", hipGetErrorString(e_), \
+                          __FILE__, __LINE__);                                        \
+            set_error(m_);                                                            \
+            return -1;                                                                \
+        }                                                                             \
+    } while (0)
+
+#define HIP_CHECK_VOID(expr)                                                          \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            std::fprintf(stderr, "courant: HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                         \
+            std::abort();                                                             \
+        }                                                                             \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// kernel argument blocks
+// ---------------------------------------------------------------------------
+struct Grid {
+    int nx, ny, nz, S;      // S = (ny+1)(nz+1) points per x plane
+    long long si, sj, N;
+    int nchunk;             // ceil(S / 64) wavefront chunks per plane
+};
+
+struct Pml {                // CPML per derivative axis a: slot per index (-1 outside a slab)
+    const int *slot[3];
+    const float *b[3], *c[3];
+    long long per[3];       // points per slab plane
+};
+
+struct Sparse {             // per (comp, i, chunk) slot tables of one sparse edge kind
+    const unsigned *base;
+    const unsigned long long *mask;
+};
+
+struct HArgs {
+    Grid g;
+    const float *E;         // 3N
+    float *H;               // 3N
+    const float *hs[3];     // dt/mu0 * idh per axis
+    Pml pml;
+    float *psi[3][2];       // psiH[c][w]
+    float dth;
+    int has_pml;
+};
+
+struct EArgs {
+    Grid g;
+    float *E;               // 3N
+    const float *H;         // 3N
+    const float *ca, *cb;   // 3N each
+    const float *es[3];     // ide per axis
+    Pml pml;
+    float *psi[3][2];       // psiE[c][w]
+    int has_pml;
+    // Debye
+    Sparse dtab[3];
+    int ndisp, npole;
+    float *J;               // npole * ndisp (pole-major)
+    const float *bdt;       // beta/dt, npole * ndisp
+    float kc[kMaxPoles];    // (1 + k)/2
+    float km1[kMaxPoles];   // k - 1 (computed without cancellation)
+    // SIBC
+    Sparse stab[3];
+    int nsb, nterm;
+    float *Y;               // nterm * nsb (term-major)
+    const unsigned char *face;
+    float zc[kSibcTerms];   // c_k
+    float om[kSibcTerms];   // 1 - e_k, from -expm1(-a_k dt)
+    // sources
+    Sparse xtab[3];
+    const float *wl, *wm;   // lumped cs and modal weight per source slot
+    float vs, gm;           // waveform values at tH of this step
+};
+
+struct GatherArgs {
+    const float *F;         // 6N: E then H
+    const long long *tidx;  // term field index into F
+    const double *tw;       // term weight
+    const int *toff;        // nsamp + 1
+    const double *post;     // per-sample factor applied after the sum
+    int nsamp, nip, nport, nvp;
+    double *vlast;          // port V after the previous E update
+    double *row_ip, *row_vold, *row_vnew, *row_vp;   // series row of this step
+    int mid_only;           // 1: only refresh vlast (V before the E update)
+};
+
+struct DftArgs {
+    int nf, nport, nvp, nip, nacc, rows;
+    long long step0;
+    double dt;
+    const double *omega;    // 2 pi f
+    const double *rot;      // e^{-j w dt}, 2 nf
+    const double *ph_in;    // phE (2nf) | phH (2nf) at the first row
+    double *ph_out;         // same, advanced past the last row
+    const double *ser_ip, *ser_vold, *ser_vnew, *ser_vp;
+    const double *R;        // port resistances
+    int src_port, has_lumped, has_mode, direct;
+    double amp, t0, tau;    // lumped Gaussian (sim.c:834-838)
+    double mt0, mtau;       // modal differentiated Gaussian (modeport.c:124-128)
+    double *acc;            // nacc * 2nf
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_chunk(int p) {
+    return __builtin_amdgcn_readfirstlane(p >> 6);
+}
+
+__device__ __forceinline__ bool sparse_slot(const Sparse &t, long long row, int chunk, int lane, unsigned *slot) {
+    unsigned long long m = t.mask[row + chunk];
+    if (!((m >> lane) & 1ull)) return false;
+    *slot = t.base[row + chunk] + (unsigned)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1) {
+    const Grid &g = a.g;
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= g.S) return;
+    const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
+    const int ib = i0 + blockIdx.y * kIseg;
+    const int ie = min(i1, ib + kIseg);
+    const float *Ex = a.E, *Ey = a.E + g.N, *Ez = a.E + 2 * g.N;
+    float *Hx = a.H, *Hy = a.H + g.N, *Hz = a.H + 2 * g.N;
+    const bool jin = j < g.ny, kin = k < g.nz;
+    const float hy = jin ? a.hs[1][j] : 0.f, hz = kin ? a.hs[2][k] : 0.f;
+    const int sy = a.has_pml && jin ? a.pml.slot[1][j] : -1;
+    const int sz = a.has_pml && kin ? a.pml.slot[2][k] : -1;
+    for (int i = ib; i < ie; i++) {
+        const long long q = (long long)i * g.si + p;
+        const bool iin = i < g.nx;
+        const float ex = Ex[q], ey = Ey[q], ez = Ez[q];
+        const float dz_ey = kin ? Ey[q + 1] - ey : 0.f;
+        const float dy_ez = jin ? Ez[q + g.sj] - ez : 0.f;
+        const float dz_ex = kin ? Ex[q + 1] - ex : 0.f;
+        const float dy_ex = jin ? Ex[q + g.sj] - ex : 0.f;
+        const float dx_ez = iin ? Ez[q + g.si] - ez : 0.f;
+        const float dx_ey = iin ? Ey[q + g.si] - ey : 0.f;
+        const float hx = iin ? a.hs[0][i] : 0.f;
+        const int sx = a.has_pml && iin ? a.pml.slot[0][i] : -1;
+        if (jin && kin) {   // Hx: all i
+            float h = Hx[q] - (dy_ez * hy - dz_ey * hz);
+            if (sy >= 0) {
+                float *ps = a.psi[0][0] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
+                float v = a.pml.b[1][j] * *ps + a.pml.c[1][j] * dy_ez;
+                *ps = v;
+                h -= a.dth * v;
+            }
+            if (sz >= 0) {
+                float *ps = a.psi[0][1] + sz * a.pml.per[2] + (long long)i * (g.ny + 1) + j;
+                float v = a.pml.b[2][k] * *ps + a.pml.c[2][k] * dz_ey;
+                *ps = v;
+                h += a.dth * v;
+            }
+            Hx[q] = h;
+        }
+        if (iin && kin) {   // Hy: j <= ny
+            float h = Hy[q] - (dz_ex * hz - dx_ez * hx);
+            if (sx >= 0) {
+                float *ps = a.psi[1][1] + sx * a.pml.per[0] + p;
+                float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_ez;
+                *ps = v;
+                h += a.dth * v;
+            }
+            if (sz >= 0) {
+                float *ps = a.psi[1][0] + sz * a.pml.per[2] + (long long)i * (g.ny + 1) + j;
+                float v = a.pml.b[2][k] * *ps + a.pml.c[2][k] * dz_ex;
+                *ps = v;
+                h -= a.dth * v;
+            }
+            Hy[q] = h;
+        }
+        if (iin && jin) {   // Hz: k <= nz
+            float h = Hz[q] - (dx_ey * hx - dy_ex * hy);
+            if (sx >= 0) {
+                float *ps = a.psi[2][0] + sx * a.pml.per[0] + p;
+                float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_ey;
+                *ps = v;
+                h -= a.dth * v;
+            }
+            if (sy >= 0) {
+                float *ps = a.psi[2][1] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
+                float v = a.pml.b[1][j] * *ps + a.pml.c[1][j] * dy_ex;
+                *ps = v;
+                h += a.dth * v;
+            }
+            Hz[q] = h;
+        }
+    }
+}
+
+// Post-phase physics of one E edge (comp c, slot tables row r): Debye correction,
+// SIBC overwrite, sources, Debye current update, in the reference order.
+__device__ __forceinline__ float e_post(const EArgs &a, int c, long long row, int chunk, int lane, long long q,
+                                        float e, float e_old, float cb) {
+    const Grid &g = a.g;
+    unsigned dslot = 0;
+    const bool deb = a.ndisp && sparse_slot(a.dtab[c], row, chunk, lane, &dslot);
+    if (deb) {
+        float acc = 0.f;
+        for (int p = 0; p < a.npole; p++) acc += a.kc[p] * a.J[(long long)p * a.ndisp + dslot];
+        e -= cb * acc;
+    }
+    unsigned sslot = 0;
+    if (a.nsb && sparse_slot(a.stab[c], row, chunk, lane, &sslot)) {
+        const int u = (c + 1) % 3, v = (c + 2) % 3;
+        const long long su = u == 0 ? g.si : u == 1 ? g.sj : 1;
+        const long long sv = v == 0 ? g.si : v == 1 ? g.sj : 1;
+        const float *Hu = a.H + u * g.N, *Hv = a.H + v * g.N;
+        const unsigned m = a.face[sslot];
+        const float inv = 1.0f / (float)__builtin_popcount(m);
+        float Js = 0.f;
+        if (m & 1u) Js += -inv * Hu[q];
+        if (m & 2u) Js += inv * Hu[q - sv];
+        if (m & 4u) Js += inv * Hv[q];
+        if (m & 8u) Js += -inv * Hv[q - su];
+        float es = 0.f;
+        for (int t = 0; t < a.nterm; t++) {
+            float *y = a.Y + (long long)t * a.nsb + sslot;
+            float yv = *y;
+            yv += a.om[t] * (Js - yv);
+            *y = yv;
+            es += a.zc[t] * (Js - yv);
+        }
+        e = es;
+    }
+    unsigned xslot = 0;
+    if (sparse_slot(a.xtab[c], row, chunk, lane, &xslot)) e += a.wl[xslot] * a.vs + a.wm[xslot] * a.gm;
+    if (deb) {
+        const float de = e - e_old;
+        for (int p = 0; p < a.npole; p++) {
+            float *J = a.J + (long long)p * a.ndisp + dslot;
+            float jv = *J;
+            *J = jv + a.km1[p] * jv + a.bdt[(long long)p * a.ndisp + dslot] * de;
+        }
+    }
+    return e;
+}
+
+__global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1) {
+    const Grid &g = a.g;
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int chunk = wave_chunk(blockIdx.x * kBlock + threadIdx.x);
+    const int lane = threadIdx.x & 63;
+    if (p >= g.S) return;
+    const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
+    const int ib = i0 + blockIdx.y * kIseg;
+    const int ie = min(i1, ib + kIseg);
+    float *Ex = a.E, *Ey = a.E + g.N, *Ez = a.E + 2 * g.N;
+    const float *Hx = a.H, *Hy = a.H + g.N, *Hz = a.H + 2 * g.N;
+    const bool jb = j >= 1 && j < g.ny, kb = k >= 1 && k < g.nz;   // interior node ranges
+    const float ey_s = j <= g.ny ? a.es[1][j] : 0.f, ez_s = k <= g.nz ? a.es[2][k] : 0.f;
+    const int sy = a.has_pml ? a.pml.slot[1][j] : -1;
+    const int sz = a.has_pml ? a.pml.slot[2][k] : -1;
+    const long long plane = (long long)(g.nx + 1) * g.nchunk;
+    for (int i = ib; i < ie; i++) {
+        const long long q = (long long)i * g.si + p;
+        const bool ib_ = i >= 1 && i < g.nx;
+        const float ex_s = a.es[0][i];
+        const int sx = a.has_pml ? a.pml.slot[0][i] : -1;
+        const long long rbase = (long long)i * g.nchunk;
+        if (i < g.nx) {     // Ex
+            const float e0 = Ex[q];
+            float e = e0;
+            const float cb = a.cb[q];
+            if (jb && kb) {
+                const float dy_hz = Hz[q] - Hz[q - g.sj], dz_hy = Hy[q] - Hy[q - 1];
+                e = a.ca[q] * e0 + cb * (dy_hz * ey_s - dz_hy * ez_s);
+                if (sy >= 0) {
+                    float *ps = a.psi[0][0] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
+                    float v = a.pml.b[1][j] * *ps + a.pml.c[1][j] * dy_hz;
+                    *ps = v;
+                    e += cb * v;
+                }
+                if (sz >= 0) {
+                    float *ps = a.psi[0][1] + sz * a.pml.per[2] + (long long)i * (g.ny + 1) + j;
+                    float v = a.pml.b[2][k] * *ps + a.pml.c[2][k] * dz_hy;
+                    *ps = v;
+                    e -= cb * v;
+                }
+            }
+            e = e_post(a, 0, 0 * plane + rbase, chunk, lane, q, e, e0, cb);
+            if (e != e0) Ex[q] = e;
+        }
+        if (j < g.ny) {     // Ey
+            const float e0 = Ey[q];
+            float e = e0;
+            const float cb = a.cb[g.N + q];
+            if (ib_ && kb) {
+                const float dz_hx = Hx[q] - Hx[q - 1], dx_hz = Hz[q] - Hz[q - g.si];
+                e = a.ca[g.N + q] * e0 + cb * (dz_hx * ez_s - dx_hz * ex_s);
+                if (sx >= 0) {
+                    float *ps = a.psi[1][1] + sx * a.pml.per[0] + p;
+                    float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hz;
+                    *ps = v;
+                    e -= cb * v;
+                }
+                if (sz >= 0) {
+                    float *ps = a.psi[1][0] + sz * a.pml.per[2] + (long long)i * (g.ny + 1) + j;
+                    float v = a.pml.b[2][k] * *ps + a.pml.c[2][k] * dz_hx;
+                    *ps = v;
+                    e += cb * v;
+                }
+            }
+            e = e_post(a, 1, 1 * plane + rbase, chunk, lane, q, e, e0, cb);
+            if (e != e0) Ey[q] = e;
+        }
+        if (k < g.nz) {     // Ez
+            const float e0 = Ez[q];
+            float e = e0;
+            const float cb = a.cb[2 * g.N + q];
+            if (ib_ && jb) {
+                const float dx_hy = Hy[q] - Hy[q - g.si], dy_hx = Hx[q] - Hx[q - g.sj];
+                e = a.ca[2 * g.N + q] * e0 + cb * (dx_hy * ex_s - dy_hx * ey_s);
+                if (sx >= 0) {
+                    float *ps = a.psi[2][0] + sx * a.pml.per[0] + p;
+                    float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hy;
+                    *ps = v;
+                    e += cb * v;
+                }
+                if (sy >= 0) {
+                    float *ps = a.psi[2][1] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
+                    float v = a.pml.b[1][j] * *ps + a.pml.c[1][j] * dy_hx;
+                    *ps = v;
+                    e -= cb * v;
+                }
+            }
+            e = e_post(a, 2, 2 * plane + rbase, chunk, lane, q, e, e0, cb);
+            if (e != e0) Ez[q] = e;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather(GatherArgs a) {
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= a.nsamp) return;
+    const bool is_port = wave >= a.nip && wave < a.nip + a.nport;
+    if (a.mid_only && !is_port) return;
+    double acc = 0.0;
+    for (int t = a.toff[wave] + lane; t < a.toff[wave + 1]; t += 64) acc += (double)a.F[a.tidx[t]] * a.tw[t];
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane != 0) return;
+    const double v = acc * a.post[wave];
+    if (wave < a.nip) {
+        a.row_ip[wave] = v;
+    } else if (is_port) {
+        const int pp = wave - a.nip;
+        if (!a.mid_only) {
+            a.row_vold[pp] = a.vlast[pp];
+            a.row_vnew[pp] = v;
+        }
+        a.vlast[pp] = v;
+    } else {
+        a.row_vp[wave - a.nip - a.nport] = v;
+    }
+}
+
+__device__ __forceinline__ void dft_acc(double *acc, double v, double pr, double pi) {
+    acc[0] += v * pr;
+    acc[1] += v * pi;
+}
+
+__device__ __forceinline__ void phasor_at(double w, double t, double *pr, double *pi) {
+    *pr = cos(w * t);
+    *pi = -sin(w * t);
+}
+
+// One thread per (accumulator, frequency); accumulators: port V, port I,
+// voltage probes, current probes, source spectrum.  Threads with acc_id == nacc
+// only advance the phasor pair of their frequency (ping-pong buffers, so no
+// thread reads a phasor another block has already advanced).
+__global__ void k_dft(DftArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nf * (a.nacc + 1)) return;
+    const int f = t % a.nf, acc_id = t / a.nf;
+    int kind, idx;
+    if (acc_id < a.nport) { kind = 0; idx = acc_id; }
+    else if (acc_id < 2 * a.nport) { kind = 1; idx = acc_id - a.nport; }
+    else if (acc_id < 2 * a.nport + a.nvp) { kind = 2; idx = acc_id - 2 * a.nport; }
+    else if (acc_id < 2 * a.nport + a.nvp + a.nip) { kind = 3; idx = acc_id - 2 * a.nport - a.nvp; }
+    else if (acc_id < a.nacc) { kind = 4; idx = 0; }
+    else { kind = 5; idx = 0; }
+    const double w = a.omega[f], rr = a.rot[2 * f], ri = a.rot[2 * f + 1];
+    double er = a.ph_in[2 * f], ei = a.ph_in[2 * f + 1];
+    double hr = a.ph_in[2 * a.nf + 2 * f], hi = a.ph_in[2 * a.nf + 2 * f + 1];
+    double acc[2] = {0, 0};
+    double *dst = kind < 5 ? a.acc + (long long)acc_id * 2 * a.nf + 2 * f : nullptr;
+    if (dst) { acc[0] = dst[0]; acc[1] = dst[1]; }
+    for (int r = 0; r < a.rows; r++) {
+        const long long n = a.step0 + r;
+        if (a.direct || (n & 1023) == 0) {
+            phasor_at(w, (double)(n + 1) * a.dt, &er, &ei);
+            phasor_at(w, ((double)n + 0.5) * a.dt, &hr, &hi);
+        }
+        const double tH = ((double)n + 0.5) * a.dt;
+        double vs = 0, gm = 0;
+        if ((kind == 1 || kind == 4) && a.has_lumped) {
+            const double x = (tH - a.t0) / a.tau;
+            vs = a.amp * exp(-x * x);
+        }
+        if (kind == 4 && a.has_mode) {
+            const double x = (tH - a.mt0) / a.mtau;
+            gm = -2 * x * exp(-x * x);
+        }
+        switch (kind) {
+        case 0: {
+            const double V = 0.5 * (a.ser_vnew[(long long)r * a.nport + idx] + a.ser_vold[(long long)r * a.nport + idx]);
+            dft_acc(acc, V * a.dt, hr, hi);
+            break;
+        }
+        case 1: {
+            const double V = 0.5 * (a.ser_vnew[(long long)r * a.nport + idx] + a.ser_vold[(long long)r * a.nport + idx]);
+            const double I = ((idx == a.src_port ? vs : 0.0) - V) / a.R[idx];
+            dft_acc(acc, I * a.dt, hr, hi);
+            break;
+        }
+        case 2: dft_acc(acc, a.ser_vp[(long long)r * a.nvp + idx] * a.dt, er, ei); break;
+        case 3: dft_acc(acc, a.ser_ip[(long long)r * a.nip + idx] * a.dt, hr, hi); break;
+        case 4:
+            if (a.has_lumped) dft_acc(acc, vs * a.dt, hr, hi);
+            if (a.has_mode) dft_acc(acc, gm * a.dt, hr, hi);
+            break;
+        default: break;
+        }
+        double nr = er * rr - ei * ri, ni = er * ri + ei * rr;
+        er = nr;
+        ei = ni;
+        nr = hr * rr - hi * ri;
+        ni = hr * ri + hi * rr;
+        hr = nr;
+        hi = ni;
+    }
+    if (dst) { dst[0] = acc[0]; dst[1] = acc[1]; }
+    if (kind == 5) {
+        if (a.direct) {   // leave the phasors where the next step expects them (export.c:146-148)
+            const long long n = a.step0 + a.rows;
+            phasor_at(w, (double)(n + 1) * a.dt, &er, &ei);
+            phasor_at(w, ((double)n + 0.5) * a.dt, &hr, &hi);
+        }
+        a.ph_out[2 * f] = er;
+        a.ph_out[2 * f + 1] = ei;
+        a.ph_out[2 * a.nf + 2 * f] = hr;
+        a.ph_out[2 * a.nf + 2 * f + 1] = hi;
+    }
+}
+
+struct EnergyArgs {
+    Grid g;
+    const float *E, *H, *cb;
+    const double *d[3], *dd[3];
+    double dt;
+};
+
+// W = sum 1/2 eps E^2 V_dual + 1/2 mu0 H^2 V_primal, eps = dt/cb (sim.c:970-994)
+__global__ void __launch_bounds__(kBlock) k_energy(EnergyArgs a, double *partial) {
+    __shared__ double red[kBlock];
+    const Grid &g = a.g;
+    double acc = 0.0;
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < g.N; q += stride) {
+        const int i = (int)(q / g.si);
+        const int rem = (int)(q - (long long)i * g.si);
+        const int j = rem / (g.nz + 1), k = rem - j * (g.nz + 1);
+        const int ijk[3] = {i, j, k};
+        const int n[3] = {g.nx, g.ny, g.nz};
+        for (int c = 0; c < 3; c++) {
+            const int u = (c + 1) % 3, v = (c + 2) % 3;
+            const float cbv = a.cb[c * g.N + q];
+            if (ijk[c] < n[c] && cbv != 0.f) {
+                const double vol = a.d[c][ijk[c]] * a.dd[u][ijk[u]] * a.dd[v][ijk[v]];
+                const double eps = a.dt / (double)cbv;
+                const double e = a.E[c * g.N + q];
+                acc += 0.5 * eps * e * e * vol;
+            }
+            if (ijk[u] < n[u] && ijk[v] < n[v]) {
+                const double vol = a.dd[c][ijk[c]] * a.d[u][ijk[u]] * a.d[v][ijk[v]];
+                const double h = a.H[c * g.N + q];
+                acc += 0.5 * CEM_MU0 * h * h * vol;
+            }
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum(const double *partial, int n, double *out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0;
+        for (int i = 0; i < n; i++) s += partial[i];
+        *out = s;
+    }
+}
+
+__global__ void k_f32_to_f64(const float *src, double *dst, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) dst[t] = src[t];
+}
+
+__global__ void k_f64_to_f32(const double *src, float *dst, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) dst[t] = (float)src[t];
+}
+
+// ---------------------------------------------------------------------------
+// device state of one simulation
+// ---------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        n = count;
+        if (!count) return 0;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            set_error("device out of memory");
+            return -1;
+        }
+        return 0;
+    }
+    int upload(const T *src, size_t count, hipStream_t st) {   // synchronous: src may be a temporary
+        if (!count) return 0;
+        if (hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            set_error("device upload failed");
+            return -1;
+        }
+        return 0;
+    }
+    int zero(hipStream_t st) {
+        if (!n) return 0;
+        return hipMemsetAsync(p, 0, n * sizeof(T), st) == hipSuccess ? 0 : -1;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Per-kernel GPU time from event pairs recorded around each launch on the
+// engine's stream; pairs are only read back (and the host synchronised) when
+// the pool is full or the statistics are requested.
+struct KernelTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    size_t used = 0;
+    double total_ms = 0;
+    long count = 0;
+    void fold() {
+        if (!used) return;
+        HIP_CHECK_VOID(hipEventSynchronize(pool[used - 1].second));
+        for (size_t k = 0; k < used; k++) {
+            float ms = 0;
+            HIP_CHECK_VOID(hipEventElapsedTime(&ms, pool[k].first, pool[k].second));
+            total_ms += ms;
+        }
+        count += (long)used;
+        used = 0;
+    }
+    void destroy() {
+        for (auto &e : pool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        pool.clear();
+        used = 0;
+    }
+};
+
+struct Device {
+    hipStream_t st = nullptr;
+    Grid g{};
+    DBuf<float> F;               // E (3N) then H (3N)
+    DBuf<float> ca, cb;          // 3N each
+    DBuf<float> axis1d;          // hs[3], es[3], pml b/c for H and E per axis
+    DBuf<int> slots;             // CPML slot tables
+    DBuf<float> psi;             // all psi arrays
+    float *psiE[3][2] = {}, *psiH[3][2] = {};
+    size_t psiE_len[3][2] = {}, psiH_len[3][2] = {};
+    const float *hs[3] = {}, *es[3] = {};
+    Pml pmlH{}, pmlE{};
+    // sparse tables: kind 0 Debye, 1 SIBC, 2 source
+    DBuf<unsigned> tbase[3];
+    DBuf<unsigned long long> tmask[3];
+    DBuf<float> J, bdt, Y, wl, wm;
+    DBuf<unsigned char> face;
+    float kc[kMaxPoles] = {}, km1[kMaxPoles] = {};
+    float zc[kSibcTerms] = {}, om[kSibcTerms] = {};
+    bool sources_dirty = true;
+    // probes
+    DBuf<long long> tidx;
+    DBuf<double> tw, post;
+    DBuf<int> toff;
+    int nsamp = 0;
+    DBuf<double> vlast;
+    bool vlast_stale = false;
+    // series and DFT
+    DBuf<double> ser;            // ip | vold | vnew | vp, kSeriesCap rows each
+    double *ser_ip = nullptr, *ser_vold = nullptr, *ser_vnew = nullptr, *ser_vp = nullptr;
+    int rows = 0;
+    long long row0 = 0;
+    DBuf<double> omega, rot, acc, R;
+    DBuf<double> ph[2];          // ping-pong phasors: phE (2nf) | phH (2nf)
+    int ph_cur = 0;
+    int nacc = 0;
+    // energy
+    DBuf<double> d1d, partial;
+    int energy_blocks = 0;
+    // timing
+    bool profiling = false;
+    KernelTimer tH, tE, tG, tD;
+};
+
+}  // namespace cem
+
+// ---------------------------------------------------------------------------
+// plan upload
+// ---------------------------------------------------------------------------
+namespace {
+
+// Sparse edge tables for one kind: for every (comp, i, 64-point chunk) the slot
+// base and membership mask.  `member` lists (comp, q) in increasing order.
+int build_sparse(cem_sim *s, Device *D, int kind, const std::vector<std::pair<int, size_t>> &member) {
+    const Grid &g = D->g;
+    const size_t rows = 3 * (size_t)(g.nx + 1) * g.nchunk;
+    std::vector<unsigned> base(rows, 0);
+    std::vector<unsigned long long> mask(rows, 0);
+    size_t m = 0;
+    for (size_t r = 0; r < rows && m < member.size(); r++) {
+        const int c = (int)(r / ((size_t)(g.nx + 1) * g.nchunk));
+        const size_t rem = r % ((size_t)(g.nx + 1) * g.nchunk);
+        const size_t i = rem / g.nchunk, ch = rem % g.nchunk;
+        base[r] = (unsigned)m;
+        const size_t q0 = i * g.si + ch * 64;
+        while (m < member.size() && member[m].first == c && member[m].second < q0 + 64 &&
+               member[m].second >= q0 && member[m].second < (i + 1) * g.si) {
+            mask[r] |= 1ull << (member[m].second - q0);
+            m++;
+        }
+        // entries in rows of a later comp or plane keep base = m
+    }
+    if (m != member.size()) {
+        set_error("internal: sparse edge list not in grid order");
+        return -1;
+    }
+    if (D->tbase[kind].n != rows) {
+        D->tbase[kind].release();
+        D->tmask[kind].release();
+        if (D->tbase[kind].alloc(rows) || D->tmask[kind].alloc(rows)) return -1;
+    }
+    if (D->tbase[kind].upload(base.data(), rows, D->st) || D->tmask[kind].upload(mask.data(), rows, D->st)) return -1;
+    return hipStreamSynchronize(D->st) == hipSuccess ? 0 : -1;
+}
+
+int upload_sources(cem_sim *s, Device *D) {
+    std::vector<std::pair<std::pair<int, size_t>, std::pair<double, double>>> src;
+    if (s->src_port >= 0) {
+        const LumpedPort &pt = s->port[s->src_port];
+        for (size_t e = 0; e < pt.edge.size(); e++) src.push_back({{pt.axis, pt.edge[e]}, {pt.cs[e], 0.0}});
+    }
+    if (s->msrc >= 0) {
+        const ModePort &mp = s->mport[s->msrc];
+        for (size_t e = 0; e < mp.src_idx.size(); e++)
+            src.push_back({{mp.src_comp[e], mp.src_idx[e]}, {0.0, mp.src_w[e]}});
+    }
+    std::stable_sort(src.begin(), src.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    std::vector<std::pair<int, size_t>> member;
+    std::vector<float> wl, wm;
+    for (auto &e : src) {
+        if (!member.empty() && member.back() == e.first) {
+            wl.back() += (float)e.second.first;
+            wm.back() += (float)e.second.second;
+            continue;
+        }
+        member.push_back(e.first);
+        wl.push_back((float)e.second.first);
+        wm.push_back((float)e.second.second);
+    }
+    D->wl.release();
+    D->wm.release();
+    if (D->wl.alloc(wl.size()) || D->wm.alloc(wm.size())) return -1;
+    if (D->wl.upload(wl.data(), wl.size(), D->st) || D->wm.upload(wm.data(), wm.size(), D->st)) return -1;
+    if (build_sparse(s, D, 2, member)) return -1;
+    D->sources_dirty = false;
+    return 0;
+}
+
+int upload_plan(cem_sim *s, Device *D) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    Grid &g = D->g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.S = (ny + 1) * (nz + 1);
+    g.si = (long long)s->si;
+    g.sj = (long long)s->sj;
+    g.N = (long long)s->N;
+    g.nchunk = (g.S + 63) / 64;
+    const size_t N = s->N;
+    hipStream_t st = D->st;
+
+    // fields (zero) and coefficients
+    if (D->F.alloc(6 * N) || D->F.zero(st)) return -1;
+    if (D->ca.alloc(3 * N) || D->cb.alloc(3 * N)) return -1;
+    {
+        std::vector<float> tmp(N);
+        for (int c = 0; c < 3; c++) {
+            for (size_t q = 0; q < N; q++) tmp[q] = (float)s->ca[c][q];
+            HIP_OK(hipMemcpy(D->ca.p + c * N, tmp.data(), N * sizeof(float), hipMemcpyHostToDevice));
+            for (size_t q = 0; q < N; q++) tmp[q] = (float)s->cb[c][q];
+            HIP_OK(hipMemcpy(D->cb.p + c * N, tmp.data(), N * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+
+    // 1D axis arrays: hs (n), es (n+1), bH, cH (n), bE, cE (n+1) per axis
+    const double dth = s->dt / CEM_MU0;
+    std::vector<float> ax;
+    size_t off_hs[3], off_es[3], off_bh[3], off_ch[3], off_be[3], off_ce[3];
+    for (int a = 0; a < 3; a++) {
+        const int n = s->n[a];
+        off_hs[a] = ax.size();
+        for (int i = 0; i < n; i++) ax.push_back((float)(dth * s->idh[a][i]));
+        off_es[a] = ax.size();
+        for (int i = 0; i <= n; i++) ax.push_back((float)s->ide[a][i]);
+        off_bh[a] = ax.size();
+        for (int i = 0; i < n; i++) ax.push_back((float)s->bh[a][i]);
+        off_ch[a] = ax.size();
+        for (int i = 0; i < n; i++) ax.push_back((float)(s->ch[a][i] * (1.0 / s->d[a][i])));
+        off_be[a] = ax.size();
+        for (int i = 0; i <= n; i++) ax.push_back((float)s->be[a][i]);
+        off_ce[a] = ax.size();
+        for (int i = 0; i <= n; i++) ax.push_back((float)(s->dd[a][i] > 0 ? s->ce[a][i] * (1.0 / s->dd[a][i]) : 0.0));
+    }
+    if (D->axis1d.alloc(ax.size()) || D->axis1d.upload(ax.data(), ax.size(), st)) return -1;
+    for (int a = 0; a < 3; a++) {
+        D->hs[a] = D->axis1d.p + off_hs[a];
+        D->es[a] = D->axis1d.p + off_es[a];
+        D->pmlH.b[a] = D->axis1d.p + off_bh[a];
+        D->pmlH.c[a] = D->axis1d.p + off_ch[a];
+        D->pmlE.b[a] = D->axis1d.p + off_be[a];
+        D->pmlE.c[a] = D->axis1d.p + off_ce[a];
+    }
+
+    // CPML slot tables and psi storage (compact layout of sim.c:655-678)
+    std::vector<int> slots;
+    size_t off_sh[3], off_se[3];
+    for (int a = 0; a < 3; a++) {
+        for (int is_e = 0; is_e < 2; is_e++) {
+            const int len = s->n[a] + (is_e ? 1 : 0);
+            (is_e ? off_se : off_sh)[a] = slots.size();
+            std::vector<int> sl(len + 1, -1);
+            if (s->has_pml) {
+                int r[2][2];
+                pml_ranges(s, a, is_e, r);
+                size_t lo = r[0][1] >= r[0][0] ? (size_t)(r[0][1] - r[0][0] + 1) : 0;
+                for (int sd = 0; sd < 2; sd++)
+                    for (int x = r[sd][0]; x <= r[sd][1]; x++)
+                        sl[x] = (int)(sd == 0 ? x - r[0][0] : lo + (x - r[1][0]));
+            }
+            slots.insert(slots.end(), sl.begin(), sl.end());
+        }
+        D->pmlH.per[a] = D->pmlE.per[a] = (long long)pml_per(s, a);
+    }
+    if (D->slots.alloc(slots.size()) || D->slots.upload(slots.data(), slots.size(), st)) return -1;
+    for (int a = 0; a < 3; a++) {
+        D->pmlH.slot[a] = D->slots.p + off_sh[a];
+        D->pmlE.slot[a] = D->slots.p + off_se[a];
+    }
+    size_t psi_total = 0;
+    for (int c = 0; c < 3; c++)
+        for (int w = 0; w < 2; w++) {
+            D->psiE_len[c][w] = psi_len(s, 1, c, w);
+            D->psiH_len[c][w] = psi_len(s, 0, c, w);
+            psi_total += D->psiE_len[c][w] + D->psiH_len[c][w];
+        }
+    if (D->psi.alloc(psi_total + 1) || D->psi.zero(st)) return -1;
+    {
+        size_t o = 0;
+        for (int c = 0; c < 3; c++)
+            for (int w = 0; w < 2; w++) {
+                D->psiE[c][w] = D->psi.p + o;
+                o += D->psiE_len[c][w];
+                D->psiH[c][w] = D->psi.p + o;
+                o += D->psiH_len[c][w];
+            }
+    }
+
+    // Debye ADE tables
+    {
+        std::vector<std::pair<int, size_t>> member(s->ndisp);
+        for (int e = 0; e < s->ndisp; e++) member[e] = {s->d_comp[e], s->d_idx[e]};
+        if (build_sparse(s, D, 0, member)) return -1;
+        const size_t nj = (size_t)s->npole * s->ndisp;
+        if (D->J.alloc(nj) || D->J.zero(st) || D->bdt.alloc(nj)) return -1;
+        std::vector<float> b(nj);
+        for (size_t q = 0; q < nj; q++) b[q] = (float)(s->d_beta[q] * (1.0 / s->dt));
+        if (D->bdt.upload(b.data(), nj, st)) return -1;
+        for (int p = 0; p < s->npole; p++) {
+            const double tau = 1.0 / (2 * M_PI * s->pole_f[p]);
+            const double x = s->dt / (2 * tau);
+            D->kc[p] = (float)(1.0 / (1.0 + x));          // (1 + k)/2
+            D->km1[p] = (float)(-2.0 * x / (1.0 + x));     // k - 1
+        }
+    }
+    // SIBC tables
+    {
+        std::vector<std::pair<int, size_t>> member(s->nsb);
+        for (int e = 0; e < s->nsb; e++) member[e] = {s->sb_comp[e], s->sb_idx[e]};
+        if (build_sparse(s, D, 1, member)) return -1;
+        const size_t ny_ = (size_t)s->nsb * s->sibc_npole;
+        if (D->Y.alloc(ny_) || D->Y.zero(st) || D->face.alloc(s->nsb)) return -1;
+        if (D->face.upload(s->sb_mask.data(), s->nsb, st)) return -1;
+        for (int t = 0; t < s->sibc_npole; t++) {
+            D->zc[t] = (float)s->sibc_c[t];
+            D->om[t] = (float)(-std::expm1(-s->sibc_a[t] * s->dt));
+        }
+    }
+    if (upload_sources(s, D)) return -1;
+
+    // probes: samples ip | ports | vp
+    {
+        std::vector<long long> tidx;
+        std::vector<double> tw, post;
+        std::vector<int> toff;
+        for (auto &p : s->ip) {
+            toff.push_back((int)tidx.size());
+            for (size_t e = 0; e < p.idx.size(); e++) {
+                tidx.push_back((long long)p.comp[e] * (long long)N + (long long)p.idx[e]);
+                tw.push_back(p.w[e]);
+            }
+            post.push_back(1.0);
+        }
+        for (auto &p : s->port) {
+            toff.push_back((int)tidx.size());
+            for (size_t e = 0; e < p.edge.size(); e++) {
+                tidx.push_back((long long)p.axis * (long long)N + (long long)p.edge[e]);
+                tw.push_back(p.dl[e]);
+            }
+            post.push_back((double)p.dir);
+        }
+        for (auto &p : s->vp) {
+            toff.push_back((int)tidx.size());
+            for (size_t e = 0; e < p.edge.size(); e++) {
+                tidx.push_back((long long)p.axis * (long long)N + (long long)p.edge[e]);
+                tw.push_back(p.dl[e]);
+            }
+            post.push_back(1.0);
+        }
+        toff.push_back((int)tidx.size());
+        D->nsamp = (int)post.size();
+        if (D->tidx.alloc(tidx.size() + 1) || D->tw.alloc(tw.size() + 1) || D->toff.alloc(toff.size()) ||
+            D->post.alloc(post.size() + 1))
+            return -1;
+        if (D->tidx.upload(tidx.data(), tidx.size(), st) || D->tw.upload(tw.data(), tw.size(), st) ||
+            D->toff.upload(toff.data(), toff.size(), st) || D->post.upload(post.data(), post.size(), st))
+            return -1;
+        if (D->vlast.alloc(s->port.size() + 1) || D->vlast.zero(st)) return -1;
+    }
+    // series and DFT state
+    {
+        const size_t nip = s->ip.size(), np = s->port.size(), nvp = s->vp.size();
+        const size_t per = nip + 2 * np + nvp;
+        if (D->ser.alloc(per * kSeriesCap + 1)) return -1;
+        double *b = D->ser.p;
+        D->ser_ip = b;
+        b += nip * kSeriesCap;
+        D->ser_vold = b;
+        b += np * kSeriesCap;
+        D->ser_vnew = b;
+        b += np * kSeriesCap;
+        D->ser_vp = b;
+        const int nf = s->nf;
+        D->nacc = (int)(2 * np + nvp + nip + 1);
+        std::vector<double> om(nf), R(np + 1, 1.0);
+        for (int f = 0; f < nf; f++) om[f] = 2 * M_PI * s->freq[f];
+        for (size_t p = 0; p < np; p++) R[p] = s->port[p].R;
+        if (D->omega.alloc(nf) || D->omega.upload(om.data(), nf, st)) return -1;
+        if (D->rot.alloc(2 * nf) || D->rot.upload(s->rotE.data(), 2 * nf, st)) return -1;
+        for (int b2 = 0; b2 < 2; b2++)
+            if (D->ph[b2].alloc(4 * nf) || D->ph[b2].zero(st)) return -1;
+        D->ph_cur = 0;
+        if (D->acc.alloc((size_t)D->nacc * 2 * nf) || D->acc.zero(st)) return -1;
+        if (D->R.alloc(np + 1) || D->R.upload(R.data(), np + 1, st)) return -1;
+        D->rows = 0;
+        D->row0 = 0;
+    }
+    // energy
+    {
+        std::vector<double> d1;
+        for (int a = 0; a < 3; a++) d1.insert(d1.end(), s->d[a].begin(), s->d[a].end());
+        for (int a = 0; a < 3; a++) d1.insert(d1.end(), s->dd[a].begin(), s->dd[a].end());
+        if (D->d1d.alloc(d1.size()) || D->d1d.upload(d1.data(), d1.size(), st)) return -1;
+        D->energy_blocks = 1024;
+        if (D->partial.alloc(D->energy_blocks + 1)) return -1;
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+HArgs h_args(cem_sim *s, Device *D) {
+    HArgs a{};
+    a.g = D->g;
+    a.E = D->F.p;
+    a.H = D->F.p + 3 * D->g.N;
+    for (int i = 0; i < 3; i++) a.hs[i] = D->hs[i];
+    a.pml = D->pmlH;
+    for (int c = 0; c < 3; c++)
+        for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiH[c][w];
+    a.dth = (float)(s->dt / CEM_MU0);
+    a.has_pml = s->has_pml;
+    return a;
+}
+
+EArgs e_args(cem_sim *s, Device *D, long step) {
+    EArgs a{};
+    a.g = D->g;
+    a.E = D->F.p;
+    a.H = D->F.p + 3 * D->g.N;
+    a.ca = D->ca.p;
+    a.cb = D->cb.p;
+    for (int i = 0; i < 3; i++) a.es[i] = D->es[i];
+    a.pml = D->pmlE;
+    for (int c = 0; c < 3; c++)
+        for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiE[c][w];
+    a.has_pml = s->has_pml;
+    const long long rows = 3LL * (D->g.nx + 1) * D->g.nchunk;
+    (void)rows;
+    for (int c = 0; c < 3; c++) {
+        a.dtab[c] = Sparse{D->tbase[0].p, D->tmask[0].p};
+        a.stab[c] = Sparse{D->tbase[1].p, D->tmask[1].p};
+        a.xtab[c] = Sparse{D->tbase[2].p, D->tmask[2].p};
+    }
+    a.ndisp = s->ndisp;
+    a.npole = s->npole;
+    a.J = D->J.p;
+    a.bdt = D->bdt.p;
+    std::memcpy(a.kc, D->kc, sizeof a.kc);
+    std::memcpy(a.km1, D->km1, sizeof a.km1);
+    a.nsb = s->nsb;
+    a.nterm = s->sibc_npole;
+    a.Y = D->Y.p;
+    a.face = D->face.p;
+    std::memcpy(a.zc, D->zc, sizeof a.zc);
+    std::memcpy(a.om, D->om, sizeof a.om);
+    a.wl = D->wl.p;
+    a.wm = D->wm.p;
+    const double tH = (step + 0.5) * s->dt;
+    a.vs = (float)lumped_waveform(s, tH);
+    a.gm = (float)modeport_waveform(s, tH);
+    return a;
+}
+
+void timed_begin(Device *D, KernelTimer &t) {
+    if (!D->profiling) return;
+    if (t.used == 4096) t.fold();
+    if (t.used == t.pool.size()) {
+        hipEvent_t a, b;
+        HIP_CHECK_VOID(hipEventCreate(&a));
+        HIP_CHECK_VOID(hipEventCreate(&b));
+        t.pool.push_back({a, b});
+    }
+    HIP_CHECK_VOID(hipEventRecord(t.pool[t.used].first, D->st));
+}
+
+void timed_end(Device *D, KernelTimer &t) {
+    if (!D->profiling) return;
+    HIP_CHECK_VOID(hipEventRecord(t.pool[t.used].second, D->st));
+    t.used++;
+}
+
+void launch_h(cem_sim *s, Device *D, int i0, int i1) {
+    if (i1 <= i0) return;
+    HArgs a = h_args(s, D);
+    dim3 grid((D->g.S + kBlock - 1) / kBlock, (i1 - i0 + kIseg - 1) / kIseg);
+    timed_begin(D, D->tH);
+    hipLaunchKernelGGL(k_update_h, grid, dim3(kBlock), 0, D->st, a, i0, i1);
+    timed_end(D, D->tH);
+}
+
+void launch_e(cem_sim *s, Device *D, int i0, int i1) {
+    if (i1 <= i0) return;
+    EArgs a = e_args(s, D, s->step);
+    dim3 grid((D->g.S + kBlock - 1) / kBlock, (i1 - i0 + kIseg - 1) / kIseg);
+    timed_begin(D, D->tE);
+    hipLaunchKernelGGL(k_update_e, grid, dim3(kBlock), 0, D->st, a, i0, i1);
+    timed_end(D, D->tE);
+}
+
+void launch_dft(cem_sim *s, Device *D, int direct) {
+    if (!D->rows) return;
+    DftArgs a{};
+    a.nf = s->nf;
+    a.nport = (int)s->port.size();
+    a.nvp = (int)s->vp.size();
+    a.nip = (int)s->ip.size();
+    a.nacc = D->nacc;
+    a.rows = D->rows;
+    a.step0 = D->row0;
+    a.dt = s->dt;
+    a.omega = D->omega.p;
+    a.rot = D->rot.p;
+    a.ph_in = D->ph[D->ph_cur].p;
+    a.ph_out = D->ph[1 - D->ph_cur].p;
+    a.ser_ip = D->ser_ip;
+    a.ser_vold = D->ser_vold;
+    a.ser_vnew = D->ser_vnew;
+    a.ser_vp = D->ser_vp;
+    a.R = D->R.p;
+    a.src_port = s->src_port;
+    a.has_lumped = s->src_port >= 0;
+    a.has_mode = s->msrc >= 0;
+    a.direct = direct;
+    a.amp = s->src_amp;
+    a.t0 = s->src_t0;
+    a.tau = s->src_tau;
+    a.mt0 = s->msrc_t0;
+    a.mtau = s->msrc_tau;
+    a.acc = D->acc.p;
+    const int nt = a.nf * (a.nacc + 1);
+    timed_begin(D, D->tD);
+    hipLaunchKernelGGL(k_dft, dim3((nt + 127) / 128), dim3(128), 0, D->st, a);
+    timed_end(D, D->tD);
+    D->ph_cur = 1 - D->ph_cur;
+    D->row0 += D->rows;
+    D->rows = 0;
+}
+
+void launch_gather(cem_sim *s, Device *D, int mid_only) {
+    if (!D->nsamp) return;
+    GatherArgs a{};
+    a.F = D->F.p;
+    a.tidx = D->tidx.p;
+    a.tw = D->tw.p;
+    a.toff = D->toff.p;
+    a.post = D->post.p;
+    a.nsamp = D->nsamp;
+    a.nip = (int)s->ip.size();
+    a.nport = (int)s->port.size();
+    a.nvp = (int)s->vp.size();
+    a.vlast = D->vlast.p;
+    const int r = D->rows;
+    a.row_ip = D->ser_ip + (size_t)r * a.nip;
+    a.row_vold = D->ser_vold + (size_t)r * a.nport;
+    a.row_vnew = D->ser_vnew + (size_t)r * a.nport;
+    a.row_vp = D->ser_vp + (size_t)r * a.nvp;
+    a.mid_only = mid_only;
+    const int threads = D->nsamp * 64;
+    timed_begin(D, D->tG);
+    hipLaunchKernelGGL(k_gather, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, a);
+    timed_end(D, D->tG);
+}
+
+// post phase of one step: record samples, advance the step counter, flush the DFT when full
+void finish_step(cem_sim *s, Device *D) {
+    if (D->rows == 0) D->row0 = s->step;
+    launch_gather(s, D, 0);
+    D->rows++;
+    s->step++;
+    if (D->rows == kSeriesCap) launch_dft(s, D, 0);
+}
+
+int ensure_vlast(cem_sim *s, Device *D) {
+    if (D->vlast_stale) {
+        launch_gather(s, D, 1);
+        D->vlast_stale = false;
+    }
+    return 0;
+}
+
+void download_f32(Device *D, const float *src, double *dst, size_t n) {
+    std::vector<float> tmp(n);
+    HIP_CHECK_VOID(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, D->st));
+    HIP_CHECK_VOID(hipStreamSynchronize(D->st));
+    for (size_t q = 0; q < n; q++) dst[q] = tmp[q];
+}
+
+void upload_f32(Device *D, float *dst, const double *src, size_t n) {
+    std::vector<float> tmp(n);
+    for (size_t q = 0; q < n; q++) tmp[q] = (float)src[q];
+    HIP_CHECK_VOID(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, D->st));
+    HIP_CHECK_VOID(hipStreamSynchronize(D->st));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// engine entry points (called from setup.cpp)
+// ---------------------------------------------------------------------------
+int cem::dev_ensure(cem_sim *s) {
+    if (s->dev) {
+        if (s->dev->sources_dirty && upload_sources(s, s->dev)) return -1;
+        return 0;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error("no GPU device available for the courant engine");
+        return -1;
+    }
+    Device *D = new Device();
+    if (hipStreamCreateWithFlags(&D->st, hipStreamNonBlocking) != hipSuccess) {
+        set_error("cannot create HIP stream");
+        delete D;
+        return -1;
+    }
+    s->dev = D;
+    if (upload_plan(s, D)) {
+        dev_destroy(s);
+        return -1;
+    }
+    D->profiling = std::getenv("COURANT_PROFILE") != nullptr;
+    s->spec_src.assign(2 * s->nf, 0.0);
+    for (size_t p = 0; p < s->port.size(); p++) {
+        s->spec_port_v[p].assign(2 * s->nf, 0.0);
+        s->spec_port_i[p].assign(2 * s->nf, 0.0);
+    }
+    s->spec_vp.assign(s->vp.size(), dvec(2 * s->nf, 0.0));
+    s->spec_ip.assign(s->ip.size(), dvec(2 * s->nf, 0.0));
+    // phasors of step 0 are set by the DFT kernel's resync at n % 1024 == 0
+    for (int c = 0; c < 6; c++)
+        if (s->field_mirror[c]) D->vlast_stale = true;
+    if (s->step != 0) D->vlast_stale = true;
+    return 0;
+}
+
+void cem::dev_destroy(cem_sim *s) {
+    Device *D = s->dev;
+    if (!D) return;
+    (void)hipStreamSynchronize(D->st);
+    D->F.release();
+    D->ca.release();
+    D->cb.release();
+    D->axis1d.release();
+    D->slots.release();
+    D->psi.release();
+    for (int k = 0; k < 3; k++) {
+        D->tbase[k].release();
+        D->tmask[k].release();
+    }
+    D->J.release();
+    D->bdt.release();
+    D->Y.release();
+    D->wl.release();
+    D->wm.release();
+    D->face.release();
+    D->tidx.release();
+    D->tw.release();
+    D->post.release();
+    D->toff.release();
+    D->vlast.release();
+    D->ser.release();
+    D->omega.release();
+    D->rot.release();
+    D->ph[0].release();
+    D->ph[1].release();
+    D->acc.release();
+    D->R.release();
+    D->d1d.release();
+    D->partial.release();
+    for (KernelTimer *t : {&D->tH, &D->tE, &D->tG, &D->tD}) t->destroy();
+    (void)hipStreamDestroy(D->st);
+    delete D;
+    s->dev = nullptr;
+}
+
+void cem::dev_sources_changed(cem_sim *s) {
+    if (s->dev) s->dev->sources_dirty = true;
+}
+
+void cem::dev_step(cem_sim *s, long nsteps) {
+    Device *D = s->dev;
+    ensure_vlast(s, D);
+    const int nx = s->n[0];
+    for (long it = 0; it < nsteps; it++) {
+        launch_h(s, D, 0, nx + 1);
+        launch_e(s, D, 0, nx + 1);
+        finish_step(s, D);
+    }
+}
+
+void cem::dev_mt_H(cem_sim *s, int i0, int i1) {
+    Device *D = s->dev;
+    ensure_vlast(s, D);
+    launch_h(s, D, std::max(0, i0), std::min(s->n[0] + 1, i1));
+}
+
+void cem::dev_mt_E(cem_sim *s, int i0, int i1) {
+    launch_e(s, s->dev, std::max(0, i0), std::min(s->n[0], i1));
+}
+
+void cem::dev_mt_post(cem_sim *s) {
+    Device *D = s->dev;
+    launch_e(s, D, s->n[0], s->n[0] + 1);   // post-phase edges on the last x plane
+    finish_step(s, D);
+}
+
+double cem::dev_energy(cem_sim *s) {
+    Device *D = s->dev;
+    EnergyArgs a{};
+    a.g = D->g;
+    a.E = D->F.p;
+    a.H = D->F.p + 3 * D->g.N;
+    a.cb = D->cb.p;
+    size_t o = 0;
+    for (int c = 0; c < 3; c++) {
+        a.d[c] = D->d1d.p + o;
+        o += s->n[c];
+    }
+    for (int c = 0; c < 3; c++) {
+        a.dd[c] = D->d1d.p + o;
+        o += s->n[c] + 1;
+    }
+    a.dt = s->dt;
+    hipLaunchKernelGGL(k_energy, dim3(D->energy_blocks), dim3(kBlock), 0, D->st, a, D->partial.p);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, D->st, D->partial.p, D->energy_blocks,
+                       D->partial.p + D->energy_blocks);
+    double W = 0;
+    HIP_CHECK_VOID(hipMemcpyAsync(&W, D->partial.p + D->energy_blocks, sizeof(double), hipMemcpyDeviceToHost, D->st));
+    HIP_CHECK_VOID(hipStreamSynchronize(D->st));
+    return W;
+}
+
+void cem::dev_field_download(cem_sim *s, int comp, double *dst) {
+    Device *D = s->dev;
+    if (!D) {
+        std::memset(dst, 0, s->N * sizeof(double));
+        return;
+    }
+    download_f32(D, D->F.p + (size_t)comp * s->N, dst, s->N);
+}
+
+static float *state_ptr(cem_sim *s, int which, int comp, int w, size_t *len) {
+    Device *D = s->dev;
+    switch (which) {
+    case 0: *len = D->psiE_len[comp][w]; return D->psiE[comp][w];
+    case 1: *len = D->psiH_len[comp][w]; return D->psiH[comp][w];
+    case 2: *len = (size_t)s->npole * s->ndisp; return D->J.p;
+    case 5: *len = s->N; return D->F.p + (size_t)comp * s->N;
+    default: *len = 0; return nullptr;
+    }
+}
+
+void cem::dev_state_download(cem_sim *s, int which, int comp, int w, double *dst) {
+    Device *D = s->dev;
+    if (which == 3) {       // Debye E_old: E at the dispersive edges (refreshed every step)
+        std::vector<double> e(s->N);
+        int last = -1;
+        for (int k = 0; k < s->ndisp; k++) {
+            if (s->d_comp[k] != last) {
+                last = s->d_comp[k];
+                download_f32(D, D->F.p + (size_t)last * s->N, e.data(), s->N);
+            }
+            dst[k] = e[s->d_idx[k]];
+        }
+        return;
+    }
+    if (which == 4) {       // SIBC states: device term-major -> reference edge-major
+        const size_t K = s->sibc_npole, n = s->nsb;
+        std::vector<double> t(K * n);
+        download_f32(D, D->Y.p, t.data(), K * n);
+        for (size_t e = 0; e < n; e++)
+            for (size_t k = 0; k < K; k++) dst[e * K + k] = t[k * n + e];
+        return;
+    }
+    size_t len = 0;
+    float *p = state_ptr(s, which, comp, w, &len);
+    if (p) download_f32(D, p, dst, len);
+}
+
+void cem::dev_state_upload(cem_sim *s, int which, int comp, int w, const double *src) {
+    Device *D = s->dev;
+    if (which == 3) return;  // E_old is rewritten from E before every use
+    if (which == 4) {
+        const size_t K = s->sibc_npole, n = s->nsb;
+        std::vector<double> t(K * n);
+        for (size_t e = 0; e < n; e++)
+            for (size_t k = 0; k < K; k++) t[k * n + e] = src[e * K + k];
+        upload_f32(D, D->Y.p, t.data(), K * n);
+        return;
+    }
+    size_t len = 0;
+    float *p = state_ptr(s, which, comp, w, &len);
+    if (p) upload_f32(D, p, src, len);
+    if (which == 5 && comp < 3) D->vlast_stale = true;
+}
+
+void cem::dev_sync_in(cem_sim *s) {
+    Device *D = s->dev;
+    for (int c = 0; c < 6; c++)
+        if (s->field_mirror[c]) dev_state_upload(s, 5, c, 0, s->field_mirror[c].get());
+    if (s->state_exposed) {
+        for (int c = 0; c < 3; c++)
+            for (int w = 0; w < 2; w++) {
+                if (s->psiE_mirror[c][w].size()) dev_state_upload(s, 0, c, w, s->psiE_mirror[c][w].data());
+                if (s->psiH_mirror[c][w].size()) dev_state_upload(s, 1, c, w, s->psiH_mirror[c][w].data());
+            }
+        if (s->dJ_mirror.size()) dev_state_upload(s, 2, 0, 0, s->dJ_mirror.data());
+        if (s->sby_mirror.size()) dev_state_upload(s, 4, 0, 0, s->sby_mirror.data());
+    }
+    (void)D;
+}
+
+void cem::dev_sync_out(cem_sim *s) {
+    for (int c = 0; c < 6; c++)
+        if (s->field_mirror[c]) dev_field_download(s, c, s->field_mirror[c].get());
+    if (s->state_exposed) {
+        for (int c = 0; c < 3; c++)
+            for (int w = 0; w < 2; w++) {
+                if (s->psiE_mirror[c][w].size()) dev_state_download(s, 0, c, w, s->psiE_mirror[c][w].data());
+                if (s->psiH_mirror[c][w].size()) dev_state_download(s, 1, c, w, s->psiH_mirror[c][w].data());
+            }
+        if (s->dJ_mirror.size()) dev_state_download(s, 2, 0, 0, s->dJ_mirror.data());
+        if (s->dEold_mirror.size()) dev_state_download(s, 3, 0, 0, s->dEold_mirror.data());
+        if (s->sby_mirror.size()) dev_state_download(s, 4, 0, 0, s->sby_mirror.data());
+    }
+    if (s->spectra_exposed) dev_spectra_download(s);
+}
+
+void cem::dev_spectra_download(cem_sim *s) {
+    Device *D = s->dev;
+    if (!D) return;   // nothing stepped yet: spectra are zero
+    launch_dft(s, D, 0);
+    const int nf = s->nf;
+    std::vector<double> all((size_t)D->nacc * 2 * nf);
+    HIP_CHECK_VOID(hipMemcpyAsync(all.data(), D->acc.p, all.size() * sizeof(double), hipMemcpyDeviceToHost, D->st));
+    HIP_CHECK_VOID(hipStreamSynchronize(D->st));
+    const size_t np = s->port.size(), nvp = s->vp.size(), nip = s->ip.size();
+    auto blk = [&](size_t a) { return all.data() + a * 2 * nf; };
+    for (size_t p = 0; p < np; p++) {
+        std::memcpy(s->spec_port_v[p].data(), blk(p), 2 * nf * sizeof(double));
+        std::memcpy(s->spec_port_i[p].data(), blk(np + p), 2 * nf * sizeof(double));
+    }
+    for (size_t p = 0; p < nvp; p++) std::memcpy(s->spec_vp[p].data(), blk(2 * np + p), 2 * nf * sizeof(double));
+    for (size_t p = 0; p < nip; p++) std::memcpy(s->spec_ip[p].data(), blk(2 * np + nvp + p), 2 * nf * sizeof(double));
+    std::memcpy(s->spec_src.data(), blk(2 * np + nvp + nip), 2 * nf * sizeof(double));
+}
+
+// host-recorded samples of an external engine (export.c:121-149): the reference
+// recomputes the phasors from cos/sin at every sample, so the DFT runs "direct"
+void cem::dev_push_samples(cem_sim *s, int n, const double *ip, const double *vold, const double *vnew,
+                           const double *vp) {
+    Device *D = s->dev;
+    launch_dft(s, D, 0);
+    const size_t nip = s->ip.size(), np = s->port.size(), nvp = s->vp.size();
+    int done = 0;
+    while (done < n) {
+        const int m = std::min(n - done, kSeriesCap);
+        if (nip) HIP_CHECK_VOID(hipMemcpy(D->ser_ip, ip + (size_t)done * nip, m * nip * sizeof(double), hipMemcpyHostToDevice));
+        if (np) {
+            HIP_CHECK_VOID(hipMemcpy(D->ser_vold, vold + (size_t)done * np, m * np * sizeof(double), hipMemcpyHostToDevice));
+            HIP_CHECK_VOID(hipMemcpy(D->ser_vnew, vnew + (size_t)done * np, m * np * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (nvp) HIP_CHECK_VOID(hipMemcpy(D->ser_vp, vp + (size_t)done * nvp, m * nvp * sizeof(double), hipMemcpyHostToDevice));
+        D->row0 = s->step;
+        D->rows = m;
+        launch_dft(s, D, 1);
+        s->step += m;
+        done += m;
+    }
+    if (s->spectra_exposed) dev_spectra_download(s);
+}
+
+void cem::dev_phasors(cem_sim *s, double *phE, double *phH, int download) {
+    Device *D = s->dev;
+    launch_dft(s, D, 0);
+    const size_t nb = 2 * s->nf * sizeof(double);
+    double *cur = D->ph[D->ph_cur].p;
+    HIP_CHECK_VOID(hipStreamSynchronize(D->st));
+    if (download) {
+        HIP_CHECK_VOID(hipMemcpy(phE, cur, nb, hipMemcpyDeviceToHost));
+        HIP_CHECK_VOID(hipMemcpy(phH, cur + 2 * s->nf, nb, hipMemcpyDeviceToHost));
+    } else {
+        HIP_CHECK_VOID(hipMemcpy(cur, phE, nb, hipMemcpyHostToDevice));
+        HIP_CHECK_VOID(hipMemcpy(cur + 2 * s->nf, phH, nb, hipMemcpyHostToDevice));
+    }
+}
+
+// all DFT accumulators in checkpoint order: port V/I pairs, vprobes, iprobes,
+// source, then phE, phH (export.c:204-210)
+void cem::dev_acc_io(cem_sim *s, double *buf, int download) {
+    Device *D = s->dev;
+    launch_dft(s, D, 0);
+    const int nf = s->nf;
+    const size_t blk = 2 * (size_t)nf;
+    const size_t np = s->port.size(), nvp = s->vp.size(), nip = s->ip.size();
+    std::vector<double> all((size_t)D->nacc * blk);
+    if (download) {
+        HIP_CHECK_VOID(hipMemcpy(all.data(), D->acc.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    // device order: V(np) I(np) vp ip src; checkpoint order: (V, I) per port, vp, ip, src
+    size_t o = 0;
+    auto io = [&](size_t dev_block) {
+        if (download)
+            std::memcpy(buf + o, all.data() + dev_block * blk, blk * sizeof(double));
+        else
+            std::memcpy(all.data() + dev_block * blk, buf + o, blk * sizeof(double));
+        o += blk;
+    };
+    for (size_t p = 0; p < np; p++) {
+        io(p);
+        io(np + p);
+    }
+    for (size_t p = 0; p < nvp; p++) io(2 * np + p);
+    for (size_t p = 0; p < nip; p++) io(2 * np + nvp + p);
+    io(2 * np + nvp + nip);
+    if (!download) HIP_CHECK_VOID(hipMemcpy(D->acc.p, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+    dev_phasors(s, buf + o, buf + o + blk, download);
+}
+
+void cem::dev_set_step(cem_sim *s, long step) {
+    Device *D = s->dev;
+    if (!D) return;
+    launch_dft(s, D, 0);
+    D->row0 = step;
+}
+
+// ---------------------------------------------------------------------------
+// engine extensions (include/courant_amd.h)
+// ---------------------------------------------------------------------------
+extern "C" int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count) {
+    if (!s || !s->dev) return -1;
+    KernelTimer *t[4] = {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD};
+    if (which < 0 || which > 3) return -1;
+    t[which]->fold();
+    *total_ms = t[which]->total_ms;
+    *count = t[which]->count;
+    return 0;
+}
+
+extern "C" int cem_amd_set_profiling(cem_sim *s, int on) {
+    if (!s || !s->finalized || dev_ensure(s)) return -1;
+    s->dev->profiling = on != 0;
+    for (KernelTimer *t : {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD}) {
+        t->fold();
+        t->total_ms = 0;
+        t->count = 0;
+    }
+    return 0;
+}
+
+extern "C" int cem_amd_synchronize(cem_sim *s) {
+    if (!s || !s->dev) return -1;
+    return hipStreamSynchronize(s->dev->st) == hipSuccess ? 0 : -1;
+}
+
+// Advance n steps without touching host mirrors or spectra (timed loops).
+extern "C" int cem_amd_step_device(cem_sim *s, long n) {
+    if (!s || !s->finalized || dev_ensure(s)) return -1;
+    dev_step(s, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -1,0 +1,34 @@
+/*
+ * courant_amd.h - MI355X engine extensions of libcourant.so.
+ *
+ * These entry points have no counterpart in the reference API; they serve
+ * benchmarking and serving code that drives the GPU engine directly.
+ */
+#ifndef COURANT_AMD_H
+#define COURANT_AMD_H
+
+#include "courant.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Advance n steps on the GPU without refreshing host mirrors of fields or
+ * spectra (timed loops).  Returns 0, or -1 with cem_last_error() set. */
+CEM_API int cem_amd_step_device(cem_sim *s, long n);
+
+/* Wait until every queued kernel of this simulation has finished. */
+CEM_API int cem_amd_synchronize(cem_sim *s);
+
+/* Record HIP events around every kernel launch (on the engine's own stream)
+ * and reset the statistics. */
+CEM_API int cem_amd_set_profiling(cem_sim *s, int on);
+
+/* Total GPU time (ms) and launch count of one kernel since profiling was
+ * switched on: which = 0 H update, 1 E update, 2 probe gather, 3 DFT. */
+CEM_API int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,221 @@
+"""GPU parity: the FP32 HIP engine against the reference f64 core on the same scenes.
+
+Tolerances (stated per test):
+  * fields after N steps: max |dF| <= 1e-4 * max |F|   (web/test/gpu.test.mjs:25)
+  * DFT spectra: relative error <= 1e-4 at the scene's frequencies
+  * S-parameters: ||S21| in dB| <= 1e-4 dB, angle <= 1e-3 deg; |dS11| <= 1e-4
+  * engine-internal properties (checkpoint resume, slab-split phase API,
+    host-written fields) are bit-exact
+"""
+import numpy as np
+import pytest
+
+from courant_amd import lib as L
+from courant_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-4
+SPEC_TOL = 1e-4
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def fields(sim):
+    return [np.array(sim.field(c)) for c in range(6)]
+
+
+def print_case(name):
+    print(f"\n[gpu case] {name}", flush=True)
+
+
+def test_parity_scene_matches_reference(cem, ref, gpu_available):
+    """solver/apps/parity.c after 1500 steps: V1, V2, Vp, Ip spectra, energy and fields."""
+    print_case("parity scene 1500 steps")
+    a, ia = S.parity_scene(ref, steps=1500)
+    b, ib = S.parity_scene(cem, steps=1500)
+    for get in (lambda s, i: s.port_v(i["ports"][0]), lambda s, i: s.port_v(i["ports"][1]),
+                lambda s, i: s.vprobe(i["vprobe"]), lambda s, i: s.iprobe(i["iprobe"]),
+                lambda s, i: s.port_i(i["ports"][0]), lambda s, i: s.source_spectrum()):
+        assert rel(get(b, ib), get(a, ia)) < SPEC_TOL
+    assert abs(b.energy() - a.energy()) / a.energy() < 1e-4
+    for fa, fb in zip(fields(a), fields(b)):
+        assert np.max(np.abs(fa - fb)) <= FIELD_TOL * max(np.max(np.abs(fa)), 1e-30)
+    assert b.steps == 1500
+
+
+def test_dispersive_sibc_scene_matches_reference(cem, ref, gpu_available):
+    """solver/tests/test_checkpoint.c scene (CPML, 3 Debye poles, copper SIBC) after 900 steps."""
+    print_case("Debye + SIBC scene 900 steps")
+    a = S.checkpoint_scene(ref)
+    b = S.checkpoint_scene(cem)
+    a.step(900)
+    b.step(900)
+    for p in (0, 1):
+        assert rel(b.port_v(p), a.port_v(p)) < SPEC_TOL
+        assert rel(b.port_i(p), a.port_i(p)) < SPEC_TOL
+    assert rel(b.vprobe(0), a.vprobe(0)) < SPEC_TOL
+    assert rel(b.iprobe(0), a.iprobe(0)) < SPEC_TOL
+    for fa, fb in zip(fields(a), fields(b)):
+        assert np.max(np.abs(fa - fb)) <= FIELD_TOL * max(np.max(np.abs(fa)), 1e-30)
+    # Debye currents and SIBC states track the reference as well
+    for which in (2, 4):
+        na = ref.lib.cem_state_len(a.h, which, 0, 0)
+        sa = np.ctypeslib.as_array(ref.lib.cem_state_array(a.h, which, 0, 0), shape=(na,))
+        sb = np.ctypeslib.as_array(cem.lib.cem_state_array(b.h, which, 0, 0), shape=(na,))
+        assert np.max(np.abs(sa - sb)) <= 1e-3 * np.max(np.abs(sa))
+
+
+GCPW_MINI = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=3e-3, yair=0.4e-3, zair=0.6e-3, zbelow=0.2e-3,
+                 pml_below=True, df=0.5e9)
+
+
+def db_deg_error(sa, sb):
+    ddb = np.abs(20 * np.log10(np.abs(sb)) - 20 * np.log10(np.abs(sa)))
+    ddeg = np.abs(np.angle(sb / sa, deg=True))
+    return ddb, ddeg
+
+
+def test_gcpw_lumped_sparameters(cem, ref, gpu_available):
+    """Lumped-port GCPW (CPML on all faces): S21 within 1e-4 dB / 1e-3 deg of the f64 core."""
+    print_case("gcpw lumped 4000 steps")
+    a, ia = S.gcpw(ref, **GCPW_MINI)
+    b, ib = S.gcpw(cem, **GCPW_MINI)
+    a.step(4000)
+    b.step(4000)
+    s11a, s21a = S.gcpw_sparams(a, ia)
+    s11b, s21b = S.gcpw_sparams(b, ib)
+    band = (ia["freqs"] > 0) & (ia["freqs"] <= 40e9)
+    ddb, ddeg = db_deg_error(s21a[band], s21b[band])
+    print("max |S21| err %.2e dB, %.2e deg; |dS11| %.2e" % (ddb.max(), ddeg.max(), np.abs(s11a - s11b)[band].max()))
+    assert ddb.max() <= 1e-4
+    assert ddeg.max() <= 1e-3
+    assert np.abs(s11a - s11b)[band].max() <= 1e-4
+
+
+MODE_MINI = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=1.5e-3, feed=1.2e-3, mon=0.6e-3, yair=0.3e-3,
+                 zair=0.4e-3, ports="mode", lossy=True, tcu=35e-6, mask=25e-6, dc=False, df=4e9)
+
+
+def test_lossy_mode_port_waves(cem, ref, gpu_available):
+    """Mode ports + Debye substrate + Huray SIBC copper: wave amplitudes and S21."""
+    print_case("lossy mode-port gcpw 3000 steps")
+    a, ia = S.gcpw(ref, **MODE_MINI)
+    b, ib = S.gcpw(cem, **MODE_MINI)
+    a.step(3000)
+    b.step(3000)
+    s11a, s21a = S.gcpw_sparams(a, ia)
+    s11b, s21b = S.gcpw_sparams(b, ib)
+    ddb, ddeg = db_deg_error(s21a, s21b)
+    print("max |S21| err %.2e dB, %.2e deg; |dS11| %.2e" % (ddb.max(), ddeg.max(), np.abs(s11a - s11b).max()))
+    assert ddb.max() <= 1e-4
+    assert ddeg.max() <= 1e-3
+    assert np.abs(s11a - s11b).max() <= 1e-4
+
+
+def test_checkpoint_resume_is_bit_exact(cem, gpu_available):
+    """900 straight steps == 400 + save + load into a fresh sim + 500 (test_checkpoint.c:245-264)."""
+    print_case("checkpoint resume")
+    A, B, Cs = S.checkpoint_scene(cem), S.checkpoint_scene(cem), S.checkpoint_scene(cem)
+    A.step(900)
+    B.step(400)
+    buf = B.checkpoint()
+    Cs.restore(buf)
+    Cs.step(500)
+    for fa, fc in zip(fields(A), fields(Cs)):
+        assert np.array_equal(fa, fc)
+    assert np.array_equal(A.port_v(1), Cs.port_v(1))
+    assert np.array_equal(A.vprobe(0), Cs.vprobe(0))
+    assert np.array_equal(A.iprobe(0), Cs.iprobe(0))
+
+
+def test_checkpoint_from_reference_resumes(cem, ref, gpu_available):
+    """A checkpoint written by the f64 core resumes on the GPU engine (and vice versa)."""
+    print_case("checkpoint cross-engine")
+    a = S.checkpoint_scene(ref)
+    a.step(400)
+    buf = a.checkpoint()
+    b = S.checkpoint_scene(cem)
+    b.restore(buf)
+    a.step(300)
+    b.step(300)
+    assert rel(b.port_v(1), a.port_v(1)) < SPEC_TOL
+    back = S.checkpoint_scene(ref)
+    back.restore(b.checkpoint())
+    back.step(200)
+    a.step(200)
+    assert rel(back.port_v(1), a.port_v(1)) < SPEC_TOL
+
+
+def test_phase_api_matches_step(cem, gpu_available):
+    """cem_mt_pre/H/mid/E/post over arbitrary slab splits equals cem_step, bit for bit."""
+    print_case("phase api")
+    a = S.checkpoint_scene(cem)
+    b = S.checkpoint_scene(cem)
+    a.step(300)
+    nx = 24
+    for n in range(300):
+        cuts = [0, 7, 13, nx + 1] if n % 2 else [0, 11, nx + 1]
+        b.lib.cem_mt_pre(b.h)
+        for i0, i1 in zip(cuts[:-1], cuts[1:]):
+            b.lib.cem_mt_H(b.h, i0, i1)
+        b.lib.cem_mt_mid(b.h)
+        ecuts = [0, 5, 17, nx]
+        for i0, i1 in zip(ecuts[:-1], ecuts[1:]):
+            b.lib.cem_mt_E(b.h, i0, i1)
+        b.lib.cem_mt_post(b.h)
+    for fa, fb in zip(fields(a), fields(b)):
+        assert np.array_equal(fa, fb)
+    assert np.array_equal(a.port_v(0), b.port_v(0))
+
+
+def test_field_written_by_host_then_stepped(cem, ref, gpu_available):
+    """Write-then-step and step-then-read through a held cem_field pointer (test_cavity.c:62-71,
+    test_cpml.c:31-36): a PEC cavity started in TE101 evolves as on the f64 core."""
+    print_case("cavity write-then-step")
+    out = []
+    for lib in (ref, cem):
+        a, b, d = 10e-3, 5e-3, 8e-3
+        nx, ny, nz = 20, 10, 16
+        x = np.linspace(0, a, nx + 1)
+        y = np.linspace(0, b, ny + 1)
+        z = np.linspace(0, d, nz + 1)
+        s = lib.create_sim(x, y, z)
+        s.set_frequencies([15e9])
+        s.add_vprobe(L.Y, 7, 5, 3, 4, 1)
+        s.finalize(0.99)
+        Ey = s.field(L.Y)
+        Ey[1:nx, 0:ny, 1:nz] = (np.sin(np.pi * x[1:nx] / a)[:, None, None] * np.sin(np.pi * z[1:nz] / d)[None, None, :])
+        Ez = s.field(L.Z)
+        trace = []
+        for _ in range(200):
+            s.step(1)
+            trace.append(Ey[7, 5, 3])
+        out.append((np.array(trace), s.energy(), s.vprobe(0)))
+    (ta, wa, va), (tb, wb, vb) = out
+    assert np.max(np.abs(ta - tb)) <= FIELD_TOL * np.max(np.abs(ta))
+    assert abs(wa - wb) / wa < 1e-5
+    assert rel(vb, va) < SPEC_TOL
+
+
+def test_push_samples_matches_reference(cem, ref, gpu_available):
+    """External-engine samples (export.c:121-149) accumulate into the same spectra."""
+    print_case("push samples")
+    rng = np.random.default_rng(3)
+    res = []
+    for lib in (ref, cem):
+        s, info = S.parity_scene(lib)
+        n = 700
+        ip = rng.standard_normal(n * 1)
+        vold = rng.standard_normal(n * 2)
+        vnew = rng.standard_normal(n * 2)
+        vp = rng.standard_normal(n * 1)
+        rng = np.random.default_rng(3)
+        lib.lib.cem_push_samples(s.h, n, ip.ctypes.data_as(L._DP), vold.ctypes.data_as(L._DP),
+                                 vnew.ctypes.data_as(L._DP), vp.ctypes.data_as(L._DP))
+        res.append([s.port_v(0), s.port_i(1), s.vprobe(0), s.iprobe(0), s.source_spectrum(), s.steps])
+    for x, y in zip(res[0][:-1], res[1][:-1]):
+        assert rel(y, x) < 1e-12
+    assert res[0][-1] == res[1][-1] == 700

@@ -1,0 +1,164 @@
+"""CPU parity of the host set-up against the compiled reference core.
+
+Integer work (edge coefficients' PEC mask, CPML layout, port / probe / Debye /
+SIBC / source index lists) must match bit for bit; set-up coefficients are
+computed with the reference's operation order and must match bit for bit too.
+The only FP64 quantity allowed to differ is the mode-source profile (an
+eigenvector from inverse iteration), to 1e-10 of its maximum.
+"""
+import numpy as np
+import pytest
+
+import ctypes
+
+from courant_amd import lib as L
+from courant_amd import scenes as S
+
+SMALL_LUMPED = dict(hmin=40e-6, hmax=100e-6, dx=200e-6, length=2e-3, yair=0.3e-3, zair=0.4e-3, zbelow=0.2e-3,
+                    pml_below=True, df=1e9)
+SMALL_MODE = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=1e-3, feed=1.2e-3, mon=0.6e-3, yair=0.3e-3,
+                  zair=0.4e-3, ports="mode", lossy=True, tcu=35e-6, mask=25e-6, dc=False, df=2e9)
+
+SCENES = {
+    "parity": lambda c: S.parity_scene(c)[0],
+    "checkpoint": S.checkpoint_scene,
+    "gcpw_lumped": lambda c: S.gcpw(c, **SMALL_LUMPED)[0],
+    "gcpw_mode_lossy": lambda c: S.gcpw(c, **SMALL_MODE)[0],
+}
+
+
+def assert_plan_equal(a, b, float_tol_keys=()):
+    for k in a:
+        va, vb = a[k], b[k]
+        if isinstance(va, list) and va and isinstance(va[0], np.ndarray):
+            for x, y in zip(va, vb):
+                np.testing.assert_array_equal(x, y, err_msg=k)
+        elif isinstance(va, list) and va and isinstance(va[0], dict):
+            assert len(va) == len(vb), k
+            for x, y in zip(va, vb):
+                for kk in x:
+                    np.testing.assert_array_equal(x[kk], y[kk], err_msg=f"{k}.{kk}")
+        elif isinstance(va, dict):
+            assert va.keys() == vb.keys(), k
+            for kk in va:
+                if (k, kk) in float_tol_keys:
+                    xa, xb = np.asarray(va[kk]), np.asarray(vb[kk])
+                    assert xa.shape == xb.shape
+                    assert np.max(np.abs(xa - xb), initial=0) <= 1e-10 * max(1.0, np.max(np.abs(xa), initial=0))
+                else:
+                    np.testing.assert_array_equal(np.asarray(va[kk]), np.asarray(vb[kk]), err_msg=f"{k}.{kk}")
+        else:
+            assert va == vb, (k, va, vb)
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_plan_matches_reference(name, cem, ref):
+    a = SCENES[name](ref).plan()
+    b = SCENES[name](cem).plan()
+    assert_plan_equal(a, b, float_tol_keys={("mode_source", "w")})
+
+
+def test_lossy_scene_has_every_sparse_kind(cem):
+    p = SCENES["gcpw_mode_lossy"](cem).plan()
+    assert p["ndisp"] > 0 and p["nsibc"] > 0 and len(p["mode_source"]["idx"]) > 0 and p["has_pml"]
+
+
+def test_waveforms_match_reference(cem, ref):
+    a = S.parity_scene(ref)[0].waveforms(0, 2000)
+    b = S.parity_scene(cem)[0].waveforms(0, 2000)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_djordjevic_sarkar_fit_matches_reference(cem, ref):
+    for args in [(4.1, 0.02, 1e9, 0.1e9, 40e9, 6), (3.66, 0.0037, 10e9, 0.5e9, 40e9, 8), (4.3, 0.01, 1e9, 1e9, 20e9, 1)]:
+        a, b = ref.fit_djordjevic_sarkar(*args), cem.fit_djordjevic_sarkar(*args)
+        assert a["eps_inf"] == b["eps_inf"] and a["tan_delta_error"] == b["tan_delta_error"]
+        np.testing.assert_array_equal(a["fp"], b["fp"])
+        np.testing.assert_array_equal(a["deps"], b["deps"])
+
+
+@pytest.mark.parametrize("rough,p1,p2", [(0, 0, 0), (1, 1e-6, 0), (2, 0.5e-6, 6.366e11)])
+def test_conductor_fit_matches_reference(cem, ref, rough, p1, p2):
+    out = []
+    for lib in (ref, cem):
+        g = np.linspace(0, 1e-3, 11)
+        s = lib.create_sim(g, g, g)
+        s.set_conductor_loss(5.8e7, rough, p1, p2, 0.1e9, 40e9)
+        vals = []
+        for f in (1e9, 10e9, 35e9):
+            r, i = ctypes.c_double(), ctypes.c_double()
+            lib.lib.cem_conductor_zs(s.h, f, ctypes.byref(r), ctypes.byref(i))
+            vals += [r.value, i.value]
+        out.append((s.conductor_fit_error(), vals))
+    assert out[0] == out[1]
+
+
+def test_mode2d_wr28_closed_form(cem):
+    """WR-28 TE10 at 35 GHz: beta equals the Yee discrete dispersion to 1e-8 and the
+    perturbational copper alpha_c the closed form to 1% (solver/tests/test_mode2d.c)."""
+    import math
+    a, b, NY, NZ = 7.112e-3, 3.556e-3, 48, 24
+    y = np.array([a * j / NY for j in range(NY + 1)])
+    z = np.array([b * k / NZ for k in range(NZ + 1)])
+    lib = cem.lib
+    m = lib.cem_mode2d_create(NY, NZ, y.ctypes.data_as(L._DP), z.ctypes.data_as(L._DP))
+    f = 35e9
+    w = 2 * math.pi * f
+    k0 = w / L.C0
+    dy = a / NY
+    kc_d = 2 / dy * math.sin(math.pi / a * dy / 2)
+    beta_d = math.sqrt(k0 * k0 - kc_d * kc_d)
+    beta = math.sqrt(k0 * k0 - (math.pi / a) ** 2)
+    rc = lib.cem_mode2d_solve(m, f, beta / k0, None, 300, 1e-13)
+    Rs = math.sqrt(w * L.MU0 / (2 * 5.8e7))
+    eta = 376.730313668
+    ac = lib.cem_mode2d_conductor_alpha(m, Rs)
+    an = Rs / (a ** 3 * b * beta * k0 * eta) * (2 * b * math.pi ** 2 + a ** 3 * k0 * k0)
+    assert rc == 0
+    assert abs(lib.cem_mode2d_beta(m) - beta_d) / beta_d < 1e-8
+    assert abs(ac - an) / an < 0.01
+    lib.cem_mode2d_destroy(m)
+
+
+def test_mode2d_matches_reference(cem, ref):
+    a, b, NY, NZ = 7.112e-3, 3.556e-3, 24, 12
+    y = np.array([a * j / NY for j in range(NY + 1)])
+    z = np.array([b * k / NZ for k in range(NZ + 1)])
+    out = []
+    for lib in (ref, cem):
+        m = lib.lib.cem_mode2d_create(NY, NZ, y.ctypes.data_as(L._DP), z.ctypes.data_as(L._DP))
+        rc = lib.lib.cem_mode2d_solve(m, 35e9, 0.7, None, 300, 1e-13)
+        eu = np.concatenate([np.ctypeslib.as_array(lib.lib.cem_mode2d_eu(m), shape=(2 * NY * (NZ + 1),)),
+                             np.ctypeslib.as_array(lib.lib.cem_mode2d_ev(m), shape=(2 * (NY + 1) * NZ,))])
+        V, I = np.zeros(2), np.zeros(2)
+        lib.lib.cem_mode2d_vi(m, NY // 2, 0, NZ, 2, NY - 3, 1, NZ - 2, V.ctypes.data_as(L._DP), I.ctypes.data_as(L._DP))
+        out.append((rc, lib.lib.cem_mode2d_beta(m), lib.lib.cem_mode2d_power(m), eu, V, I))
+        lib.lib.cem_mode2d_destroy(m)
+    (r0, b0, p0, e0, v0, i0), (r1, b1, p1, e1, v1, i1) = out
+    assert r0 == r1 == 0
+    assert abs(b0 - b1) <= 1e-12 * abs(b0)
+    assert abs(p0 - p1) <= 1e-9 * abs(p0)
+    assert np.max(np.abs(e0 - e1)) <= 1e-10 * np.max(np.abs(e0))
+    assert np.allclose(v0, v1, rtol=1e-9, atol=1e-12 * np.max(np.abs(v0)))
+
+
+def test_renormalize_matches_reference_and_identity(cem, ref):
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        S4 = list(rng.standard_normal(4) * 0.3 + 1j * rng.standard_normal(4) * 0.3)
+        z1, z2 = rng.uniform(20, 90, 2)
+        a = ref.renormalize2(S4, z1, z2, 50.0)
+        b = cem.renormalize2(S4, z1, z2, 50.0)
+        assert np.allclose(a, b, rtol=1e-14, atol=1e-15)
+        # renormalising to the ports' own impedance is the identity
+        c = cem.renormalize2(S4, z1, z1, z1)
+        assert np.allclose(c, S4, atol=1e-12)
+
+
+def test_cfg_scene_sizes():
+    for cfg, cells in ((S.CFG1, (120, 80, 60)), (S.CFG2, (300, 152, 99))):
+        p = dict(S.GCPW_DEFAULTS)
+        p.update(cfg)
+        X, Y, Z, _ = S.gcpw_mesh(p)
+        assert (len(X) - 1, len(Y) - 1, len(Z) - 1) == cells
