@@ -77,7 +77,14 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-only", type=float, default=0,
+                    help="only run the stream-copy kernel on this many GB (byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_only:
+        from courant_amd import lib
+        gbs = lib.Courant().lib.cem_amd_copy_bandwidth(int(args.copy_only * 1e9), 5)
+        print(json.dumps({"copy_gb": args.copy_only, "copy_gbs": round(gbs, 1), "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this engine runs one board per GPU and --gpus {args.gpus} is not "
               "supported (multi-GPU slab sharding is design-only)", file=sys.stderr)
@@ -100,6 +107,7 @@ def main():
     stats = {k: sim.kernel_stats(i) for i, k in enumerate(("H", "E", "gather", "dft"))}
     sim.set_profiling(False)
     ab = algorithmic_bytes(sim)
+    copy_gbs = cem.lib.cem_amd_copy_bandwidth(2 << 30, 10)
     value = cells * args.steps / wall / 1e6
     dom = max(("H", "E"), key=lambda k: stats[k][0])
     avg_ms = stats[dom][0] / max(stats[dom][1], 1)
@@ -126,7 +134,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": f"k_update_{dom.lower()}", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 5),
-                     "step_frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "step_frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+                     "measured_copy_gbs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs > 0 else None},
         "kernels_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in stats.items()},
     }
     sim.close()

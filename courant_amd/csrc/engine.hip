@@ -96,6 +96,16 @@ struct HArgs {
     int has_pml;
 };
 
+// Per-edge physics parameters, resident on the device so the E kernel's
+// argument block stays small.
+struct PostParams {
+    int ndisp, npole, nsb, nterm;
+    float kc[kMaxPoles];    // (1 + k)/2
+    float km1[kMaxPoles];   // k - 1 (computed without cancellation)
+    float zc[kSibcTerms];   // c_k
+    float om[kSibcTerms];   // 1 - e_k, from -expm1(-a_k dt)
+};
+
 struct EArgs {
     Grid g;
     float *E;               // 3N
@@ -105,22 +115,14 @@ struct EArgs {
     Pml pml;
     float *psi[3][2];       // psiE[c][w]
     int has_pml;
-    // Debye
-    Sparse dtab[3];
-    int ndisp, npole;
+    // sparse edge sets: kind 0 Debye, 1 SIBC, 2 sources; rows (comp, i, chunk)
+    const unsigned short *flags;   // per (i, chunk): bit 3*comp + kind
+    Sparse tab[3];
+    const PostParams *pp;
     float *J;               // npole * ndisp (pole-major)
     const float *bdt;       // beta/dt, npole * ndisp
-    float kc[kMaxPoles];    // (1 + k)/2
-    float km1[kMaxPoles];   // k - 1 (computed without cancellation)
-    // SIBC
-    Sparse stab[3];
-    int nsb, nterm;
     float *Y;               // nterm * nsb (term-major)
     const unsigned char *face;
-    float zc[kSibcTerms];   // c_k
-    float om[kSibcTerms];   // 1 - e_k, from -expm1(-a_k dt)
-    // sources
-    Sparse xtab[3];
     const float *wl, *wm;   // lumped cs and modal weight per source slot
     float vs, gm;           // waveform values at tH of this step
 };
@@ -160,49 +162,58 @@ __device__ __forceinline__ int wave_chunk(int p) {
     return __builtin_amdgcn_readfirstlane(p >> 6);
 }
 
-__device__ __forceinline__ bool sparse_slot(const Sparse &t, long long row, int chunk, int lane, unsigned *slot) {
-    unsigned long long m = t.mask[row + chunk];
+__device__ __forceinline__ bool sparse_slot(const Sparse &t, long long row, int lane, unsigned *slot) {
+    const unsigned long long m = t.mask[row];
     if (!((m >> lane) & 1ull)) return false;
-    *slot = t.base[row + chunk] + (unsigned)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+    *slot = t.base[row] + (unsigned)__builtin_popcountll(m & ((1ull << lane) - 1ull));
     return true;
 }
 
-__global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1) {
-    const Grid &g = a.g;
+// One thread per (j,k) column of an i-segment; the E values of plane i+1 are
+// carried in registers into the next iteration.
+__global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, int iseg) {
+    const Grid g = a.g;
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= g.S) return;
     const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
-    const int ib = i0 + blockIdx.y * kIseg;
-    const int ie = min(i1, ib + kIseg);
-    const float *Ex = a.E, *Ey = a.E + g.N, *Ez = a.E + 2 * g.N;
-    float *Hx = a.H, *Hy = a.H + g.N, *Hz = a.H + 2 * g.N;
+    const int ib = i0 + blockIdx.y * iseg;
+    const int ie = min(i1, ib + iseg);
+    const float *__restrict__ Ex = a.E;
+    const float *__restrict__ Ey = a.E + g.N;
+    const float *__restrict__ Ez = a.E + 2 * g.N;
+    float *__restrict__ Hx = a.H;
+    float *__restrict__ Hy = a.H + g.N;
+    float *__restrict__ Hz = a.H + 2 * g.N;
     const bool jin = j < g.ny, kin = k < g.nz;
     const float hy = jin ? a.hs[1][j] : 0.f, hz = kin ? a.hs[2][k] : 0.f;
     const int sy = a.has_pml && jin ? a.pml.slot[1][j] : -1;
     const int sz = a.has_pml && kin ? a.pml.slot[2][k] : -1;
-    for (int i = ib; i < ie; i++) {
-        const long long q = (long long)i * g.si + p;
+    const float by = sy >= 0 ? a.pml.b[1][j] : 0.f, cy = sy >= 0 ? a.pml.c[1][j] : 0.f;
+    const float bz = sz >= 0 ? a.pml.b[2][k] : 0.f, cz = sz >= 0 ? a.pml.c[2][k] : 0.f;
+    long long q = (long long)ib * g.si + p;
+    float ey = Ey[q], ez = Ez[q];
+    for (int i = ib; i < ie; i++, q += g.si) {
         const bool iin = i < g.nx;
-        const float ex = Ex[q], ey = Ey[q], ez = Ez[q];
-        const float dz_ey = kin ? Ey[q + 1] - ey : 0.f;
-        const float dy_ez = jin ? Ez[q + g.sj] - ez : 0.f;
-        const float dz_ex = kin ? Ex[q + 1] - ex : 0.f;
-        const float dy_ex = jin ? Ex[q + g.sj] - ex : 0.f;
-        const float dx_ez = iin ? Ez[q + g.si] - ez : 0.f;
-        const float dx_ey = iin ? Ey[q + g.si] - ey : 0.f;
+        const float ex = Ex[q];
+        const float ey_k = kin ? Ey[q + 1] : 0.f, ex_k = kin ? Ex[q + 1] : 0.f;
+        const float ez_j = jin ? Ez[q + g.sj] : 0.f, ex_j = jin ? Ex[q + g.sj] : 0.f;
+        const float ey_i = iin ? Ey[q + g.si] : 0.f, ez_i = iin ? Ez[q + g.si] : 0.f;
+        const float dz_ey = ey_k - ey, dy_ez = ez_j - ez;
+        const float dz_ex = ex_k - ex, dy_ex = ex_j - ex;
+        const float dx_ez = ez_i - ez, dx_ey = ey_i - ey;
         const float hx = iin ? a.hs[0][i] : 0.f;
         const int sx = a.has_pml && iin ? a.pml.slot[0][i] : -1;
         if (jin && kin) {   // Hx: all i
             float h = Hx[q] - (dy_ez * hy - dz_ey * hz);
             if (sy >= 0) {
                 float *ps = a.psi[0][0] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
-                float v = a.pml.b[1][j] * *ps + a.pml.c[1][j] * dy_ez;
+                const float v = by * *ps + cy * dy_ez;
                 *ps = v;
                 h -= a.dth * v;
             }
             if (sz >= 0) {
                 float *ps = a.psi[0][1] + sz * a.pml.per[2] + (long long)i * (g.ny + 1) + j;
-                float v = a.pml.b[2][k] * *ps + a.pml.c[2][k] * dz_ey;
+                const float v = bz * *ps + cz * dz_ey;
                 *ps = v;
                 h += a.dth * v;
             }
@@ -212,13 +223,13 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1) {
             float h = Hy[q] - (dz_ex * hz - dx_ez * hx);
             if (sx >= 0) {
                 float *ps = a.psi[1][1] + sx * a.pml.per[0] + p;
-                float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_ez;
+                const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_ez;
                 *ps = v;
                 h += a.dth * v;
             }
             if (sz >= 0) {
                 float *ps = a.psi[1][0] + sz * a.pml.per[2] + (long long)i * (g.ny + 1) + j;
-                float v = a.pml.b[2][k] * *ps + a.pml.c[2][k] * dz_ex;
+                const float v = bz * *ps + cz * dz_ex;
                 *ps = v;
                 h -= a.dth * v;
             }
@@ -228,35 +239,39 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1) {
             float h = Hz[q] - (dx_ey * hx - dy_ex * hy);
             if (sx >= 0) {
                 float *ps = a.psi[2][0] + sx * a.pml.per[0] + p;
-                float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_ey;
+                const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_ey;
                 *ps = v;
                 h -= a.dth * v;
             }
             if (sy >= 0) {
                 float *ps = a.psi[2][1] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
-                float v = a.pml.b[1][j] * *ps + a.pml.c[1][j] * dy_ex;
+                const float v = by * *ps + cy * dy_ex;
                 *ps = v;
                 h += a.dth * v;
             }
             Hz[q] = h;
         }
+        ey = ey_i;
+        ez = ez_i;
     }
 }
 
-// Post-phase physics of one E edge (comp c, slot tables row r): Debye correction,
-// SIBC overwrite, sources, Debye current update, in the reference order.
-__device__ __forceinline__ float e_post(const EArgs &a, int c, long long row, int chunk, int lane, long long q,
-                                        float e, float e_old, float cb) {
+// Post-phase physics of one E edge of comp c (kinds: bit0 Debye, bit1 SIBC,
+// bit2 source), in the reference order: Debye correction, SIBC overwrite,
+// sources, Debye current update.
+__device__ __forceinline__ float e_post(const EArgs &a, int c, unsigned kinds, long long row, int lane, long long q,
+                                     float e, float e_old, float cb) {
     const Grid &g = a.g;
+    const PostParams &pp = *a.pp;
     unsigned dslot = 0;
-    const bool deb = a.ndisp && sparse_slot(a.dtab[c], row, chunk, lane, &dslot);
+    const bool deb = (kinds & 1u) && sparse_slot(a.tab[0], row, lane, &dslot);
     if (deb) {
         float acc = 0.f;
-        for (int p = 0; p < a.npole; p++) acc += a.kc[p] * a.J[(long long)p * a.ndisp + dslot];
+        for (int p = 0; p < pp.npole; p++) acc += pp.kc[p] * a.J[(long long)p * pp.ndisp + dslot];
         e -= cb * acc;
     }
     unsigned sslot = 0;
-    if (a.nsb && sparse_slot(a.stab[c], row, chunk, lane, &sslot)) {
+    if ((kinds & 2u) && sparse_slot(a.tab[1], row, lane, &sslot)) {
         const int u = (c + 1) % 3, v = (c + 2) % 3;
         const long long su = u == 0 ? g.si : u == 1 ? g.sj : 1;
         const long long sv = v == 0 ? g.si : v == 1 ? g.sj : 1;
@@ -269,119 +284,133 @@ __device__ __forceinline__ float e_post(const EArgs &a, int c, long long row, in
         if (m & 4u) Js += inv * Hv[q];
         if (m & 8u) Js += -inv * Hv[q - su];
         float es = 0.f;
-        for (int t = 0; t < a.nterm; t++) {
-            float *y = a.Y + (long long)t * a.nsb + sslot;
+        for (int t = 0; t < pp.nterm; t++) {
+            float *y = a.Y + (long long)t * pp.nsb + sslot;
             float yv = *y;
-            yv += a.om[t] * (Js - yv);
+            yv += pp.om[t] * (Js - yv);
             *y = yv;
-            es += a.zc[t] * (Js - yv);
+            es += pp.zc[t] * (Js - yv);
         }
         e = es;
     }
     unsigned xslot = 0;
-    if (sparse_slot(a.xtab[c], row, chunk, lane, &xslot)) e += a.wl[xslot] * a.vs + a.wm[xslot] * a.gm;
+    if ((kinds & 4u) && sparse_slot(a.tab[2], row, lane, &xslot)) e += a.wl[xslot] * a.vs + a.wm[xslot] * a.gm;
     if (deb) {
         const float de = e - e_old;
-        for (int p = 0; p < a.npole; p++) {
-            float *J = a.J + (long long)p * a.ndisp + dslot;
-            float jv = *J;
-            *J = jv + a.km1[p] * jv + a.bdt[(long long)p * a.ndisp + dslot] * de;
+        for (int p = 0; p < pp.npole; p++) {
+            float *J = a.J + (long long)p * pp.ndisp + dslot;
+            const float jv = *J;
+            *J = jv + pp.km1[p] * jv + a.bdt[(long long)p * pp.ndisp + dslot] * de;
         }
     }
     return e;
 }
 
-__global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1) {
-    const Grid &g = a.g;
+// One thread per (j,k) column of an i-segment; the H values of plane i-1 are
+// carried in registers from the previous iteration.
+__global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, int iseg) {
+    const Grid g = a.g;
     const int p = blockIdx.x * kBlock + threadIdx.x;
     const int chunk = wave_chunk(blockIdx.x * kBlock + threadIdx.x);
     const int lane = threadIdx.x & 63;
     if (p >= g.S) return;
     const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
-    const int ib = i0 + blockIdx.y * kIseg;
-    const int ie = min(i1, ib + kIseg);
-    float *Ex = a.E, *Ey = a.E + g.N, *Ez = a.E + 2 * g.N;
-    const float *Hx = a.H, *Hy = a.H + g.N, *Hz = a.H + 2 * g.N;
+    const int ib = i0 + blockIdx.y * iseg;
+    const int ie = min(i1, ib + iseg);
+    float *__restrict__ Ex = a.E;
+    float *__restrict__ Ey = a.E + g.N;
+    float *__restrict__ Ez = a.E + 2 * g.N;
+    const float *__restrict__ Hx = a.H;
+    const float *__restrict__ Hy = a.H + g.N;
+    const float *__restrict__ Hz = a.H + 2 * g.N;
+    const float *__restrict__ ca = a.ca;
+    const float *__restrict__ cb = a.cb;
     const bool jb = j >= 1 && j < g.ny, kb = k >= 1 && k < g.nz;   // interior node ranges
-    const float ey_s = j <= g.ny ? a.es[1][j] : 0.f, ez_s = k <= g.nz ? a.es[2][k] : 0.f;
+    const float ey_s = a.es[1][j], ez_s = a.es[2][k];
     const int sy = a.has_pml ? a.pml.slot[1][j] : -1;
     const int sz = a.has_pml ? a.pml.slot[2][k] : -1;
+    const float by = sy >= 0 ? a.pml.b[1][j] : 0.f, cy = sy >= 0 ? a.pml.c[1][j] : 0.f;
+    const float bz = sz >= 0 ? a.pml.b[2][k] : 0.f, cz = sz >= 0 ? a.pml.c[2][k] : 0.f;
     const long long plane = (long long)(g.nx + 1) * g.nchunk;
-    for (int i = ib; i < ie; i++) {
-        const long long q = (long long)i * g.si + p;
+    long long q = (long long)ib * g.si + p;
+    float hy_m = ib >= 1 ? Hy[q - g.si] : 0.f, hz_m = ib >= 1 ? Hz[q - g.si] : 0.f;
+    for (int i = ib; i < ie; i++, q += g.si) {
         const bool ib_ = i >= 1 && i < g.nx;
+        const float hx = Hx[q], hy = Hy[q], hz = Hz[q];
+        const float hx_k = k >= 1 ? Hx[q - 1] : 0.f, hy_k = k >= 1 ? Hy[q - 1] : 0.f;
+        const float hx_j = j >= 1 ? Hx[q - g.sj] : 0.f, hz_j = j >= 1 ? Hz[q - g.sj] : 0.f;
         const float ex_s = a.es[0][i];
         const int sx = a.has_pml ? a.pml.slot[0][i] : -1;
-        const long long rbase = (long long)i * g.nchunk;
+        const unsigned fl = a.flags[(long long)i * g.nchunk + chunk];
+        const long long rbase = (long long)i * g.nchunk + chunk;
         if (i < g.nx) {     // Ex
-            const float e0 = Ex[q];
+            const float e0 = Ex[q], cbv = cb[q];
             float e = e0;
-            const float cb = a.cb[q];
             if (jb && kb) {
-                const float dy_hz = Hz[q] - Hz[q - g.sj], dz_hy = Hy[q] - Hy[q - 1];
-                e = a.ca[q] * e0 + cb * (dy_hz * ey_s - dz_hy * ez_s);
+                const float dy_hz = hz - hz_j, dz_hy = hy - hy_k;
+                e = ca[q] * e0 + cbv * (dy_hz * ey_s - dz_hy * ez_s);
                 if (sy >= 0) {
                     float *ps = a.psi[0][0] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
-                    float v = a.pml.b[1][j] * *ps + a.pml.c[1][j] * dy_hz;
+                    const float v = by * *ps + cy * dy_hz;
                     *ps = v;
-                    e += cb * v;
+                    e += cbv * v;
                 }
                 if (sz >= 0) {
                     float *ps = a.psi[0][1] + sz * a.pml.per[2] + (long long)i * (g.ny + 1) + j;
-                    float v = a.pml.b[2][k] * *ps + a.pml.c[2][k] * dz_hy;
+                    const float v = bz * *ps + cz * dz_hy;
                     *ps = v;
-                    e -= cb * v;
+                    e -= cbv * v;
                 }
             }
-            e = e_post(a, 0, 0 * plane + rbase, chunk, lane, q, e, e0, cb);
+            if (fl & 7u) e = e_post(a, 0, fl & 7u, rbase, lane, q, e, e0, cbv);
             if (e != e0) Ex[q] = e;
         }
         if (j < g.ny) {     // Ey
-            const float e0 = Ey[q];
+            const float e0 = Ey[q], cbv = cb[g.N + q];
             float e = e0;
-            const float cb = a.cb[g.N + q];
             if (ib_ && kb) {
-                const float dz_hx = Hx[q] - Hx[q - 1], dx_hz = Hz[q] - Hz[q - g.si];
-                e = a.ca[g.N + q] * e0 + cb * (dz_hx * ez_s - dx_hz * ex_s);
+                const float dz_hx = hx - hx_k, dx_hz = hz - hz_m;
+                e = ca[g.N + q] * e0 + cbv * (dz_hx * ez_s - dx_hz * ex_s);
                 if (sx >= 0) {
                     float *ps = a.psi[1][1] + sx * a.pml.per[0] + p;
-                    float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hz;
+                    const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hz;
                     *ps = v;
-                    e -= cb * v;
+                    e -= cbv * v;
                 }
                 if (sz >= 0) {
                     float *ps = a.psi[1][0] + sz * a.pml.per[2] + (long long)i * (g.ny + 1) + j;
-                    float v = a.pml.b[2][k] * *ps + a.pml.c[2][k] * dz_hx;
+                    const float v = bz * *ps + cz * dz_hx;
                     *ps = v;
-                    e += cb * v;
+                    e += cbv * v;
                 }
             }
-            e = e_post(a, 1, 1 * plane + rbase, chunk, lane, q, e, e0, cb);
+            if (fl & 56u) e = e_post(a, 1, (fl >> 3) & 7u, plane + rbase, lane, q, e, e0, cbv);
             if (e != e0) Ey[q] = e;
         }
         if (k < g.nz) {     // Ez
-            const float e0 = Ez[q];
+            const float e0 = Ez[q], cbv = cb[2 * g.N + q];
             float e = e0;
-            const float cb = a.cb[2 * g.N + q];
             if (ib_ && jb) {
-                const float dx_hy = Hy[q] - Hy[q - g.si], dy_hx = Hx[q] - Hx[q - g.sj];
-                e = a.ca[2 * g.N + q] * e0 + cb * (dx_hy * ex_s - dy_hx * ey_s);
+                const float dx_hy = hy - hy_m, dy_hx = hx - hx_j;
+                e = ca[2 * g.N + q] * e0 + cbv * (dx_hy * ex_s - dy_hx * ey_s);
                 if (sx >= 0) {
                     float *ps = a.psi[2][0] + sx * a.pml.per[0] + p;
-                    float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hy;
+                    const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hy;
                     *ps = v;
-                    e += cb * v;
+                    e += cbv * v;
                 }
                 if (sy >= 0) {
                     float *ps = a.psi[2][1] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
-                    float v = a.pml.b[1][j] * *ps + a.pml.c[1][j] * dy_hx;
+                    const float v = by * *ps + cy * dy_hx;
                     *ps = v;
-                    e -= cb * v;
+                    e -= cbv * v;
                 }
             }
-            e = e_post(a, 2, 2 * plane + rbase, chunk, lane, q, e, e0, cb);
+            if (fl & 448u) e = e_post(a, 2, (fl >> 6) & 7u, 2 * plane + rbase, lane, q, e, e0, cbv);
             if (e != e0) Ez[q] = e;
         }
+        hy_m = hy;
+        hz_m = hz;
     }
 }
 
@@ -551,6 +580,13 @@ __global__ void k_sum(const double *partial, int n, double *out) {
     }
 }
 
+// plain streaming copy, 16 B per lane per access: the measured-bandwidth reference
+__global__ void __launch_bounds__(kBlock) k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n) {
+    long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (; t < n; t += stride) dst[t] = src[t];
+}
+
 __global__ void k_f32_to_f64(const float *src, double *dst, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
@@ -644,10 +680,11 @@ struct Device {
     // sparse tables: kind 0 Debye, 1 SIBC, 2 source
     DBuf<unsigned> tbase[3];
     DBuf<unsigned long long> tmask[3];
+    std::vector<unsigned long long> hmask[3];   // host copies, for the flag table
+    DBuf<unsigned short> flags;  // per (i, chunk): bit 3*comp + kind
     DBuf<float> J, bdt, Y, wl, wm;
     DBuf<unsigned char> face;
-    float kc[kMaxPoles] = {}, km1[kMaxPoles] = {};
-    float zc[kSibcTerms] = {}, om[kSibcTerms] = {};
+    DBuf<PostParams> pp;
     bool sources_dirty = true;
     // probes
     DBuf<long long> tidx;
@@ -668,6 +705,8 @@ struct Device {
     // energy
     DBuf<double> d1d, partial;
     int energy_blocks = 0;
+    // launch shape: x planes per thread in the stencil kernels
+    int iseg_h = kIseg, iseg_e = kIseg;
     // timing
     bool profiling = false;
     KernelTimer tH, tE, tG, tD;
@@ -711,7 +750,27 @@ int build_sparse(cem_sim *s, Device *D, int kind, const std::vector<std::pair<in
         if (D->tbase[kind].alloc(rows) || D->tmask[kind].alloc(rows)) return -1;
     }
     if (D->tbase[kind].upload(base.data(), rows, D->st) || D->tmask[kind].upload(mask.data(), rows, D->st)) return -1;
-    return hipStreamSynchronize(D->st) == hipSuccess ? 0 : -1;
+    D->hmask[kind] = std::move(mask);
+    return 0;
+}
+
+// flag word per (i, chunk): which (comp, kind) tables have members there
+int build_flags(Device *D) {
+    const Grid &g = D->g;
+    const size_t per = (size_t)(g.nx + 1) * g.nchunk;
+    std::vector<unsigned short> fl(per, 0);
+    for (int kind = 0; kind < 3; kind++) {
+        const auto &m = D->hmask[kind];
+        if (m.size() != 3 * per) continue;
+        for (int c = 0; c < 3; c++)
+            for (size_t r = 0; r < per; r++)
+                if (m[c * per + r]) fl[r] |= (unsigned short)(1u << (3 * c + kind));
+    }
+    if (D->flags.n != per) {
+        D->flags.release();
+        if (D->flags.alloc(per)) return -1;
+    }
+    return D->flags.upload(fl.data(), per, D->st);
 }
 
 int upload_sources(cem_sim *s, Device *D) {
@@ -743,6 +802,7 @@ int upload_sources(cem_sim *s, Device *D) {
     if (D->wl.alloc(wl.size()) || D->wm.alloc(wm.size())) return -1;
     if (D->wl.upload(wl.data(), wl.size(), D->st) || D->wm.upload(wm.data(), wm.size(), D->st)) return -1;
     if (build_sparse(s, D, 2, member)) return -1;
+    if (build_flags(D)) return -1;
     D->sources_dirty = false;
     return 0;
 }
@@ -768,9 +828,9 @@ int upload_plan(cem_sim *s, Device *D) {
         std::vector<float> tmp(N);
         for (int c = 0; c < 3; c++) {
             for (size_t q = 0; q < N; q++) tmp[q] = (float)s->ca[c][q];
-            HIP_OK(hipMemcpy(D->ca.p + c * N, tmp.data(), N * sizeof(float), hipMemcpyHostToDevice));
+            if (hipMemcpyAsync(D->ca.p + c * N, tmp.data(), N * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return -1;
             for (size_t q = 0; q < N; q++) tmp[q] = (float)s->cb[c][q];
-            HIP_OK(hipMemcpy(D->cb.p + c * N, tmp.data(), N * sizeof(float), hipMemcpyHostToDevice));
+            if (hipMemcpyAsync(D->cb.p + c * N, tmp.data(), N * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return -1;
         }
     }
 
@@ -857,12 +917,22 @@ int upload_plan(cem_sim *s, Device *D) {
         std::vector<float> b(nj);
         for (size_t q = 0; q < nj; q++) b[q] = (float)(s->d_beta[q] * (1.0 / s->dt));
         if (D->bdt.upload(b.data(), nj, st)) return -1;
+        PostParams pp{};
+        pp.ndisp = s->ndisp;
+        pp.npole = s->npole;
+        pp.nsb = s->nsb;
+        pp.nterm = s->sibc_npole;
         for (int p = 0; p < s->npole; p++) {
             const double tau = 1.0 / (2 * M_PI * s->pole_f[p]);
             const double x = s->dt / (2 * tau);
-            D->kc[p] = (float)(1.0 / (1.0 + x));          // (1 + k)/2
-            D->km1[p] = (float)(-2.0 * x / (1.0 + x));     // k - 1
+            pp.kc[p] = (float)(1.0 / (1.0 + x));          // (1 + k)/2
+            pp.km1[p] = (float)(-2.0 * x / (1.0 + x));     // k - 1
         }
+        for (int t = 0; t < s->sibc_npole; t++) {
+            pp.zc[t] = (float)s->sibc_c[t];
+            pp.om[t] = (float)(-std::expm1(-s->sibc_a[t] * s->dt));
+        }
+        if (D->pp.alloc(1) || D->pp.upload(&pp, 1, st)) return -1;
     }
     // SIBC tables
     {
@@ -872,10 +942,6 @@ int upload_plan(cem_sim *s, Device *D) {
         const size_t ny_ = (size_t)s->nsb * s->sibc_npole;
         if (D->Y.alloc(ny_) || D->Y.zero(st) || D->face.alloc(s->nsb)) return -1;
         if (D->face.upload(s->sb_mask.data(), s->nsb, st)) return -1;
-        for (int t = 0; t < s->sibc_npole; t++) {
-            D->zc[t] = (float)s->sibc_c[t];
-            D->om[t] = (float)(-std::expm1(-s->sibc_a[t] * s->dt));
-        }
     }
     if (upload_sources(s, D)) return -1;
 
@@ -985,25 +1051,13 @@ EArgs e_args(cem_sim *s, Device *D, long step) {
     for (int c = 0; c < 3; c++)
         for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiE[c][w];
     a.has_pml = s->has_pml;
-    const long long rows = 3LL * (D->g.nx + 1) * D->g.nchunk;
-    (void)rows;
-    for (int c = 0; c < 3; c++) {
-        a.dtab[c] = Sparse{D->tbase[0].p, D->tmask[0].p};
-        a.stab[c] = Sparse{D->tbase[1].p, D->tmask[1].p};
-        a.xtab[c] = Sparse{D->tbase[2].p, D->tmask[2].p};
-    }
-    a.ndisp = s->ndisp;
-    a.npole = s->npole;
+    a.flags = D->flags.p;
+    for (int k = 0; k < 3; k++) a.tab[k] = Sparse{D->tbase[k].p, D->tmask[k].p};
+    a.pp = D->pp.p;
     a.J = D->J.p;
     a.bdt = D->bdt.p;
-    std::memcpy(a.kc, D->kc, sizeof a.kc);
-    std::memcpy(a.km1, D->km1, sizeof a.km1);
-    a.nsb = s->nsb;
-    a.nterm = s->sibc_npole;
     a.Y = D->Y.p;
     a.face = D->face.p;
-    std::memcpy(a.zc, D->zc, sizeof a.zc);
-    std::memcpy(a.om, D->om, sizeof a.om);
     a.wl = D->wl.p;
     a.wm = D->wm.p;
     const double tH = (step + 0.5) * s->dt;
@@ -1033,18 +1087,18 @@ void timed_end(Device *D, KernelTimer &t) {
 void launch_h(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     HArgs a = h_args(s, D);
-    dim3 grid((D->g.S + kBlock - 1) / kBlock, (i1 - i0 + kIseg - 1) / kIseg);
+    dim3 grid((D->g.S + kBlock - 1) / kBlock, (i1 - i0 + D->iseg_h - 1) / D->iseg_h);
     timed_begin(D, D->tH);
-    hipLaunchKernelGGL(k_update_h, grid, dim3(kBlock), 0, D->st, a, i0, i1);
+    hipLaunchKernelGGL(k_update_h, grid, dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_h);
     timed_end(D, D->tH);
 }
 
 void launch_e(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     EArgs a = e_args(s, D, s->step);
-    dim3 grid((D->g.S + kBlock - 1) / kBlock, (i1 - i0 + kIseg - 1) / kIseg);
+    dim3 grid((D->g.S + kBlock - 1) / kBlock, (i1 - i0 + D->iseg_e - 1) / D->iseg_e);
     timed_begin(D, D->tE);
-    hipLaunchKernelGGL(k_update_e, grid, dim3(kBlock), 0, D->st, a, i0, i1);
+    hipLaunchKernelGGL(k_update_e, grid, dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_e);
     timed_end(D, D->tE);
 }
 
@@ -1129,6 +1183,13 @@ int ensure_vlast(cem_sim *s, Device *D) {
     return 0;
 }
 
+// host <-> device copies ordered after all work queued on the engine stream
+void copy_sync(Device *D, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    if (!bytes) return;
+    HIP_CHECK_VOID(hipMemcpyAsync(dst, src, bytes, kind, D->st));
+    HIP_CHECK_VOID(hipStreamSynchronize(D->st));
+}
+
 void download_f32(Device *D, const float *src, double *dst, size_t n) {
     std::vector<float> tmp(n);
     HIP_CHECK_VOID(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, D->st));
@@ -1170,6 +1231,8 @@ int cem::dev_ensure(cem_sim *s) {
         return -1;
     }
     D->profiling = std::getenv("COURANT_PROFILE") != nullptr;
+    if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("COURANT_ISEG_E")) D->iseg_e = std::max(1, std::atoi(v));
     s->spec_src.assign(2 * s->nf, 0.0);
     for (size_t p = 0; p < s->port.size(); p++) {
         s->spec_port_v[p].assign(2 * s->nf, 0.0);
@@ -1204,6 +1267,8 @@ void cem::dev_destroy(cem_sim *s) {
     D->wl.release();
     D->wm.release();
     D->face.release();
+    D->flags.release();
+    D->pp.release();
     D->tidx.release();
     D->tw.release();
     D->post.release();
@@ -1406,12 +1471,12 @@ void cem::dev_push_samples(cem_sim *s, int n, const double *ip, const double *vo
     int done = 0;
     while (done < n) {
         const int m = std::min(n - done, kSeriesCap);
-        if (nip) HIP_CHECK_VOID(hipMemcpy(D->ser_ip, ip + (size_t)done * nip, m * nip * sizeof(double), hipMemcpyHostToDevice));
+        if (nip) copy_sync(D, D->ser_ip, ip + (size_t)done * nip, m * nip * sizeof(double), hipMemcpyHostToDevice);
         if (np) {
-            HIP_CHECK_VOID(hipMemcpy(D->ser_vold, vold + (size_t)done * np, m * np * sizeof(double), hipMemcpyHostToDevice));
-            HIP_CHECK_VOID(hipMemcpy(D->ser_vnew, vnew + (size_t)done * np, m * np * sizeof(double), hipMemcpyHostToDevice));
+            copy_sync(D, D->ser_vold, vold + (size_t)done * np, m * np * sizeof(double), hipMemcpyHostToDevice);
+            copy_sync(D, D->ser_vnew, vnew + (size_t)done * np, m * np * sizeof(double), hipMemcpyHostToDevice);
         }
-        if (nvp) HIP_CHECK_VOID(hipMemcpy(D->ser_vp, vp + (size_t)done * nvp, m * nvp * sizeof(double), hipMemcpyHostToDevice));
+        if (nvp) copy_sync(D, D->ser_vp, vp + (size_t)done * nvp, m * nvp * sizeof(double), hipMemcpyHostToDevice);
         D->row0 = s->step;
         D->rows = m;
         launch_dft(s, D, 1);
@@ -1428,11 +1493,11 @@ void cem::dev_phasors(cem_sim *s, double *phE, double *phH, int download) {
     double *cur = D->ph[D->ph_cur].p;
     HIP_CHECK_VOID(hipStreamSynchronize(D->st));
     if (download) {
-        HIP_CHECK_VOID(hipMemcpy(phE, cur, nb, hipMemcpyDeviceToHost));
-        HIP_CHECK_VOID(hipMemcpy(phH, cur + 2 * s->nf, nb, hipMemcpyDeviceToHost));
+        copy_sync(D, phE, cur, nb, hipMemcpyDeviceToHost);
+        copy_sync(D, phH, cur + 2 * s->nf, nb, hipMemcpyDeviceToHost);
     } else {
-        HIP_CHECK_VOID(hipMemcpy(cur, phE, nb, hipMemcpyHostToDevice));
-        HIP_CHECK_VOID(hipMemcpy(cur + 2 * s->nf, phH, nb, hipMemcpyHostToDevice));
+        copy_sync(D, cur, phE, nb, hipMemcpyHostToDevice);
+        copy_sync(D, cur + 2 * s->nf, phH, nb, hipMemcpyHostToDevice);
     }
 }
 
@@ -1446,7 +1511,7 @@ void cem::dev_acc_io(cem_sim *s, double *buf, int download) {
     const size_t np = s->port.size(), nvp = s->vp.size(), nip = s->ip.size();
     std::vector<double> all((size_t)D->nacc * blk);
     if (download) {
-        HIP_CHECK_VOID(hipMemcpy(all.data(), D->acc.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+        copy_sync(D, all.data(), D->acc.p, all.size() * sizeof(double), hipMemcpyDeviceToHost);
     }
     // device order: V(np) I(np) vp ip src; checkpoint order: (V, I) per port, vp, ip, src
     size_t o = 0;
@@ -1464,7 +1529,7 @@ void cem::dev_acc_io(cem_sim *s, double *buf, int download) {
     for (size_t p = 0; p < nvp; p++) io(2 * np + p);
     for (size_t p = 0; p < nip; p++) io(2 * np + nvp + p);
     io(2 * np + nvp + nip);
-    if (!download) HIP_CHECK_VOID(hipMemcpy(D->acc.p, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!download) copy_sync(D, D->acc.p, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice);
     dev_phasors(s, buf + o, buf + o + blk, download);
 }
 
@@ -1509,4 +1574,37 @@ extern "C" int cem_amd_step_device(cem_sim *s, long n) {
     if (!s || !s->finalized || dev_ensure(s)) return -1;
     dev_step(s, n);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Stream-copy bandwidth of this GPU: copies `bytes` (rounded down to 16 B) between
+// two device buffers `iters` times and returns the best GB/s (read + write bytes).
+extern "C" double cem_amd_copy_bandwidth(size_t bytes, int iters) {
+    const long long n = (long long)(bytes / 16);
+    if (n <= 0 || iters <= 0) return -1;
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return -1;
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return -1;
+    }
+    (void)hipMemset(a, 0, n * 16);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int grid = 256 * 8 * 4;
+    double best = 0;
+    for (int it = 0; it < iters; it++) {
+        (void)hipEventRecord(e0, nullptr);
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, nullptr, a, b, n);
+        (void)hipEventRecord(e1, nullptr);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms > 0) best = std::max(best, 2.0 * n * 16 / (ms * 1e-3) / 1e9);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return best;
 }

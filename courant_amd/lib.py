@@ -129,6 +129,7 @@ AMD_SIGNATURES = {
     "cem_amd_set_profiling": (_I, [_P, _I]),
     "cem_amd_synchronize": (_I, [_P]),
     "cem_amd_step_device": (_I, [_P, _L]),
+    "cem_amd_copy_bandwidth": (_D, [_S, _I]),
 }
 
 
@@ -208,6 +209,9 @@ class Sim:
             raise CourantError(cem.last_error())
         self.h = h
         self.N = (self.n[0] + 1) * (self.n[1] + 1) * (self.n[2] + 1)
+        self.freqs = np.array([1e9])          # cem_finalize's default (sim.c:429)
+        self.excitation_lumped = (0.0, 0.0, 1.0)
+        self.excitation_mode = None
 
     def close(self):
         if self.h:
@@ -268,7 +272,9 @@ class Sim:
         return self._ok(self.lib.cem_add_lumped_port(self.h, axis, n1, n2, m0, m1, direction, R))
 
     def set_gaussian(self, port, amp, t0, tau):
-        return self._ok(self.lib.cem_set_gaussian(self.h, port, amp, t0, tau))
+        rc = self._ok(self.lib.cem_set_gaussian(self.h, port, amp, t0, tau))
+        self.excitation_lumped = (amp, t0, tau)
+        return rc
 
     def set_frequencies(self, f):
         f = _f64(f)
@@ -286,7 +292,9 @@ class Sim:
         return self._ok(self.lib.cem_add_mode_port(self.h, C.cast(C.pointer(d), _P)))
 
     def set_mode_excitation(self, port, t0, tau):
-        return self._ok(self.lib.cem_set_mode_excitation(self.h, port, t0, tau))
+        rc = self._ok(self.lib.cem_set_mode_excitation(self.h, port, t0, tau))
+        self.excitation_mode = (t0, tau)
+        return rc
 
     def finalize(self, cfl=0.99):
         return self._ok(self.lib.cem_finalize(self.h, cfl))

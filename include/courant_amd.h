@@ -28,6 +28,10 @@ CEM_API int cem_amd_set_profiling(cem_sim *s, int on);
  * switched on: which = 0 H update, 1 E update, 2 probe gather, 3 DFT. */
 CEM_API int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count);
 
+/* Stream-copy bandwidth of the current GPU in GB/s (read + write bytes), best
+ * of `iters` copies of `bytes` between two device buffers; -1 on error. */
+CEM_API double cem_amd_copy_bandwidth(size_t bytes, int iters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,63 @@
+"""Writes the golden fixtures of tests/golden from the compiled reference core
+(oracle/_ref/libcourant_ref.so, built from the reference sources by oracle/Makefile).
+
+  parity_native.json   the values solver/apps/parity.c prints (17 digits): dt, energy,
+                       V1, V2, Vp, Ip at 5/20/40 GHz after 1500 steps
+  checkpoint_plan.npz  integer plan of the solver/tests/test_checkpoint.c scene: Debye
+                       edge list, SIBC edge and face lists, port / probe lists, psi lengths
+  gcpw_mini.npz        S11, S21 of a small lumped-port GCPW after 4000 steps (f64)
+
+Run: python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from courant_amd import lib, scenes  # noqa: E402
+
+GCPW_MINI = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=3e-3, yair=0.4e-3, zair=0.6e-3, zbelow=0.2e-3,
+                 pml_below=True, df=0.5e9)
+
+
+def main():
+    ref = lib.Courant(os.path.join(ROOT, "oracle", "_ref", "libcourant_ref.so"), extensions=False)
+    s, info = scenes.parity_scene(ref, steps=1500)
+    out = {"dt": s.dt, "energy": s.energy()}
+    p1, p2 = info["ports"]
+    for name, v in (("V1", s.port_v(p1)), ("V2", s.port_v(p2)), ("Vp", s.vprobe(info["vprobe"])),
+                    ("Ip", s.iprobe(info["iprobe"]))):
+        out[name] = [[float(x.real), float(x.imag)] for x in v]
+    with open(os.path.join(HERE, "parity_native.json"), "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+
+    c = scenes.checkpoint_scene(ref)
+    plan = c.plan()
+    np.savez_compressed(os.path.join(HERE, "checkpoint_plan.npz"),
+                        disp_idx=plan["disp"]["idx"].astype(np.int64),
+                        sibc_idx=plan["sibc"]["idx"].astype(np.int64),
+                        sibc_hidx=plan["sibc"]["hidx"].astype(np.int64),
+                        sibc_hw=plan["sibc"]["hw"],
+                        port_idx=np.concatenate([p["idx"] for p in plan["ports"]]).astype(np.int64),
+                        vprobe_idx=np.concatenate([p["idx"] for p in plan["vprobes"]]).astype(np.int64),
+                        iprobe_idx=np.concatenate([p["idx"] for p in plan["iprobes"]]).astype(np.int64),
+                        state_len=np.array([plan["state_len"][(w, cc, k)] for w in range(5) for cc in range(3)
+                                            for k in range(2)], dtype=np.int64),
+                        cb_zero=np.concatenate([np.flatnonzero(plan["cb"][cc] == 0) + cc * c.N
+                                                for cc in range(3)]).astype(np.int64))
+
+    g, gi = scenes.gcpw(ref, **GCPW_MINI)
+    g.step(4000)
+    s11, s21 = scenes.gcpw_sparams(g, gi)
+    np.savez_compressed(os.path.join(HERE, "gcpw_mini.npz"), freqs=gi["freqs"], s11=s11, s21=s21, steps=4000)
+    print("golden fixtures written")
+
+
+if __name__ == "__main__":
+    main()

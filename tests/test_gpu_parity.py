@@ -32,17 +32,36 @@ def print_case(name):
 
 
 def test_parity_scene_matches_reference(cem, ref, gpu_available):
-    """solver/apps/parity.c after 1500 steps: V1, V2, Vp, Ip spectra, energy and fields."""
-    print_case("parity scene 1500 steps")
-    a, ia = S.parity_scene(ref, steps=1500)
-    b, ib = S.parity_scene(cem, steps=1500)
+    """solver/apps/parity.c scene.  Fields after 600 steps within 1e-4 of the largest field
+    value (the protocol of web/test/gpu-test.ts:42-55); V1, V2, Vp, Ip, I1 and source spectra
+    after 1500 steps within 1e-4; at 1500 steps, when the fields have decayed 60x, the GPU's
+    deviation from f64 is no larger than that of an FP32 CPU emulation of the same step."""
+    from oracle_binding import OracleSim
+    print_case("parity scene 600 / 1500 steps")
+    a, ia = S.parity_scene(ref)
+    b, ib = S.parity_scene(cem)
+    o32 = OracleSim(b, f32=True)
+    a.step(600)
+    b.step(600)
+    fa, fb = fields(a), fields(b)
+    scale = max(np.max(np.abs(f)) for f in fa)
+    worst = max(np.max(np.abs(x - y)) for x, y in zip(fa, fb))
+    print("600 steps: max |dF| / max |F| = %.2e" % (worst / scale))
+    assert worst <= FIELD_TOL * scale
+    a.step(900)
+    b.step(900)
+    o32.step(1500)
     for get in (lambda s, i: s.port_v(i["ports"][0]), lambda s, i: s.port_v(i["ports"][1]),
                 lambda s, i: s.vprobe(i["vprobe"]), lambda s, i: s.iprobe(i["iprobe"]),
                 lambda s, i: s.port_i(i["ports"][0]), lambda s, i: s.source_spectrum()):
         assert rel(get(b, ib), get(a, ia)) < SPEC_TOL
     assert abs(b.energy() - a.energy()) / a.energy() < 1e-4
-    for fa, fb in zip(fields(a), fields(b)):
-        assert np.max(np.abs(fa - fb)) <= FIELD_TOL * max(np.max(np.abs(fa)), 1e-30)
+    fa = fields(a)
+    scale = max(np.max(np.abs(f)) for f in fa)
+    gpu = max(np.max(np.abs(x - y)) for x, y in zip(fa, fields(b)))
+    emu = max(np.max(np.abs(x - o32.field(c))) for c, x in enumerate(fa))
+    print("1500 steps: GPU %.2e, FP32 CPU emulation %.2e of max |F|" % (gpu / scale, emu / scale))
+    assert gpu <= 3 * emu + 1e-7 * scale
     assert b.steps == 1500
 
 
@@ -219,3 +238,100 @@ def test_push_samples_matches_reference(cem, ref, gpu_available):
     for x, y in zip(res[0][:-1], res[1][:-1]):
         assert rel(y, x) < 1e-12
     assert res[0][-1] == res[1][-1] == 700
+
+
+def test_gpu_matches_oracle_restatement(cem, gpu_available):
+    """The engine against the oracle restatement driven by the engine's own plan (no
+    compiled reference involved): lossy mode-port GCPW, probe spectra after 1500 steps."""
+    from oracle_binding import OracleSim
+    print_case("gpu vs oracle restatement")
+    b, ib = S.gcpw(cem, **MODE_MINI)
+    o = OracleSim(b)
+    b.step(1500)
+    o.step(1500)
+    for p in range(cem.lib.cem_plan_nvprobe(b.h)):
+        assert rel(b.vprobe(p), o.vprobe(p)) < SPEC_TOL
+    for p in range(cem.lib.cem_plan_niprobe(b.h)):
+        assert rel(b.iprobe(p), o.iprobe(p)) < SPEC_TOL
+    assert rel(b.source_spectrum(), o.source_spectrum()) < 1e-12
+    assert abs(b.energy() - o.energy()) / o.energy() < 1e-4
+
+
+def test_cavity_discrete_dispersion_on_gpu(cem, gpu_available):
+    """PEC cavity: TE101/201/102 peaks at the Yee discrete-dispersion frequencies within 2e-4
+    (solver/tests/test_cavity.c:89-99), initial field written through cem_field."""
+    import math
+    print_case("cavity dispersion")
+    a, b, d = 10e-3, 5e-3, 8e-3
+    nx, ny, nz = 40, 20, 32
+    x, y, z = np.linspace(0, a, nx + 1), np.linspace(0, b, ny + 1), np.linspace(0, d, nz + 1)
+    modes = [(1, 1), (2, 1), (1, 2)]
+    fc = [L.C0 / 2 * math.hypot(m / a, p / d) for m, p in modes]
+    df = 30e9 / 4000
+    freqs = np.concatenate([np.arange(int(f * 0.97 / df), int(f * 1.03 / df) + 1) * df for f in fc])
+    s = cem.create_sim(x, y, z)
+    s.set_frequencies(freqs)
+    s.add_vprobe(L.Y, 13, 9, 7, 8, 1)
+    s.finalize(0.99)
+    Ey = s.field(L.Y)
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    bump = np.exp(-((X - 0.31 * a) ** 2 + (Z - 0.37 * d) ** 2) / (2 * 1.5e-3 ** 2))
+    Ey[1:nx, 0:ny, 1:nz] = bump[1:nx, None, 1:nz]
+    dt = s.dt
+    s.step(int(20e-9 / dt))
+    spec = np.abs(s.vprobe(0))
+    dx, dz = a / nx, d / nz
+    for (m, p), f0 in zip(modes, fc):
+        sel = np.flatnonzero((freqs >= f0 * 0.97) & (freqs <= f0 * 1.03))
+        k = sel[np.argmax(spec[sel])]
+        A, B, Cc = spec[k - 1], spec[k], spec[k + 1]
+        fpk = freqs[k] + 0.5 * (A - Cc) / (A - 2 * B + Cc) * df
+        kx, kz = m * math.pi / a, p * math.pi / d
+        Sv = L.C0 * dt * math.hypot(math.sin(kx * dx / 2) / dx, math.sin(kz * dz / 2) / dz)
+        fref = 2 / dt * math.asin(Sv) / (2 * math.pi)
+        print("TE%d0%d %.6f GHz vs %.6f GHz (%.1e)" % (m, p, fpk / 1e9, fref / 1e9, (fpk - fref) / fref))
+        assert abs(fpk - fref) / fref < 2e-4
+
+
+@pytest.mark.parametrize("rough,rq", [(0, 0.0), (1, 1e-6)])
+def test_copper_cavity_q_on_gpu(cem, gpu_available, rough, rq):
+    """Copper-walled cavity in TE101: Q from the energy decay within 3% of Pozar's closed
+    form, smooth and Groisse-rough (solver/tests/test_sibc.c:18-68)."""
+    import math
+    print_case(f"copper cavity Q rough={rough}")
+    a, b, d, sigma = 10e-3, 5e-3, 8e-3, 5.8e7
+    nx, ny, nz = 40, 20, 32
+    x, y, z = np.linspace(0, a, nx + 1), np.linspace(0, b, ny + 1), np.linspace(0, d, nz + 1)
+    s = cem.create_sim(x, y, z)
+    f0 = L.C0 / 2 * math.sqrt(1 / a ** 2 + 1 / d ** 2)
+    s.set_conductor_loss(sigma, rough, rq, 0, f0 / 3, f0 * 3)
+    s.set_frequencies([f0])
+    s.finalize(0.99)
+    Ey = s.field(L.Y)
+    Ey[1:nx, 0:ny, 1:nz] = (np.sin(np.pi * x[1:nx] / a)[:, None, None] * np.sin(np.pi * z[1:nz] / d)[None, None, :])
+    dt = s.dt
+    s.step(int(1e-9 / dt))
+    nsamp, chunk = 40, int(0.5e-9 / dt)
+    ts, lw = [], []
+    for _ in range(nsamp):
+        nav = 16
+        per = int(1 / f0 / dt / nav)
+        W = 0.0
+        for _ in range(nav):
+            s.step(per)
+            W += s.energy() / nav
+        s.step(chunk - per * nav)
+        ts.append(s.steps * dt)
+        lw.append(math.log(W))
+    slope = np.polyfit(ts, lw, 1)[0]
+    w0 = 2 * math.pi * f0
+    Qm = -w0 / slope
+    Rs = math.sqrt(w0 * L.MU0 / (2 * sigma))
+    k = w0 / L.C0
+    eta = 376.730313668
+    Qa = (k * a * d) ** 3 * b * eta / (2 * math.pi ** 2 * Rs) / (2 * a ** 3 * b + 2 * b * d ** 3 + a ** 3 * d + a * d ** 3)
+    if rough == 1:
+        delta = math.sqrt(2 / (w0 * L.MU0 * sigma))
+        Qa /= 1 + math.exp(-(delta / (2 * rq)) ** 1.6)
+    print("Q measured %.1f, analytic %.1f (%+.2f%%)" % (Qm, Qa, 100 * (Qm - Qa) / Qa))
+    assert abs(Qm - Qa) / Qa < 0.03

@@ -1,0 +1,199 @@
+"""Pins of the oracle restatement (oracle/oracle.c) before it is trusted as the
+checker of the GPU engine.
+
+  * golden vectors: the values solver/apps/parity.c prints, relative error <= 1e-12
+    (the tolerance of solver/tests/parity_check.py), driven by this engine's own plan;
+  * the compiled reference core: bit-identical fields and spectra on scenes that
+    exercise every per-step feature (CPML, lossy media, PEC, lumped and modal
+    sources, probes, Debye ADE, SIBC);
+  * closed forms the mathematics fixes: Yee discrete dispersion of a PEC cavity
+    (solver/tests/test_cavity.c, 2e-4), CPML reflection against a large PEC box
+    (solver/tests/test_cpml.c, < 1e-2), energy conservation of a lossless PEC box,
+    the 2-port renormalisation identity.
+A planted mistake in the restatement (a dropped CPML term, a sign flip, a wrong
+index) moves at least one of these far outside its tolerance.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from courant_amd import lib as L
+from courant_amd import scenes as S
+from oracle_binding import ORACLE_F64, OracleSim
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(ORACLE_F64), reason="oracle not built (make -C oracle restate)")
+
+
+def relerr(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def test_oracle_reproduces_parity_golden_vectors(cem):
+    gold = json.load(open(os.path.join(GOLDEN, "parity_native.json")))
+    s, info = S.parity_scene(cem)
+    assert s.dt == gold["dt"]
+    o = OracleSim(s)
+    o.step(1500)
+    assert relerr(o.energy(), gold["energy"]) <= 1e-12
+    p1, p2 = info["ports"]
+    for name, got in (("V1", o.port_v(p1)), ("V2", o.port_v(p2)), ("Vp", o.vprobe(info["vprobe"])),
+                      ("Ip", o.iprobe(info["iprobe"]))):
+        for g, (re, im) in zip(got, gold[name]):
+            scale = max(abs(re), abs(im), 1e-300)
+            assert abs(g.real - re) <= 1e-12 * scale and abs(g.imag - im) <= 1e-12 * scale, name
+
+
+def test_engine_plan_matches_golden_integer_plan(cem):
+    gold = np.load(os.path.join(GOLDEN, "checkpoint_plan.npz"))
+    s = S.checkpoint_scene(cem)
+    p = s.plan()
+    np.testing.assert_array_equal(p["disp"]["idx"].astype(np.int64), gold["disp_idx"])
+    np.testing.assert_array_equal(p["sibc"]["idx"].astype(np.int64), gold["sibc_idx"])
+    np.testing.assert_array_equal(p["sibc"]["hidx"].astype(np.int64), gold["sibc_hidx"])
+    np.testing.assert_array_equal(p["sibc"]["hw"], gold["sibc_hw"])
+    np.testing.assert_array_equal(np.concatenate([x["idx"] for x in p["ports"]]).astype(np.int64), gold["port_idx"])
+    np.testing.assert_array_equal(np.concatenate([x["idx"] for x in p["vprobes"]]).astype(np.int64),
+                                  gold["vprobe_idx"])
+    np.testing.assert_array_equal(np.concatenate([x["idx"] for x in p["iprobes"]]).astype(np.int64),
+                                  gold["iprobe_idx"])
+    lens = np.array([p["state_len"][(w, c, k)] for w in range(5) for c in range(3) for k in range(2)])
+    np.testing.assert_array_equal(lens, gold["state_len"])
+    cbz = np.concatenate([np.flatnonzero(p["cb"][c] == 0) + c * s.N for c in range(3)])
+    np.testing.assert_array_equal(cbz, gold["cb_zero"])
+
+
+MODE_MINI = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=1.5e-3, feed=1.2e-3, mon=0.6e-3, yair=0.3e-3,
+                 zair=0.4e-3, ports="mode", lossy=True, tcu=35e-6, mask=25e-6, dc=False, df=4e9)
+LUMPED_MINI = dict(hmin=40e-6, hmax=100e-6, dx=200e-6, length=2e-3, yair=0.3e-3, zair=0.4e-3, zbelow=0.2e-3,
+                   pml_below=True, df=2e9)
+
+BITWISE_SCENES = {
+    "parity": (lambda c: S.parity_scene(c)[0], 700),
+    "debye_sibc": (S.checkpoint_scene, 700),
+    "gcpw_lumped_cpml6": (lambda c: S.gcpw(c, **LUMPED_MINI)[0], 500),
+    "gcpw_mode_lossy": (lambda c: S.gcpw(c, **MODE_MINI)[0], 400),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BITWISE_SCENES))
+def test_oracle_is_bit_identical_to_reference(name, ref):
+    build, steps = BITWISE_SCENES[name]
+    a = build(ref)
+    o = OracleSim(a)
+    a.step(steps)
+    o.step(steps)
+    for c in range(6):
+        assert np.array_equal(a.field(c), o.field(c)), c
+    for p in range(ref.lib.cem_plan_nport(a.h)):
+        assert np.array_equal(a.port_v(p), o.port_v(p)) and np.array_equal(a.port_i(p), o.port_i(p))
+    for p in range(ref.lib.cem_plan_nvprobe(a.h)):
+        assert np.array_equal(a.vprobe(p), o.vprobe(p))
+    for p in range(ref.lib.cem_plan_niprobe(a.h)):
+        assert np.array_equal(a.iprobe(p), o.iprobe(p))
+    assert np.array_equal(a.source_spectrum(), o.source_spectrum())
+    assert o.energy() == a.energy()
+
+
+def test_oracle_cavity_discrete_dispersion(cem):
+    """Uniform PEC cavity started with a Gaussian Ey bump: TE101, TE201, TE102 peaks at the
+    Yee discrete-dispersion frequencies within 2e-4 (solver/tests/test_cavity.c:89-99)."""
+    a, b, d = 10e-3, 5e-3, 8e-3
+    nx, ny, nz = 20, 10, 16
+    x, y, z = np.linspace(0, a, nx + 1), np.linspace(0, b, ny + 1), np.linspace(0, d, nz + 1)
+    modes = [(1, 1), (2, 1), (1, 2)]
+    fc = [L.C0 / 2 * math.hypot(m / a, p / d) for m, p in modes]
+    df = 30e9 / 4000
+    freqs = np.concatenate([np.arange(int(f * 0.97 / df), int(f * 1.03 / df) + 1) * df for f in fc])
+    s = cem.create_sim(x, y, z)
+    s.set_frequencies(freqs)
+    s.add_vprobe(L.Y, 7, 4, 5, 6, 1)
+    s.finalize(0.99)
+    o = OracleSim(s)
+    E = np.zeros((nx + 1, ny + 1, nz + 1))
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    bump = np.exp(-((X - 0.31 * a) ** 2 + (Z - 0.37 * d) ** 2) / (2 * 1.5e-3 ** 2))
+    E[1:nx, 0:ny, 1:nz] = bump[1:nx, None, 1:nz]
+    o.set_field(L.Y, E)
+    dt = s.dt
+    o.step(int(20e-9 / dt))
+    spec = np.abs(o.vprobe(0))
+    dx, dz = a / nx, d / nz
+    for (m, p), f0 in zip(modes, fc):
+        sel = np.flatnonzero((freqs >= f0 * 0.97) & (freqs <= f0 * 1.03))
+        k = sel[np.argmax(spec[sel])]
+        A, B, Cc = spec[k - 1], spec[k], spec[k + 1]
+        fpk = freqs[k] + 0.5 * (A - Cc) / (A - 2 * B + Cc) * df
+        kx, kz = m * math.pi / a, p * math.pi / d
+        Sv = L.C0 * dt * math.hypot(math.sin(kx * dx / 2) / dx, math.sin(kz * dz / 2) / dz)
+        ref = 2 / dt * math.asin(Sv) / (2 * math.pi)
+        assert abs(fpk - ref) / ref < 2e-4, (m, p, fpk, ref)
+
+
+def test_oracle_lossless_box_conserves_energy(cem):
+    """After the source has ended, a lossless PEC box with a lumped port disconnected
+    (source only) keeps its discrete energy constant to rounding."""
+    g = np.linspace(0, 2e-3, 13)
+    s = cem.create_sim(g, g, g)
+    p = s.add_lumped_port(L.Z, 6, 6, 5, 7, 1, 1e12)   # nearly open port: negligible loss
+    s.set_gaussian(p, 1.0, 20e-12, 5e-12)
+    s.finalize(0.99)
+    o = OracleSim(s)
+    o.step(int(60e-12 / s.dt))
+    w0 = o.energy()
+    o.step(400)
+    assert abs(o.energy() - w0) / w0 < 5e-3
+
+
+def test_oracle_cpml_reflection(cem):
+    """Dipole near a 10-cell CPML against a 128^3 PEC reference box: reflection < 1e-2
+    (solver/tests/test_cpml.c:163-209)."""
+    H = 1e-3
+
+    def run(interior, npml, nsteps):
+        n = interior + 2 * npml
+        c = np.array([(i - n / 2.0) * H for i in range(n + 1)])
+        s = cem.create_sim(c, c, c)
+        if npml:
+            for f in range(6):
+                s.set_boundary(f, L.CPML, npml)
+        m = n // 2
+        p = s.add_lumped_port(L.Z, m, m, m, m + 1, 1, 50.0)
+        tau = 15e-12
+        s.set_gaussian(p, 1.0, 4 * tau, tau)
+        s.set_frequencies([1e9])
+        s.finalize(0.99)
+        o = OracleSim(s)
+        q = s.index(m + 30 // 2 - 4, m + 3, m)   # same cell offset from the source in both runs
+        rec = []
+        for _ in range(nsteps):
+            o.step(1)
+            rec.append(o.field_at(L.Z, q))
+        return np.array(rec)
+
+    nsteps = int(100 * H / L.C0 / (0.99 * H / (L.C0 * math.sqrt(3.0))))
+    a = run(30, 10, nsteps)
+    b = run(128, 0, nsteps)
+    assert np.max(np.abs(a - b)) / np.max(np.abs(b)) < 1e-2
+
+
+def test_oracle_renormalize(ref):
+    import ctypes as C
+    from oracle_binding import _load
+    orc = _load(ORACLE_F64)
+    rng = np.random.default_rng(11)
+    for _ in range(10):
+        v = rng.standard_normal(8) * 0.3
+        z1, z2 = rng.uniform(20, 90, 2)
+        a, b = v.copy(), v.copy()
+        ref.lib.cem_s2_renormalize(a.ctypes.data_as(L._DP), z1, z2, 50.0)
+        orc.orc_s2_renormalize(b.ctypes.data_as(C.POINTER(C.c_double)), z1, z2, 50.0)
+        assert np.allclose(a, b, rtol=1e-13, atol=1e-15)
+        c = v.copy()
+        orc.orc_s2_renormalize(c.ctypes.data_as(C.POINTER(C.c_double)), z1, z1, z1)
+        assert np.allclose(c, v, atol=1e-12)
