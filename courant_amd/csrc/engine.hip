@@ -40,7 +40,7 @@ using namespace cem;
 namespace cem {
 
 constexpr int kBlock = 256;        // 4 wavefronts of 64
-constexpr int kIseg = 8;           // i-planes per thread in the stencil kernels
+constexpr int kIseg = 2;           // default i-planes per thread in the stencil kernels
 constexpr int kSeriesCap = 512;    // steps recorded between DFT flushes
 
 #define HIP_OK(expr)                                                                  \
@@ -110,7 +110,9 @@ struct EArgs {
     Grid g;
     float *E;               // 3N
     const float *H;         // 3N
-    const float *ca, *cb;   // 3N each
+    const float *ca, *cb;   // 3N each (uncompressed coefficients)
+    const unsigned *cidx;   // N: packed 8-bit coefficient indices of Ex, Ey, Ez (compressed)
+    const float2 *ctab;     // 3 x 256 (ca, cb) pairs (compressed)
     const float *es[3];     // ide per axis
     Pml pml;
     float *psi[3][2];       // psiE[c][w]
@@ -158,6 +160,15 @@ struct DftArgs {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Remap
+// the dispatch id so that XCD x runs one contiguous band of logical tiles: tiles
+// that share halo rows (neighbouring p blocks, consecutive i segments) then
+// share one L2.  A bijection on [0, G).
+__device__ __forceinline__ int xcd_tile(int wg, int G) {
+    const int q = G >> 3, r = G & 7, x = wg & 7;
+    return x * q + min(x, r) + (wg >> 3);
+}
+
 __device__ __forceinline__ int wave_chunk(int p) {
     return __builtin_amdgcn_readfirstlane(p >> 6);
 }
@@ -171,12 +182,14 @@ __device__ __forceinline__ bool sparse_slot(const Sparse &t, long long row, int 
 
 // One thread per (j,k) column of an i-segment; the E values of plane i+1 are
 // carried in registers into the next iteration.
-__global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, int iseg) {
+__global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, int iseg, int npblk) {
     const Grid g = a.g;
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int pblk = tile % npblk, seg = tile / npblk;
+    const int p = pblk * kBlock + threadIdx.x;
     if (p >= g.S) return;
     const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
-    const int ib = i0 + blockIdx.y * iseg;
+    const int ib = i0 + seg * iseg;
     const int ie = min(i1, ib + iseg);
     const float *__restrict__ Ex = a.E;
     const float *__restrict__ Ey = a.E + g.N;
@@ -308,14 +321,22 @@ __device__ __forceinline__ float e_post(const EArgs &a, int c, unsigned kinds, l
 
 // One thread per (j,k) column of an i-segment; the H values of plane i-1 are
 // carried in registers from the previous iteration.
-__global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, int iseg) {
+template <bool kCmp>
+__global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, int iseg, int npblk) {
     const Grid g = a.g;
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    const int chunk = wave_chunk(blockIdx.x * kBlock + threadIdx.x);
+    __shared__ float2 tab[kCmp ? 3 * 256 : 1];
+    if (kCmp) {
+        for (int t = threadIdx.x; t < 3 * 256; t += kBlock) tab[t] = a.ctab[t];
+        __syncthreads();
+    }
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int pblk = tile % npblk, seg = tile / npblk;
+    const int p = pblk * kBlock + threadIdx.x;
+    const int chunk = wave_chunk(p);
     const int lane = threadIdx.x & 63;
     if (p >= g.S) return;
     const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
-    const int ib = i0 + blockIdx.y * iseg;
+    const int ib = i0 + seg * iseg;
     const int ie = min(i1, ib + iseg);
     float *__restrict__ Ex = a.E;
     float *__restrict__ Ey = a.E + g.N;
@@ -325,6 +346,7 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     const float *__restrict__ Hz = a.H + 2 * g.N;
     const float *__restrict__ ca = a.ca;
     const float *__restrict__ cb = a.cb;
+    const unsigned *__restrict__ cidx = a.cidx;
     const bool jb = j >= 1 && j < g.ny, kb = k >= 1 && k < g.nz;   // interior node ranges
     const float ey_s = a.es[1][j], ez_s = a.es[2][k];
     const int sy = a.has_pml ? a.pml.slot[1][j] : -1;
@@ -343,12 +365,23 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
         const int sx = a.has_pml ? a.pml.slot[0][i] : -1;
         const unsigned fl = a.flags[(long long)i * g.nchunk + chunk];
         const long long rbase = (long long)i * g.nchunk + chunk;
+        float2 cx, cy2, cz2;   // (ca, cb) of Ex, Ey, Ez at q
+        if (kCmp) {
+            const unsigned ci = cidx[q];
+            cx = tab[ci & 255u];
+            cy2 = tab[256 + ((ci >> 8) & 255u)];
+            cz2 = tab[512 + ((ci >> 16) & 255u)];
+        } else {
+            cx = make_float2(i < g.nx ? ca[q] : 0.f, i < g.nx ? cb[q] : 0.f);
+            cy2 = make_float2(j < g.ny ? ca[g.N + q] : 0.f, j < g.ny ? cb[g.N + q] : 0.f);
+            cz2 = make_float2(k < g.nz ? ca[2 * g.N + q] : 0.f, k < g.nz ? cb[2 * g.N + q] : 0.f);
+        }
         if (i < g.nx) {     // Ex
-            const float e0 = Ex[q], cbv = cb[q];
+            const float e0 = Ex[q], cbv = cx.y;
             float e = e0;
             if (jb && kb) {
                 const float dy_hz = hz - hz_j, dz_hy = hy - hy_k;
-                e = ca[q] * e0 + cbv * (dy_hz * ey_s - dz_hy * ez_s);
+                e = cx.x * e0 + cbv * (dy_hz * ey_s - dz_hy * ez_s);
                 if (sy >= 0) {
                     float *ps = a.psi[0][0] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
                     const float v = by * *ps + cy * dy_hz;
@@ -366,11 +399,11 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
             if (e != e0) Ex[q] = e;
         }
         if (j < g.ny) {     // Ey
-            const float e0 = Ey[q], cbv = cb[g.N + q];
+            const float e0 = Ey[q], cbv = cy2.y;
             float e = e0;
             if (ib_ && kb) {
                 const float dz_hx = hx - hx_k, dx_hz = hz - hz_m;
-                e = ca[g.N + q] * e0 + cbv * (dz_hx * ez_s - dx_hz * ex_s);
+                e = cy2.x * e0 + cbv * (dz_hx * ez_s - dx_hz * ex_s);
                 if (sx >= 0) {
                     float *ps = a.psi[1][1] + sx * a.pml.per[0] + p;
                     const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hz;
@@ -388,11 +421,11 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
             if (e != e0) Ey[q] = e;
         }
         if (k < g.nz) {     // Ez
-            const float e0 = Ez[q], cbv = cb[2 * g.N + q];
+            const float e0 = Ez[q], cbv = cz2.y;
             float e = e0;
             if (ib_ && jb) {
                 const float dx_hy = hy - hy_m, dy_hx = hx - hx_j;
-                e = ca[2 * g.N + q] * e0 + cbv * (dx_hy * ex_s - dy_hx * ey_s);
+                e = cz2.x * e0 + cbv * (dx_hy * ex_s - dy_hx * ey_s);
                 if (sx >= 0) {
                     float *ps = a.psi[2][0] + sx * a.pml.per[0] + p;
                     const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hy;
@@ -531,6 +564,8 @@ __global__ void k_dft(DftArgs a) {
 struct EnergyArgs {
     Grid g;
     const float *E, *H, *cb;
+    const unsigned *cidx;
+    const float2 *ctab;
     const double *d[3], *dd[3];
     double dt;
 };
@@ -549,7 +584,7 @@ __global__ void __launch_bounds__(kBlock) k_energy(EnergyArgs a, double *partial
         const int n[3] = {g.nx, g.ny, g.nz};
         for (int c = 0; c < 3; c++) {
             const int u = (c + 1) % 3, v = (c + 2) % 3;
-            const float cbv = a.cb[c * g.N + q];
+            const float cbv = a.cidx ? a.ctab[c * 256 + ((a.cidx[q] >> (8 * c)) & 255u)].y : a.cb[c * g.N + q];
             if (ijk[c] < n[c] && cbv != 0.f) {
                 const double vol = a.d[c][ijk[c]] * a.dd[u][ijk[u]] * a.dd[v][ijk[v]];
                 const double eps = a.dt / (double)cbv;
@@ -669,7 +704,10 @@ struct Device {
     hipStream_t st = nullptr;
     Grid g{};
     DBuf<float> F;               // E (3N) then H (3N)
-    DBuf<float> ca, cb;          // 3N each
+    DBuf<float> ca, cb;          // 3N each, only when more than 256 distinct pairs
+    DBuf<unsigned> cidx;         // N packed coefficient indices
+    DBuf<float2> ctab;           // 3 x 256 (ca, cb)
+    bool compressed = false;
     DBuf<float> axis1d;          // hs[3], es[3], pml b/c for H and E per axis
     DBuf<int> slots;             // CPML slot tables
     DBuf<float> psi;             // all psi arrays
@@ -823,8 +861,48 @@ int upload_plan(cem_sim *s, Device *D) {
 
     // fields (zero) and coefficients
     if (D->F.alloc(6 * N) || D->F.zero(st)) return -1;
-    if (D->ca.alloc(3 * N) || D->cb.alloc(3 * N)) return -1;
+    // Coefficient compression: the (ca, cb) pairs of each component take few
+    // distinct values (one per material mix), so each point stores three 8-bit
+    // indices into a 256-entry table; more than 256 pairs falls back to arrays.
     {
+        std::vector<unsigned> idx(N, 0);
+        std::vector<float2> tab(3 * 256, make_float2(0.f, 0.f));
+        bool ok = std::getenv("COURANT_COEF_ARRAYS") == nullptr;   // switch to force the array path
+        for (int c = 0; c < 3 && ok; c++) {
+            std::vector<std::pair<unsigned long long, unsigned>> seen;   // (ca bits << 32 | cb bits) -> slot
+            unsigned long long last_key = ~0ull;
+            unsigned last_slot = 0;
+            for (size_t q = 0; q < N && ok; q++) {
+                const float fa = (float)s->ca[c][q], fb = (float)s->cb[c][q];
+                unsigned ua, ub;
+                std::memcpy(&ua, &fa, 4);
+                std::memcpy(&ub, &fb, 4);
+                const unsigned long long key = ((unsigned long long)ua << 32) | ub;
+                unsigned slot = last_slot;
+                if (key != last_key) {
+                    auto it = std::find_if(seen.begin(), seen.end(), [&](const auto &e) { return e.first == key; });
+                    if (it == seen.end()) {
+                        if (seen.size() == 256) { ok = false; break; }
+                        slot = (unsigned)seen.size();
+                        seen.push_back({key, slot});
+                        tab[c * 256 + slot] = make_float2(fa, fb);
+                    } else {
+                        slot = it->second;
+                    }
+                    last_key = key;
+                    last_slot = slot;
+                }
+                idx[q] |= slot << (8 * c);
+            }
+        }
+        D->compressed = ok;
+        if (ok) {
+            if (D->cidx.alloc(N) || D->cidx.upload(idx.data(), N, st)) return -1;
+            if (D->ctab.alloc(3 * 256) || D->ctab.upload(tab.data(), 3 * 256, st)) return -1;
+        }
+    }
+    if (!D->compressed && (D->ca.alloc(3 * N) || D->cb.alloc(3 * N))) return -1;
+    if (!D->compressed) {
         std::vector<float> tmp(N);
         for (int c = 0; c < 3; c++) {
             for (size_t q = 0; q < N; q++) tmp[q] = (float)s->ca[c][q];
@@ -1046,6 +1124,8 @@ EArgs e_args(cem_sim *s, Device *D, long step) {
     a.H = D->F.p + 3 * D->g.N;
     a.ca = D->ca.p;
     a.cb = D->cb.p;
+    a.cidx = D->compressed ? D->cidx.p : nullptr;
+    a.ctab = D->compressed ? D->ctab.p : nullptr;
     for (int i = 0; i < 3; i++) a.es[i] = D->es[i];
     a.pml = D->pmlE;
     for (int c = 0; c < 3; c++)
@@ -1087,18 +1167,21 @@ void timed_end(Device *D, KernelTimer &t) {
 void launch_h(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     HArgs a = h_args(s, D);
-    dim3 grid((D->g.S + kBlock - 1) / kBlock, (i1 - i0 + D->iseg_h - 1) / D->iseg_h);
+    const int npblk = (D->g.S + kBlock - 1) / kBlock, nseg = (i1 - i0 + D->iseg_h - 1) / D->iseg_h;
     timed_begin(D, D->tH);
-    hipLaunchKernelGGL(k_update_h, grid, dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_h);
+    hipLaunchKernelGGL(k_update_h, dim3(npblk * nseg), dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_h, npblk);
     timed_end(D, D->tH);
 }
 
 void launch_e(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     EArgs a = e_args(s, D, s->step);
-    dim3 grid((D->g.S + kBlock - 1) / kBlock, (i1 - i0 + D->iseg_e - 1) / D->iseg_e);
+    const int npblk = (D->g.S + kBlock - 1) / kBlock, nseg = (i1 - i0 + D->iseg_e - 1) / D->iseg_e;
     timed_begin(D, D->tE);
-    hipLaunchKernelGGL(k_update_e, grid, dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_e);
+    if (D->compressed)
+        hipLaunchKernelGGL(k_update_e<true>, dim3(npblk * nseg), dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_e, npblk);
+    else
+        hipLaunchKernelGGL(k_update_e<false>, dim3(npblk * nseg), dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_e, npblk);
     timed_end(D, D->tE);
 }
 
@@ -1254,6 +1337,8 @@ void cem::dev_destroy(cem_sim *s) {
     D->F.release();
     D->ca.release();
     D->cb.release();
+    D->cidx.release();
+    D->ctab.release();
     D->axis1d.release();
     D->slots.release();
     D->psi.release();
@@ -1327,6 +1412,8 @@ double cem::dev_energy(cem_sim *s) {
     a.E = D->F.p;
     a.H = D->F.p + 3 * D->g.N;
     a.cb = D->cb.p;
+    a.cidx = D->compressed ? D->cidx.p : nullptr;
+    a.ctab = D->compressed ? D->ctab.p : nullptr;
     size_t o = 0;
     for (int c = 0; c < 3; c++) {
         a.d[c] = D->d1d.p + o;

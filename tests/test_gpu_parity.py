@@ -335,3 +335,19 @@ def test_copper_cavity_q_on_gpu(cem, gpu_available, rough, rq):
         Qa /= 1 + math.exp(-(delta / (2 * rq)) ** 1.6)
     print("Q measured %.1f, analytic %.1f (%+.2f%%)" % (Qm, Qa, 100 * (Qm - Qa) / Qa))
     assert abs(Qm - Qa) / Qa < 0.03
+
+
+def test_coefficient_table_matches_coefficient_arrays(cem, gpu_available, monkeypatch):
+    """The compressed coefficient path (8-bit indices into a (ca, cb) table) and the full
+    per-edge coefficient arrays give bit-identical fields and spectra."""
+    print_case("coefficient table vs arrays")
+    a = S.checkpoint_scene(cem)
+    a.step(300)
+    monkeypatch.setenv("COURANT_COEF_ARRAYS", "1")
+    b = S.checkpoint_scene(cem)
+    b.step(300)
+    monkeypatch.delenv("COURANT_COEF_ARRAYS")
+    for fa, fb in zip(fields(a), fields(b)):
+        assert np.array_equal(fa, fb)
+    assert np.array_equal(a.port_v(1), b.port_v(1))
+    assert a.energy() == b.energy()
