@@ -4,15 +4,15 @@
 //   k_update_h : bulk H curl-E update fused with the CPML psi_H recursions
 //                (sim.c:579-606, 680-754), one thread per (j,k) column of an
 //                i-segment, FP32;
-//   k_update_e : bulk E update fused with CPML psi_E (sim.c:608-637, 756-832)
-//                and the post-phase per-edge physics in reference order: Debye
-//                correction (dispersive.c:93-100), SIBC overwrite
-//                (sibc.c:256-269), lumped and modal sources (sim.c:900-909,
-//                modeport.c:130-135) and the Debye current update
-//                (dispersive.c:102-112).  Sparse edge sets are located with
-//                per-wavefront (base, 64-bit mask) tables: a lane's slot is
-//                base + popcount(mask below the lane), so the sparse state is
-//                read coalesced, in the reference's own (comp, i, j, k) order;
+//   k_update_e : bulk E update fused with CPML psi_E (sim.c:608-637, 756-832);
+//                (ca, cb) come from a per-component table addressed by 8-bit
+//                indices, since a grid has only a handful of distinct pairs;
+//   k_post     : the post-phase per-edge physics over the compact edge lists,
+//                in reference order: Debye correction (dispersive.c:93-100),
+//                SIBC overwrite (sibc.c:256-269), lumped and modal sources
+//                (sim.c:900-909, modeport.c:130-135) and the Debye current
+//                update (dispersive.c:102-112); one thread per listed edge, the
+//                lists in the reference's own (comp, i, j, k) order;
 //   k_gather   : port / probe line and loop integrals, one wavefront per
 //                sample, FP64 sums reduced with fixed-order shuffles, written
 //                to a per-step sample series;
@@ -80,11 +80,6 @@ struct Pml {                // CPML per derivative axis a: slot per index (-1 ou
     long long per[3];       // points per slab plane
 };
 
-struct Sparse {             // per (comp, i, chunk) slot tables of one sparse edge kind
-    const unsigned *base;
-    const unsigned long long *mask;
-};
-
 struct HArgs {
     Grid g;
     const float *E;         // 3N
@@ -117,14 +112,28 @@ struct EArgs {
     Pml pml;
     float *psi[3][2];       // psiE[c][w]
     int has_pml;
-    // sparse edge sets: kind 0 Debye, 1 SIBC, 2 sources; rows (comp, i, chunk)
-    const unsigned short *flags;   // per (i, chunk): bit 3*comp + kind
-    Sparse tab[3];
+};
+
+// Post-phase edge lists: threads [0, ndisp) Debye edges, then nsb SIBC edges,
+// then nxs source edges that are not dispersive.  Indices are global E indices
+// comp*N + q.
+struct PostArgs {
+    float *E;               // 3N
+    const float *H;         // 3N
+    long long N, si, sj;
     const PostParams *pp;
-    float *J;               // npole * ndisp (pole-major)
+    int ndisp, nsb, nxs;
+    const unsigned *d_idx;
+    const float *d_cb;      // cb of each dispersive edge
+    const int *d_src;       // source slot of a dispersive edge, -1 if none (nullptr: none at all)
+    float *J;               // npole * ndisp (pole-major, reference order)
     const float *bdt;       // beta/dt, npole * ndisp
-    float *Y;               // nterm * nsb (term-major)
+    float *Eold;            // E of each dispersive edge before this step's E update
+    const unsigned *sb_idx;
     const unsigned char *face;
+    float *Y;               // nterm * nsb (term-major)
+    const unsigned *x_idx;  // non-dispersive source edges
+    const int *x_src;       // their source slots
     const float *wl, *wm;   // lumped cs and modal weight per source slot
     float vs, gm;           // waveform values at tH of this step
 };
@@ -167,17 +176,6 @@ struct DftArgs {
 __device__ __forceinline__ int xcd_tile(int wg, int G) {
     const int q = G >> 3, r = G & 7, x = wg & 7;
     return x * q + min(x, r) + (wg >> 3);
-}
-
-__device__ __forceinline__ int wave_chunk(int p) {
-    return __builtin_amdgcn_readfirstlane(p >> 6);
-}
-
-__device__ __forceinline__ bool sparse_slot(const Sparse &t, long long row, int lane, unsigned *slot) {
-    const unsigned long long m = t.mask[row];
-    if (!((m >> lane) & 1ull)) return false;
-    *slot = t.base[row] + (unsigned)__builtin_popcountll(m & ((1ull << lane) - 1ull));
-    return true;
 }
 
 // One thread per (j,k) column of an i-segment; the E values of plane i+1 are
@@ -269,56 +267,6 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
     }
 }
 
-// Post-phase physics of one E edge of comp c (kinds: bit0 Debye, bit1 SIBC,
-// bit2 source), in the reference order: Debye correction, SIBC overwrite,
-// sources, Debye current update.
-__device__ __forceinline__ float e_post(const EArgs &a, int c, unsigned kinds, long long row, int lane, long long q,
-                                     float e, float e_old, float cb) {
-    const Grid &g = a.g;
-    const PostParams &pp = *a.pp;
-    unsigned dslot = 0;
-    const bool deb = (kinds & 1u) && sparse_slot(a.tab[0], row, lane, &dslot);
-    if (deb) {
-        float acc = 0.f;
-        for (int p = 0; p < pp.npole; p++) acc += pp.kc[p] * a.J[(long long)p * pp.ndisp + dslot];
-        e -= cb * acc;
-    }
-    unsigned sslot = 0;
-    if ((kinds & 2u) && sparse_slot(a.tab[1], row, lane, &sslot)) {
-        const int u = (c + 1) % 3, v = (c + 2) % 3;
-        const long long su = u == 0 ? g.si : u == 1 ? g.sj : 1;
-        const long long sv = v == 0 ? g.si : v == 1 ? g.sj : 1;
-        const float *Hu = a.H + u * g.N, *Hv = a.H + v * g.N;
-        const unsigned m = a.face[sslot];
-        const float inv = 1.0f / (float)__builtin_popcount(m);
-        float Js = 0.f;
-        if (m & 1u) Js += -inv * Hu[q];
-        if (m & 2u) Js += inv * Hu[q - sv];
-        if (m & 4u) Js += inv * Hv[q];
-        if (m & 8u) Js += -inv * Hv[q - su];
-        float es = 0.f;
-        for (int t = 0; t < pp.nterm; t++) {
-            float *y = a.Y + (long long)t * pp.nsb + sslot;
-            float yv = *y;
-            yv += pp.om[t] * (Js - yv);
-            *y = yv;
-            es += pp.zc[t] * (Js - yv);
-        }
-        e = es;
-    }
-    unsigned xslot = 0;
-    if ((kinds & 4u) && sparse_slot(a.tab[2], row, lane, &xslot)) e += a.wl[xslot] * a.vs + a.wm[xslot] * a.gm;
-    if (deb) {
-        const float de = e - e_old;
-        for (int p = 0; p < pp.npole; p++) {
-            float *J = a.J + (long long)p * pp.ndisp + dslot;
-            const float jv = *J;
-            *J = jv + pp.km1[p] * jv + a.bdt[(long long)p * pp.ndisp + dslot] * de;
-        }
-    }
-    return e;
-}
-
 // One thread per (j,k) column of an i-segment; the H values of plane i-1 are
 // carried in registers from the previous iteration.
 template <bool kCmp>
@@ -332,8 +280,6 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int pblk = tile % npblk, seg = tile / npblk;
     const int p = pblk * kBlock + threadIdx.x;
-    const int chunk = wave_chunk(p);
-    const int lane = threadIdx.x & 63;
     if (p >= g.S) return;
     const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
     const int ib = i0 + seg * iseg;
@@ -353,7 +299,6 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     const int sz = a.has_pml ? a.pml.slot[2][k] : -1;
     const float by = sy >= 0 ? a.pml.b[1][j] : 0.f, cy = sy >= 0 ? a.pml.c[1][j] : 0.f;
     const float bz = sz >= 0 ? a.pml.b[2][k] : 0.f, cz = sz >= 0 ? a.pml.c[2][k] : 0.f;
-    const long long plane = (long long)(g.nx + 1) * g.nchunk;
     long long q = (long long)ib * g.si + p;
     float hy_m = ib >= 1 ? Hy[q - g.si] : 0.f, hz_m = ib >= 1 ? Hz[q - g.si] : 0.f;
     for (int i = ib; i < ie; i++, q += g.si) {
@@ -363,8 +308,6 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
         const float hx_j = j >= 1 ? Hx[q - g.sj] : 0.f, hz_j = j >= 1 ? Hz[q - g.sj] : 0.f;
         const float ex_s = a.es[0][i];
         const int sx = a.has_pml ? a.pml.slot[0][i] : -1;
-        const unsigned fl = a.flags[(long long)i * g.nchunk + chunk];
-        const long long rbase = (long long)i * g.nchunk + chunk;
         float2 cx, cy2, cz2;   // (ca, cb) of Ex, Ey, Ez at q
         if (kCmp) {
             const unsigned ci = cidx[q];
@@ -395,7 +338,6 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     e -= cbv * v;
                 }
             }
-            if (fl & 7u) e = e_post(a, 0, fl & 7u, rbase, lane, q, e, e0, cbv);
             if (e != e0) Ex[q] = e;
         }
         if (j < g.ny) {     // Ey
@@ -417,7 +359,6 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     e += cbv * v;
                 }
             }
-            if (fl & 56u) e = e_post(a, 1, (fl >> 3) & 7u, plane + rbase, lane, q, e, e0, cbv);
             if (e != e0) Ey[q] = e;
         }
         if (k < g.nz) {     // Ez
@@ -439,12 +380,75 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     e -= cbv * v;
                 }
             }
-            if (fl & 448u) e = e_post(a, 2, (fl >> 6) & 7u, 2 * plane + rbase, lane, q, e, e0, cbv);
             if (e != e0) Ez[q] = e;
         }
         hy_m = hy;
         hz_m = hz;
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const PostParams &pp = *a.pp;
+    if (t < a.ndisp) {                       // Debye edge
+        const unsigned q = a.d_idx[t];
+        float e = a.E[q];
+        float acc = 0.f;
+        for (int p = 0; p < pp.npole; p++) acc += pp.kc[p] * a.J[(long long)p * a.ndisp + t];
+        e -= a.d_cb[t] * acc;
+        if (a.d_src) {
+            const int x = a.d_src[t];
+            if (x >= 0) e += a.wl[x] * a.vs + a.wm[x] * a.gm;
+        }
+        const float de = e - a.Eold[t];
+        for (int p = 0; p < pp.npole; p++) {
+            const long long o = (long long)p * a.ndisp + t;
+            const float jv = a.J[o];
+            a.J[o] = jv + pp.km1[p] * jv + a.bdt[o] * de;
+        }
+        a.E[q] = e;
+        a.Eold[t] = e;                       // E of this edge before the next E update
+        return;
+    }
+    int u = t - a.ndisp;
+    if (u < a.nsb) {                         // surface-impedance edge
+        const unsigned q = a.sb_idx[u];
+        const int c = (int)(q / a.N);
+        const long long qq = (long long)q - c * a.N;
+        const int cu = (c + 1) % 3, cv = (c + 2) % 3;
+        const long long su = cu == 0 ? a.si : cu == 1 ? a.sj : 1;
+        const long long sv = cv == 0 ? a.si : cv == 1 ? a.sj : 1;
+        const float *Hu = a.H + cu * a.N, *Hv = a.H + cv * a.N;
+        const unsigned m = a.face[u];
+        const float inv = 1.0f / (float)__builtin_popcount(m);
+        float Js = 0.f;
+        if (m & 1u) Js += -inv * Hu[qq];
+        if (m & 2u) Js += inv * Hu[qq - sv];
+        if (m & 4u) Js += inv * Hv[qq];
+        if (m & 8u) Js += -inv * Hv[qq - su];
+        float es = 0.f;
+        for (int k = 0; k < pp.nterm; k++) {
+            float *y = a.Y + (long long)k * a.nsb + u;
+            float yv = *y;
+            yv += pp.om[k] * (Js - yv);
+            *y = yv;
+            es += pp.zc[k] * (Js - yv);
+        }
+        a.E[q] = es;
+        return;
+    }
+    u -= a.nsb;
+    if (u < a.nxs) {                         // source edge without dispersion
+        const unsigned q = a.x_idx[u];
+        const int x = a.x_src[u];
+        a.E[q] += a.wl[x] * a.vs + a.wm[x] * a.gm;
+    }
+}
+
+// E of every dispersive edge, taken as E_old of the next E update
+__global__ void __launch_bounds__(kBlock) k_prime_eold(const float *E, const unsigned *d_idx, float *Eold, int n) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) Eold[t] = E[d_idx[t]];
 }
 
 __global__ void __launch_bounds__(kBlock) k_gather(GatherArgs a) {
@@ -715,14 +719,15 @@ struct Device {
     size_t psiE_len[3][2] = {}, psiH_len[3][2] = {};
     const float *hs[3] = {}, *es[3] = {};
     Pml pmlH{}, pmlE{};
-    // sparse tables: kind 0 Debye, 1 SIBC, 2 source
-    DBuf<unsigned> tbase[3];
-    DBuf<unsigned long long> tmask[3];
-    std::vector<unsigned long long> hmask[3];   // host copies, for the flag table
-    DBuf<unsigned short> flags;  // per (i, chunk): bit 3*comp + kind
-    DBuf<float> J, bdt, Y, wl, wm;
+    // post-phase edge lists
+    DBuf<unsigned> d_idx, sb_idx, x_idx;
+    DBuf<int> d_src, x_src;
+    DBuf<float> d_cb, J, bdt, Eold, Y, wl, wm;
     DBuf<unsigned char> face;
     DBuf<PostParams> pp;
+    int nxs = 0;
+    bool has_dsrc = false;
+    bool eold_stale = false;
     bool sources_dirty = true;
     // probes
     DBuf<long long> tidx;
@@ -747,7 +752,7 @@ struct Device {
     int iseg_h = kIseg, iseg_e = kIseg;
     // timing
     bool profiling = false;
-    KernelTimer tH, tE, tG, tD;
+    KernelTimer tH, tE, tG, tD, tP;
 };
 
 }  // namespace cem
@@ -757,90 +762,66 @@ struct Device {
 // ---------------------------------------------------------------------------
 namespace {
 
-// Sparse edge tables for one kind: for every (comp, i, 64-point chunk) the slot
-// base and membership mask.  `member` lists (comp, q) in increasing order.
-int build_sparse(cem_sim *s, Device *D, int kind, const std::vector<std::pair<int, size_t>> &member) {
-    const Grid &g = D->g;
-    const size_t rows = 3 * (size_t)(g.nx + 1) * g.nchunk;
-    std::vector<unsigned> base(rows, 0);
-    std::vector<unsigned long long> mask(rows, 0);
-    size_t m = 0;
-    for (size_t r = 0; r < rows && m < member.size(); r++) {
-        const int c = (int)(r / ((size_t)(g.nx + 1) * g.nchunk));
-        const size_t rem = r % ((size_t)(g.nx + 1) * g.nchunk);
-        const size_t i = rem / g.nchunk, ch = rem % g.nchunk;
-        base[r] = (unsigned)m;
-        const size_t q0 = i * g.si + ch * 64;
-        while (m < member.size() && member[m].first == c && member[m].second < q0 + 64 &&
-               member[m].second >= q0 && member[m].second < (i + 1) * g.si) {
-            mask[r] |= 1ull << (member[m].second - q0);
-            m++;
-        }
-        // entries in rows of a later comp or plane keep base = m
-    }
-    if (m != member.size()) {
-        set_error("internal: sparse edge list not in grid order");
-        return -1;
-    }
-    if (D->tbase[kind].n != rows) {
-        D->tbase[kind].release();
-        D->tmask[kind].release();
-        if (D->tbase[kind].alloc(rows) || D->tmask[kind].alloc(rows)) return -1;
-    }
-    if (D->tbase[kind].upload(base.data(), rows, D->st) || D->tmask[kind].upload(mask.data(), rows, D->st)) return -1;
-    D->hmask[kind] = std::move(mask);
-    return 0;
-}
-
-// flag word per (i, chunk): which (comp, kind) tables have members there
-int build_flags(Device *D) {
-    const Grid &g = D->g;
-    const size_t per = (size_t)(g.nx + 1) * g.nchunk;
-    std::vector<unsigned short> fl(per, 0);
-    for (int kind = 0; kind < 3; kind++) {
-        const auto &m = D->hmask[kind];
-        if (m.size() != 3 * per) continue;
-        for (int c = 0; c < 3; c++)
-            for (size_t r = 0; r < per; r++)
-                if (m[c * per + r]) fl[r] |= (unsigned short)(1u << (3 * c + kind));
-    }
-    if (D->flags.n != per) {
-        D->flags.release();
-        if (D->flags.alloc(per)) return -1;
-    }
-    return D->flags.upload(fl.data(), per, D->st);
-}
-
+// Excited sources as (global E index, lumped cs, modal weight) slots; edges that
+// are also dispersive get their slot through d_src, the others go to x_idx.
 int upload_sources(cem_sim *s, Device *D) {
-    std::vector<std::pair<std::pair<int, size_t>, std::pair<double, double>>> src;
+    std::vector<std::pair<unsigned long long, std::pair<double, double>>> src;
     if (s->src_port >= 0) {
         const LumpedPort &pt = s->port[s->src_port];
-        for (size_t e = 0; e < pt.edge.size(); e++) src.push_back({{pt.axis, pt.edge[e]}, {pt.cs[e], 0.0}});
+        for (size_t e = 0; e < pt.edge.size(); e++)
+            src.push_back({(unsigned long long)pt.axis * s->N + pt.edge[e], {pt.cs[e], 0.0}});
     }
     if (s->msrc >= 0) {
         const ModePort &mp = s->mport[s->msrc];
         for (size_t e = 0; e < mp.src_idx.size(); e++)
-            src.push_back({{mp.src_comp[e], mp.src_idx[e]}, {0.0, mp.src_w[e]}});
+            src.push_back({(unsigned long long)mp.src_comp[e] * s->N + mp.src_idx[e], {0.0, mp.src_w[e]}});
     }
     std::stable_sort(src.begin(), src.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
-    std::vector<std::pair<int, size_t>> member;
+    std::vector<unsigned long long> where;
     std::vector<float> wl, wm;
     for (auto &e : src) {
-        if (!member.empty() && member.back() == e.first) {
+        if (!where.empty() && where.back() == e.first) {
             wl.back() += (float)e.second.first;
             wm.back() += (float)e.second.second;
             continue;
         }
-        member.push_back(e.first);
+        where.push_back(e.first);
         wl.push_back((float)e.second.first);
         wm.push_back((float)e.second.second);
     }
-    D->wl.release();
-    D->wm.release();
-    if (D->wl.alloc(wl.size()) || D->wm.alloc(wm.size())) return -1;
-    if (D->wl.upload(wl.data(), wl.size(), D->st) || D->wm.upload(wm.data(), wm.size(), D->st)) return -1;
-    if (build_sparse(s, D, 2, member)) return -1;
-    if (build_flags(D)) return -1;
+    std::vector<int> d_src(s->ndisp, -1), x_src;
+    std::vector<unsigned> x_idx;
+    D->has_dsrc = false;
+    for (size_t k = 0; k < where.size(); k++) {
+        const unsigned long long g = where[k];
+        // dispersive edges are sorted by global index (reference order c, i, j, k)
+        size_t lo = 0, hi = s->ndisp;
+        while (lo < hi) {
+            size_t mid = (lo + hi) / 2;
+            unsigned long long gm = (unsigned long long)s->d_comp[mid] * s->N + s->d_idx[mid];
+            if (gm < g) lo = mid + 1; else hi = mid;
+        }
+        if (lo < (size_t)s->ndisp && (unsigned long long)s->d_comp[lo] * s->N + s->d_idx[lo] == g) {
+            d_src[lo] = (int)k;
+            D->has_dsrc = true;
+        } else {
+            x_idx.push_back((unsigned)g);
+            x_src.push_back((int)k);
+        }
+    }
+    D->nxs = (int)x_idx.size();
+    for (auto *b : {&D->wl, &D->wm}) b->release();
+    D->x_idx.release();
+    D->x_src.release();
+    D->d_src.release();
+    if (D->wl.alloc(wl.size()) || D->wm.alloc(wm.size()) || D->x_idx.alloc(x_idx.size()) ||
+        D->x_src.alloc(x_src.size()))
+        return -1;
+    if (D->wl.upload(wl.data(), wl.size(), D->st) || D->wm.upload(wm.data(), wm.size(), D->st) ||
+        D->x_idx.upload(x_idx.data(), x_idx.size(), D->st) || D->x_src.upload(x_src.data(), x_src.size(), D->st))
+        return -1;
+    if (D->has_dsrc && (D->d_src.alloc(d_src.size()) || D->d_src.upload(d_src.data(), d_src.size(), D->st)))
+        return -1;
     D->sources_dirty = false;
     return 0;
 }
@@ -987,9 +968,19 @@ int upload_plan(cem_sim *s, Device *D) {
 
     // Debye ADE tables
     {
-        std::vector<std::pair<int, size_t>> member(s->ndisp);
-        for (int e = 0; e < s->ndisp; e++) member[e] = {s->d_comp[e], s->d_idx[e]};
-        if (build_sparse(s, D, 0, member)) return -1;
+        if (3 * N > 0xffffffffull) {
+            set_error("grid too large for 32-bit edge indices");
+            return -1;
+        }
+        std::vector<unsigned> idx(s->ndisp);
+        std::vector<float> cbd(s->ndisp);
+        for (int e = 0; e < s->ndisp; e++) {
+            idx[e] = (unsigned)(s->d_comp[e] * N + s->d_idx[e]);
+            cbd[e] = (float)s->d_cb[e];
+        }
+        if (D->d_idx.alloc(s->ndisp) || D->d_idx.upload(idx.data(), s->ndisp, st)) return -1;
+        if (D->d_cb.alloc(s->ndisp) || D->d_cb.upload(cbd.data(), s->ndisp, st)) return -1;
+        if (D->Eold.alloc(s->ndisp) || D->Eold.zero(st)) return -1;
         const size_t nj = (size_t)s->npole * s->ndisp;
         if (D->J.alloc(nj) || D->J.zero(st) || D->bdt.alloc(nj)) return -1;
         std::vector<float> b(nj);
@@ -1014,9 +1005,9 @@ int upload_plan(cem_sim *s, Device *D) {
     }
     // SIBC tables
     {
-        std::vector<std::pair<int, size_t>> member(s->nsb);
-        for (int e = 0; e < s->nsb; e++) member[e] = {s->sb_comp[e], s->sb_idx[e]};
-        if (build_sparse(s, D, 1, member)) return -1;
+        std::vector<unsigned> idx(s->nsb);
+        for (int e = 0; e < s->nsb; e++) idx[e] = (unsigned)(s->sb_comp[e] * N + s->sb_idx[e]);
+        if (D->sb_idx.alloc(s->nsb) || D->sb_idx.upload(idx.data(), s->nsb, st)) return -1;
         const size_t ny_ = (size_t)s->nsb * s->sibc_npole;
         if (D->Y.alloc(ny_) || D->Y.zero(st) || D->face.alloc(s->nsb)) return -1;
         if (D->face.upload(s->sb_mask.data(), s->nsb, st)) return -1;
@@ -1131,18 +1122,6 @@ EArgs e_args(cem_sim *s, Device *D, long step) {
     for (int c = 0; c < 3; c++)
         for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiE[c][w];
     a.has_pml = s->has_pml;
-    a.flags = D->flags.p;
-    for (int k = 0; k < 3; k++) a.tab[k] = Sparse{D->tbase[k].p, D->tmask[k].p};
-    a.pp = D->pp.p;
-    a.J = D->J.p;
-    a.bdt = D->bdt.p;
-    a.Y = D->Y.p;
-    a.face = D->face.p;
-    a.wl = D->wl.p;
-    a.wm = D->wm.p;
-    const double tH = (step + 0.5) * s->dt;
-    a.vs = (float)lumped_waveform(s, tH);
-    a.gm = (float)modeport_waveform(s, tH);
     return a;
 }
 
@@ -1183,6 +1162,40 @@ void launch_e(cem_sim *s, Device *D, int i0, int i1) {
     else
         hipLaunchKernelGGL(k_update_e<false>, dim3(npblk * nseg), dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_e, npblk);
     timed_end(D, D->tE);
+}
+
+void launch_post(cem_sim *s, Device *D, long step) {
+    const int total = s->ndisp + s->nsb + D->nxs;
+    if (!total) return;
+    PostArgs a{};
+    a.E = D->F.p;
+    a.H = D->F.p + 3 * D->g.N;
+    a.N = D->g.N;
+    a.si = D->g.si;
+    a.sj = D->g.sj;
+    a.pp = D->pp.p;
+    a.ndisp = s->ndisp;
+    a.nsb = s->nsb;
+    a.nxs = D->nxs;
+    a.d_idx = D->d_idx.p;
+    a.d_cb = D->d_cb.p;
+    a.d_src = D->has_dsrc ? D->d_src.p : nullptr;
+    a.J = D->J.p;
+    a.bdt = D->bdt.p;
+    a.Eold = D->Eold.p;
+    a.sb_idx = D->sb_idx.p;
+    a.face = D->face.p;
+    a.Y = D->Y.p;
+    a.x_idx = D->x_idx.p;
+    a.x_src = D->x_src.p;
+    a.wl = D->wl.p;
+    a.wm = D->wm.p;
+    const double tH = (step + 0.5) * s->dt;
+    a.vs = (float)lumped_waveform(s, tH);
+    a.gm = (float)modeport_waveform(s, tH);
+    timed_begin(D, D->tP);
+    hipLaunchKernelGGL(k_post, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, a);
+    timed_end(D, D->tP);
 }
 
 void launch_dft(cem_sim *s, Device *D, int direct) {
@@ -1263,6 +1276,11 @@ int ensure_vlast(cem_sim *s, Device *D) {
         launch_gather(s, D, 1);
         D->vlast_stale = false;
     }
+    if (D->eold_stale && s->ndisp) {
+        hipLaunchKernelGGL(k_prime_eold, dim3((s->ndisp + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, D->F.p,
+                           D->d_idx.p, D->Eold.p, s->ndisp);
+    }
+    D->eold_stale = false;
     return 0;
 }
 
@@ -1325,8 +1343,8 @@ int cem::dev_ensure(cem_sim *s) {
     s->spec_ip.assign(s->ip.size(), dvec(2 * s->nf, 0.0));
     // phasors of step 0 are set by the DFT kernel's resync at n % 1024 == 0
     for (int c = 0; c < 6; c++)
-        if (s->field_mirror[c]) D->vlast_stale = true;
-    if (s->step != 0) D->vlast_stale = true;
+        if (s->field_mirror[c]) D->vlast_stale = D->eold_stale = true;
+    if (s->step != 0) D->vlast_stale = D->eold_stale = true;
     return 0;
 }
 
@@ -1342,18 +1360,18 @@ void cem::dev_destroy(cem_sim *s) {
     D->axis1d.release();
     D->slots.release();
     D->psi.release();
-    for (int k = 0; k < 3; k++) {
-        D->tbase[k].release();
-        D->tmask[k].release();
-    }
     D->J.release();
     D->bdt.release();
     D->Y.release();
     D->wl.release();
     D->wm.release();
     D->face.release();
-    D->flags.release();
     D->pp.release();
+    for (auto *b : {&D->d_idx, &D->sb_idx, &D->x_idx}) b->release();
+    D->d_src.release();
+    D->x_src.release();
+    D->d_cb.release();
+    D->Eold.release();
     D->tidx.release();
     D->tw.release();
     D->post.release();
@@ -1368,7 +1386,7 @@ void cem::dev_destroy(cem_sim *s) {
     D->R.release();
     D->d1d.release();
     D->partial.release();
-    for (KernelTimer *t : {&D->tH, &D->tE, &D->tG, &D->tD}) t->destroy();
+    for (KernelTimer *t : {&D->tH, &D->tE, &D->tG, &D->tD, &D->tP}) t->destroy();
     (void)hipStreamDestroy(D->st);
     delete D;
     s->dev = nullptr;
@@ -1384,7 +1402,8 @@ void cem::dev_step(cem_sim *s, long nsteps) {
     const int nx = s->n[0];
     for (long it = 0; it < nsteps; it++) {
         launch_h(s, D, 0, nx + 1);
-        launch_e(s, D, 0, nx + 1);
+        launch_e(s, D, 0, nx);
+        launch_post(s, D, s->step);
         finish_step(s, D);
     }
 }
@@ -1401,7 +1420,7 @@ void cem::dev_mt_E(cem_sim *s, int i0, int i1) {
 
 void cem::dev_mt_post(cem_sim *s) {
     Device *D = s->dev;
-    launch_e(s, D, s->n[0], s->n[0] + 1);   // post-phase edges on the last x plane
+    launch_post(s, D, s->step);
     finish_step(s, D);
 }
 
@@ -1455,16 +1474,9 @@ static float *state_ptr(cem_sim *s, int which, int comp, int w, size_t *len) {
 
 void cem::dev_state_download(cem_sim *s, int which, int comp, int w, double *dst) {
     Device *D = s->dev;
-    if (which == 3) {       // Debye E_old: E at the dispersive edges (refreshed every step)
-        std::vector<double> e(s->N);
-        int last = -1;
-        for (int k = 0; k < s->ndisp; k++) {
-            if (s->d_comp[k] != last) {
-                last = s->d_comp[k];
-                download_f32(D, D->F.p + (size_t)last * s->N, e.data(), s->N);
-            }
-            dst[k] = e[s->d_idx[k]];
-        }
+    if (which == 3) {       // Debye E_old: E of each dispersive edge before the next E update
+        ensure_vlast(s, D);
+        download_f32(D, D->Eold.p, dst, s->ndisp);
         return;
     }
     if (which == 4) {       // SIBC states: device term-major -> reference edge-major
@@ -1482,7 +1494,10 @@ void cem::dev_state_download(cem_sim *s, int which, int comp, int w, double *dst
 
 void cem::dev_state_upload(cem_sim *s, int which, int comp, int w, const double *src) {
     Device *D = s->dev;
-    if (which == 3) return;  // E_old is rewritten from E before every use
+    if (which == 3) {        // E_old is taken from E again before the next E update
+        D->eold_stale = true;
+        return;
+    }
     if (which == 4) {
         const size_t K = s->sibc_npole, n = s->nsb;
         std::vector<double> t(K * n);
@@ -1494,7 +1509,7 @@ void cem::dev_state_upload(cem_sim *s, int which, int comp, int w, const double 
     size_t len = 0;
     float *p = state_ptr(s, which, comp, w, &len);
     if (p) upload_f32(D, p, src, len);
-    if (which == 5 && comp < 3) D->vlast_stale = true;
+    if (which == 5 && comp < 3) D->vlast_stale = D->eold_stale = true;
 }
 
 void cem::dev_sync_in(cem_sim *s) {
@@ -1632,8 +1647,8 @@ void cem::dev_set_step(cem_sim *s, long step) {
 // ---------------------------------------------------------------------------
 extern "C" int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count) {
     if (!s || !s->dev) return -1;
-    KernelTimer *t[4] = {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD};
-    if (which < 0 || which > 3) return -1;
+    KernelTimer *t[5] = {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD, &s->dev->tP};
+    if (which < 0 || which > 4) return -1;
     t[which]->fold();
     *total_ms = t[which]->total_ms;
     *count = t[which]->count;
@@ -1643,7 +1658,7 @@ extern "C" int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, lon
 extern "C" int cem_amd_set_profiling(cem_sim *s, int on) {
     if (!s || !s->finalized || dev_ensure(s)) return -1;
     s->dev->profiling = on != 0;
-    for (KernelTimer *t : {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD}) {
+    for (KernelTimer *t : {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD, &s->dev->tP}) {
         t->fold();
         t->total_ms = 0;
         t->count = 0;

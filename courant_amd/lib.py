@@ -448,7 +448,7 @@ class Sim:
         return self._ok(self.lib.cem_amd_set_profiling(self.h, 1 if on else 0))
 
     def kernel_stats(self, which):
-        """(total GPU ms, launches) of kernel `which`: 0 H update, 1 E update, 2 gather, 3 DFT."""
+        """(total GPU ms, launches) of kernel `which`: 0 H update, 1 E update, 2 gather, 3 DFT, 4 post."""
         t, n = C.c_double(), C.c_long()
         self._ok(self.lib.cem_amd_kernel_stats(self.h, which, C.byref(t), C.byref(n)))
         return t.value, n.value

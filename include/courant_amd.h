@@ -25,7 +25,8 @@ CEM_API int cem_amd_synchronize(cem_sim *s);
 CEM_API int cem_amd_set_profiling(cem_sim *s, int on);
 
 /* Total GPU time (ms) and launch count of one kernel since profiling was
- * switched on: which = 0 H update, 1 E update, 2 probe gather, 3 DFT. */
+ * switched on: which = 0 H update, 1 E update, 2 probe gather, 3 DFT,
+ * 4 post-phase edge physics (Debye, SIBC, sources). */
 CEM_API int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count);
 
 /* Stream-copy bandwidth of the current GPU in GB/s (read + write bytes), best
