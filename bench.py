@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("cfg2", "cfg1"), default="cfg2",
+                    help="cfg2: lossy mode-port GCPW, 4.5 M cells (default); cfg1: lumped GCPW, 120x80x60")
     ap.add_argument("--copy-only", type=float, default=0,
                     help="only run the stream-copy kernel on this many GB (byte-counter calibration)")
     args = ap.parse_args()
@@ -96,7 +98,8 @@ def main():
     from courant_amd import lib, scenes
     cem = lib.Courant()
     t0 = time.perf_counter()
-    sim, info = scenes.gcpw(cem, **scenes.CFG2)
+    cfg = scenes.CFG2 if args.workload == "cfg2" else scenes.CFG1
+    sim, info = scenes.gcpw(cem, **cfg)
     setup_s = time.perf_counter() - t0
     nx, ny, nz = info["cells"]
     cells = nx * ny * nz
@@ -131,7 +134,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "lossy GCPW, mode ports, Debye 6 poles + Huray SIBC", "cells": [nx, ny, nz],
+        "config": {"workload": ("lossy GCPW, mode ports, Debye 6 poles + Huray SIBC" if args.workload == "cfg2"
+                                else "lossless GCPW, lumped ports, CPML on six faces"), "cells": [nx, ny, nz],
                    "grid_points": ab["N"], "nf": int(len(info["freqs"])), "ndisp": ab["ndisp"],
                    "nsibc": ab["nsibc"], "dft": "f64", "setup_s": round(setup_s, 2), "finite": ok},
         "roofline": {"bound": "hbm", "kernel": f"k_update_{dom.lower()}", "achieved": round(achieved, 1),

@@ -114,25 +114,32 @@ struct EArgs {
     int has_pml;
 };
 
-// Post-phase edge lists: threads [0, ndisp) Debye edges, then nsb SIBC edges,
-// then nxs source edges that are not dispersive.  Indices are global E indices
-// comp*N + q.
+// Post-phase edge lists.  One thread per edge runs every step that applies to
+// its edge, in the reference order; an edge can be in several lists (a
+// dielectric edge tangential to a PEC outer face is both dispersive and a
+// surface edge), so the lists are linked: threads [0, ndisp) take the
+// dispersive edges (with their SIBC slot and source slot, if any), then nsbo
+// surface edges that are not dispersive, then nxs remaining source edges.
+// Indices are global E indices comp*N + q.
 struct PostArgs {
     float *E;               // 3N
     const float *H;         // 3N
     long long N, si, sj;
     const PostParams *pp;
-    int ndisp, nsb, nxs;
+    int ndisp, nsb, nsbo, nxs;
     const unsigned *d_idx;
     const float *d_cb;      // cb of each dispersive edge
     const int *d_src;       // source slot of a dispersive edge, -1 if none (nullptr: none at all)
+    const int *d_sb;        // SIBC slot of a dispersive edge, -1 if none (nullptr: none at all)
     float *J;               // npole * ndisp (pole-major, reference order)
     const float *bdt;       // beta/dt, npole * ndisp
     float *Eold;            // E of each dispersive edge before this step's E update
-    const unsigned *sb_idx;
+    const unsigned *sb_idx; // per SIBC slot (reference order)
     const unsigned char *face;
     float *Y;               // nterm * nsb (term-major)
-    const unsigned *x_idx;  // non-dispersive source edges
+    const unsigned *sbo;    // SIBC slots of the surface edges that are not dispersive
+    const int *sb_src;      // source slot per SIBC slot, -1 if none (nullptr: none at all)
+    const unsigned *x_idx;  // source edges neither dispersive nor surface
     const int *x_src;       // their source slots
     const float *wl, *wm;   // lumped cs and modal weight per source slot
     float vs, gm;           // waveform values at tH of this step
@@ -387,15 +394,47 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     }
 }
 
+// surface-impedance overwrite of the E edge of SIBC slot u (sibc.c:256-269):
+// J from up to four H faces, then the recursive-convolution terms
+__device__ __forceinline__ float sibc_edge(const PostArgs &a, const PostParams &pp, int u) {
+    const unsigned q = a.sb_idx[u];
+    const int c = (int)(q / a.N);
+    const long long qq = (long long)q - c * a.N;
+    const int cu = (c + 1) % 3, cv = (c + 2) % 3;
+    const long long su = cu == 0 ? a.si : cu == 1 ? a.sj : 1;
+    const long long sv = cv == 0 ? a.si : cv == 1 ? a.sj : 1;
+    const float *Hu = a.H + cu * a.N, *Hv = a.H + cv * a.N;
+    const unsigned m = a.face[u];
+    const float inv = 1.0f / (float)__builtin_popcount(m);
+    float Js = 0.f;
+    if (m & 1u) Js += -inv * Hu[qq];
+    if (m & 2u) Js += inv * Hu[qq - sv];
+    if (m & 4u) Js += inv * Hv[qq];
+    if (m & 8u) Js += -inv * Hv[qq - su];
+    float es = 0.f;
+    for (int k = 0; k < pp.nterm; k++) {
+        float *y = a.Y + (long long)k * a.nsb + u;
+        float yv = *y;
+        yv += pp.om[k] * (Js - yv);
+        *y = yv;
+        es += pp.zc[k] * (Js - yv);
+    }
+    return es;
+}
+
 __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const PostParams &pp = *a.pp;
-    if (t < a.ndisp) {                       // Debye edge
+    if (t < a.ndisp) {                       // dispersive edge
         const unsigned q = a.d_idx[t];
         float e = a.E[q];
         float acc = 0.f;
         for (int p = 0; p < pp.npole; p++) acc += pp.kc[p] * a.J[(long long)p * a.ndisp + t];
         e -= a.d_cb[t] * acc;
+        if (a.d_sb) {
+            const int u = a.d_sb[t];
+            if (u >= 0) e = sibc_edge(a, pp, u);
+        }
         if (a.d_src) {
             const int x = a.d_src[t];
             if (x >= 0) e += a.wl[x] * a.vs + a.wm[x] * a.gm;
@@ -411,34 +450,18 @@ __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
         return;
     }
     int u = t - a.ndisp;
-    if (u < a.nsb) {                         // surface-impedance edge
-        const unsigned q = a.sb_idx[u];
-        const int c = (int)(q / a.N);
-        const long long qq = (long long)q - c * a.N;
-        const int cu = (c + 1) % 3, cv = (c + 2) % 3;
-        const long long su = cu == 0 ? a.si : cu == 1 ? a.sj : 1;
-        const long long sv = cv == 0 ? a.si : cv == 1 ? a.sj : 1;
-        const float *Hu = a.H + cu * a.N, *Hv = a.H + cv * a.N;
-        const unsigned m = a.face[u];
-        const float inv = 1.0f / (float)__builtin_popcount(m);
-        float Js = 0.f;
-        if (m & 1u) Js += -inv * Hu[qq];
-        if (m & 2u) Js += inv * Hu[qq - sv];
-        if (m & 4u) Js += inv * Hv[qq];
-        if (m & 8u) Js += -inv * Hv[qq - su];
-        float es = 0.f;
-        for (int k = 0; k < pp.nterm; k++) {
-            float *y = a.Y + (long long)k * a.nsb + u;
-            float yv = *y;
-            yv += pp.om[k] * (Js - yv);
-            *y = yv;
-            es += pp.zc[k] * (Js - yv);
+    if (u < a.nsbo) {                        // surface edge without dispersion
+        const int slot = (int)a.sbo[u];
+        float e = sibc_edge(a, pp, slot);
+        if (a.sb_src) {
+            const int x = a.sb_src[slot];
+            if (x >= 0) e += a.wl[x] * a.vs + a.wm[x] * a.gm;
         }
-        a.E[q] = es;
+        a.E[a.sb_idx[slot]] = e;
         return;
     }
-    u -= a.nsb;
-    if (u < a.nxs) {                         // source edge without dispersion
+    u -= a.nsbo;
+    if (u < a.nxs) {                         // source edge only
         const unsigned q = a.x_idx[u];
         const int x = a.x_src[u];
         a.E[q] += a.wl[x] * a.vs + a.wm[x] * a.gm;
@@ -720,8 +743,10 @@ struct Device {
     const float *hs[3] = {}, *es[3] = {};
     Pml pmlH{}, pmlE{};
     // post-phase edge lists
-    DBuf<unsigned> d_idx, sb_idx, x_idx;
-    DBuf<int> d_src, x_src;
+    DBuf<unsigned> d_idx, sb_idx, sbo, x_idx;
+    DBuf<int> d_src, d_sb, sb_src, x_src;
+    int nsbo = 0;
+    bool has_dsb = false, has_sbsrc = false;
     DBuf<float> d_cb, J, bdt, Eold, Y, wl, wm;
     DBuf<unsigned char> face;
     DBuf<PostParams> pp;
@@ -762,6 +787,26 @@ struct Device {
 // ---------------------------------------------------------------------------
 namespace {
 
+// Slot of global E index g in the dispersive / surface edge lists (both sorted by
+// global index, the reference's (comp, i, j, k) order), or -1.
+long dispersive_slot(const cem_sim *s, unsigned long long g) {
+    size_t lo = 0, hi = s->ndisp;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        if ((unsigned long long)s->d_comp[mid] * s->N + s->d_idx[mid] < g) lo = mid + 1; else hi = mid;
+    }
+    return lo < (size_t)s->ndisp && (unsigned long long)s->d_comp[lo] * s->N + s->d_idx[lo] == g ? (long)lo : -1;
+}
+
+long surface_slot(const cem_sim *s, unsigned long long g) {
+    size_t lo = 0, hi = s->nsb;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        if ((unsigned long long)s->sb_comp[mid] * s->N + s->sb_idx[mid] < g) lo = mid + 1; else hi = mid;
+    }
+    return lo < (size_t)s->nsb && (unsigned long long)s->sb_comp[lo] * s->N + s->sb_idx[lo] == g ? (long)lo : -1;
+}
+
 // Excited sources as (global E index, lumped cs, modal weight) slots; edges that
 // are also dispersive get their slot through d_src, the others go to x_idx.
 int upload_sources(cem_sim *s, Device *D) {
@@ -789,21 +834,19 @@ int upload_sources(cem_sim *s, Device *D) {
         wl.push_back((float)e.second.first);
         wm.push_back((float)e.second.second);
     }
-    std::vector<int> d_src(s->ndisp, -1), x_src;
+    std::vector<int> d_src(s->ndisp, -1), sb_src(s->nsb, -1), x_src;
     std::vector<unsigned> x_idx;
-    D->has_dsrc = false;
+    D->has_dsrc = D->has_sbsrc = false;
     for (size_t k = 0; k < where.size(); k++) {
         const unsigned long long g = where[k];
-        // dispersive edges are sorted by global index (reference order c, i, j, k)
-        size_t lo = 0, hi = s->ndisp;
-        while (lo < hi) {
-            size_t mid = (lo + hi) / 2;
-            unsigned long long gm = (unsigned long long)s->d_comp[mid] * s->N + s->d_idx[mid];
-            if (gm < g) lo = mid + 1; else hi = mid;
-        }
-        if (lo < (size_t)s->ndisp && (unsigned long long)s->d_comp[lo] * s->N + s->d_idx[lo] == g) {
-            d_src[lo] = (int)k;
+        const long t = dispersive_slot(s, g);
+        const long u = t < 0 ? surface_slot(s, g) : -1;
+        if (t >= 0) {
+            d_src[t] = (int)k;
             D->has_dsrc = true;
+        } else if (u >= 0) {
+            sb_src[u] = (int)k;
+            D->has_sbsrc = true;
         } else {
             x_idx.push_back((unsigned)g);
             x_src.push_back((int)k);
@@ -814,6 +857,7 @@ int upload_sources(cem_sim *s, Device *D) {
     D->x_idx.release();
     D->x_src.release();
     D->d_src.release();
+    D->sb_src.release();
     if (D->wl.alloc(wl.size()) || D->wm.alloc(wm.size()) || D->x_idx.alloc(x_idx.size()) ||
         D->x_src.alloc(x_src.size()))
         return -1;
@@ -821,6 +865,8 @@ int upload_sources(cem_sim *s, Device *D) {
         D->x_idx.upload(x_idx.data(), x_idx.size(), D->st) || D->x_src.upload(x_src.data(), x_src.size(), D->st))
         return -1;
     if (D->has_dsrc && (D->d_src.alloc(d_src.size()) || D->d_src.upload(d_src.data(), d_src.size(), D->st)))
+        return -1;
+    if (D->has_sbsrc && (D->sb_src.alloc(sb_src.size()) || D->sb_src.upload(sb_src.data(), sb_src.size(), D->st)))
         return -1;
     D->sources_dirty = false;
     return 0;
@@ -1005,9 +1051,23 @@ int upload_plan(cem_sim *s, Device *D) {
     }
     // SIBC tables
     {
-        std::vector<unsigned> idx(s->nsb);
-        for (int e = 0; e < s->nsb; e++) idx[e] = (unsigned)(s->sb_comp[e] * N + s->sb_idx[e]);
+        std::vector<unsigned> idx(s->nsb), sbo;
+        std::vector<int> d_sb(s->ndisp, -1);
+        D->has_dsb = false;
+        for (int e = 0; e < s->nsb; e++) {
+            idx[e] = (unsigned)(s->sb_comp[e] * N + s->sb_idx[e]);
+            const long t = dispersive_slot(s, idx[e]);
+            if (t >= 0) {
+                d_sb[t] = e;
+                D->has_dsb = true;
+            } else {
+                sbo.push_back((unsigned)e);
+            }
+        }
+        D->nsbo = (int)sbo.size();
         if (D->sb_idx.alloc(s->nsb) || D->sb_idx.upload(idx.data(), s->nsb, st)) return -1;
+        if (D->sbo.alloc(sbo.size()) || D->sbo.upload(sbo.data(), sbo.size(), st)) return -1;
+        if (D->has_dsb && (D->d_sb.alloc(s->ndisp) || D->d_sb.upload(d_sb.data(), s->ndisp, st))) return -1;
         const size_t ny_ = (size_t)s->nsb * s->sibc_npole;
         if (D->Y.alloc(ny_) || D->Y.zero(st) || D->face.alloc(s->nsb)) return -1;
         if (D->face.upload(s->sb_mask.data(), s->nsb, st)) return -1;
@@ -1165,7 +1225,7 @@ void launch_e(cem_sim *s, Device *D, int i0, int i1) {
 }
 
 void launch_post(cem_sim *s, Device *D, long step) {
-    const int total = s->ndisp + s->nsb + D->nxs;
+    const int total = s->ndisp + D->nsbo + D->nxs;
     if (!total) return;
     PostArgs a{};
     a.E = D->F.p;
@@ -1176,10 +1236,14 @@ void launch_post(cem_sim *s, Device *D, long step) {
     a.pp = D->pp.p;
     a.ndisp = s->ndisp;
     a.nsb = s->nsb;
+    a.nsbo = D->nsbo;
     a.nxs = D->nxs;
     a.d_idx = D->d_idx.p;
     a.d_cb = D->d_cb.p;
     a.d_src = D->has_dsrc ? D->d_src.p : nullptr;
+    a.d_sb = D->has_dsb ? D->d_sb.p : nullptr;
+    a.sbo = D->sbo.p;
+    a.sb_src = D->has_sbsrc ? D->sb_src.p : nullptr;
     a.J = D->J.p;
     a.bdt = D->bdt.p;
     a.Eold = D->Eold.p;
@@ -1367,8 +1431,10 @@ void cem::dev_destroy(cem_sim *s) {
     D->wm.release();
     D->face.release();
     D->pp.release();
-    for (auto *b : {&D->d_idx, &D->sb_idx, &D->x_idx}) b->release();
+    for (auto *b : {&D->d_idx, &D->sb_idx, &D->sbo, &D->x_idx}) b->release();
     D->d_src.release();
+    D->d_sb.release();
+    D->sb_src.release();
     D->x_src.release();
     D->d_cb.release();
     D->Eold.release();
