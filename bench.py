@@ -6,13 +6,20 @@ copper meshed with 3 cells, SIBC copper (5.8e7 S/m, Huray 0.5 um, ratio 2),
 eigenmode ports, 401 DFT frequencies, CPML on five faces; FP32 fields, FP64 DFT.
 A "step" is one full leapfrog step (H, E with CPML / Debye / SIBC / sources,
 probe gather, amortised DFT) of that board, with every input resident in HBM.
+The timed steps run as the engine runs them (chunks replayed from a captured
+HIP graph); per-kernel durations come from a second, event-bracketed pass of
+direct launches of the same kernels.
 
 Prints one JSON line; `value` is Mcell-updates/s = nx*ny*nz*steps / wall.
 """
 import argparse
+import csv
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -74,12 +81,62 @@ def cpu_baseline(target_s=12.0):
             "sample": f"reference f64 OpenMP core, same board with lumped ports, {n} steps ({wall:.1f} s)"}
 
 
+def _pmc_pass(counter, args_tail, tmp, tag):
+    """One rocprofv3 counter pass over a child run of this script; mean counter value per
+    dispatch, by kernel name (template arguments and parameter lists dropped)."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    d = os.path.join(tmp, f"{tag}_{counter}")
+    cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc",
+           "--", sys.executable, os.path.abspath(__file__)] + args_tail
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=tmp)
+    path = os.path.join(d, "pmc_counter_collection.csv")
+    if r.returncode != 0 or not os.path.exists(path):
+        return None
+    acc = {}
+    for row in csv.DictReader(open(path)):
+        name = row["Kernel_Name"].split("(")[0].replace("void ", "").split("<")[0]
+        acc.setdefault(name, []).append(float(row["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def pmc_traffic(workload):
+    """HBM bytes per dispatch of each stepper kernel: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch, one counter per rocprofv3 run), each scaled by what the same counter reads for
+    the plain copy kernel moving 5e8 B in and 5e8 B out per dispatch."""
+    tmp = tempfile.mkdtemp(prefix="courant_pmc_", dir="/tmp")
+    try:
+        out = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            run = _pmc_pass(counter, ["--kernel-pass", "--workload", workload, "--steps", "8", "--warmup", "2"],
+                            tmp, "step")
+            cal = _pmc_pass(counter, ["--copy-only", "0.5"], tmp, "copy")
+            if not run or not cal or not cal.get("cem::k_copy"):
+                return None
+            scale = 0.5e9 / 1024 / cal["cem::k_copy"]
+            out[counter] = {"scale": scale, "kib": run}
+        kernels = {}
+        for k in ("cem::k_update_h", "cem::k_update_e", "cem::k_post", "cem::k_gather"):
+            f = out["FETCH_SIZE"]["kib"].get(k)
+            w = out["WRITE_SIZE"]["kib"].get(k)
+            if f is not None and w is not None:
+                kernels[k] = (f * out["FETCH_SIZE"]["scale"] + w * out["WRITE_SIZE"]["scale"]) * 1024
+        return {"kernels": kernels, "fetch_scale": round(out["FETCH_SIZE"]["scale"], 3),
+                "write_scale": round(out["WRITE_SIZE"]["scale"], 3)}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 passes that measure HBM traffic")
+    ap.add_argument("--kernel-pass", action="store_true",
+                    help="only set up the workload and run warmup + steps (the child run of a counter pass)")
     ap.add_argument("--workload", choices=("cfg2", "cfg1"), default="cfg2",
                     help="cfg2: lossy mode-port GCPW, 4.5 M cells (default); cfg1: lumped GCPW, 120x80x60")
     ap.add_argument("--copy-only", type=float, default=0,
@@ -105,11 +162,25 @@ def main():
     cells = nx * ny * nz
     sim.step_device(args.warmup)
     sim.synchronize()
-    sim.set_profiling(True)
+    if args.kernel_pass:
+        sim.set_profiling(True)   # direct launches: one dispatch per kernel and step
+        sim.step_device(args.steps)
+        sim.synchronize()
+        sim.close()
+        return
+    # timed region: the production path (runs of steps replayed from a captured graph)
     t0 = time.perf_counter()
     sim.step_device(args.steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
+    # per-kernel durations: a separate pass of direct launches bracketed by HIP events
+    # on the engine's stream (same kernels and arguments as the graph)
+    nprof = min(args.steps, 512)
+    sim.set_profiling(True)
+    t1 = time.perf_counter()
+    sim.step_device(nprof)
+    sim.synchronize()
+    wall_direct = time.perf_counter() - t1
     stats = {k: sim.kernel_stats(i) for i, k in enumerate(("H", "E", "gather", "dft", "post"))}
     sim.set_profiling(False)
     ab = algorithmic_bytes(sim)
@@ -145,8 +216,18 @@ def main():
                      "measured_copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs > 0 else None},
         "kernels_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in stats.items()},
+        "algorithmic_mb": {k: round(ab[k] / 1e6, 1) for k in ("H", "E", "post")},
+        "launch": {"timed": "graph replay", "ms_per_step_direct_launches": round(wall_direct / nprof * 1e3, 5),
+                   "profiled_steps": nprof},
     }
     sim.close()
+    if not args.no_pmc:
+        tr = pmc_traffic(args.workload)
+        if tr:
+            kname = {"H": "cem::k_update_h", "E": "cem::k_update_e"}[dom]
+            out["roofline"]["traffic"] = round(tr["kernels"].get(kname, 0.0))
+            out["roofline"]["traffic_by_kernel"] = {k: round(v) for k, v in tr["kernels"].items()}
+            out["roofline"]["pmc_scale"] = {"FETCH_SIZE": tr["fetch_scale"], "WRITE_SIZE": tr["write_scale"]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -14,21 +14,26 @@
 //                update (dispersive.c:102-112); one thread per listed edge, the
 //                lists in the reference's own (comp, i, j, k) order;
 //   k_gather   : port / probe line and loop integrals, one wavefront per
-//                sample, FP64 sums reduced with fixed-order shuffles, written
-//                to a per-step sample series;
+//                sample in one workgroup, FP64 sums reduced with fixed-order
+//                shuffles, written to a per-step sample series; it then
+//                advances the device step and series-row counters;
 //   k_dft      : every chunk of steps, the FP64 running DFT of all samples
 //                (sim.c:840-861, export.c:121-149), one thread per
 //                (accumulator, frequency), phasors resynchronised every 1024
 //                steps and rotated in between exactly as the reference does.
 // Fields are component-major FP32 arrays with the reference index
-// i*si + j*sj + k, so every index list of the plan applies unchanged.
+// i*si + j*sj + k, so every index list of the plan applies unchanged.  With
+// the step and series row held on the device, every launch argument of a step
+// is constant, and runs of steps are replayed from a captured HIP graph.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
 #include <chrono>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -42,6 +47,8 @@ namespace cem {
 constexpr int kBlock = 256;        // 4 wavefronts of 64
 constexpr int kIseg = 2;           // default i-planes per thread in the stencil kernels
 constexpr int kSeriesCap = 512;    // steps recorded between DFT flushes
+constexpr int kGraphSteps = 64;
+constexpr int kGatherBlock = 1024;    // steps per captured chunk (divides kSeriesCap)
 
 #define HIP_OK(expr)                                                                  \
     do {                                                                              \
@@ -114,6 +121,34 @@ struct EArgs {
     int has_pml;
 };
 
+// Device-side step and series-row counters: every per-step kernel argument is
+// then constant, so a chunk of steps can be captured once and replayed.
+struct Ctl {
+    long long step;          // step about to complete (sets the source waveform time)
+    int row;                 // series row of this step's samples
+    unsigned pad;
+};
+
+struct Waveform {            // excitation parameters (sim.c:834-838, modeport.c:124-128)
+    int has_lumped, has_mode;
+    double amp, t0, tau, mt0, mtau, dt;
+};
+
+__device__ __forceinline__ void waveforms(const Waveform &w, long long n, float *vs, float *gm) {
+    const double tH = ((double)n + 0.5) * w.dt;
+    double a = 0, b = 0;
+    if (w.has_lumped) {
+        const double x = (tH - w.t0) / w.tau;
+        a = w.amp * exp(-x * x);
+    }
+    if (w.has_mode) {
+        const double x = (tH - w.mt0) / w.mtau;
+        b = -2 * x * exp(-x * x);
+    }
+    *vs = (float)a;
+    *gm = (float)b;
+}
+
 // Post-phase edge lists.  One thread per edge runs every step that applies to
 // its edge, in the reference order; an edge can be in several lists (a
 // dielectric edge tangential to a PEC outer face is both dispersive and a
@@ -128,11 +163,12 @@ struct PostArgs {
     const PostParams *pp;
     int ndisp, nsb, nsbo, nxs;
     const unsigned *d_idx;
-    const float *d_cb;      // cb of each dispersive edge
+    const unsigned *d_mat;  // material of each dispersive edge: row of dtab
+    const float *dtab;      // per material: cb, then beta/dt per pole
+    int dstride;            // 1 + npole
     const int *d_src;       // source slot of a dispersive edge, -1 if none (nullptr: none at all)
     const int *d_sb;        // SIBC slot of a dispersive edge, -1 if none (nullptr: none at all)
     float *J;               // npole * ndisp (pole-major, reference order)
-    const float *bdt;       // beta/dt, npole * ndisp
     float *Eold;            // E of each dispersive edge before this step's E update
     const unsigned *sb_idx; // per SIBC slot (reference order)
     const unsigned char *face;
@@ -142,7 +178,8 @@ struct PostArgs {
     const unsigned *x_idx;  // source edges neither dispersive nor surface
     const int *x_src;       // their source slots
     const float *wl, *wm;   // lumped cs and modal weight per source slot
-    float vs, gm;           // waveform values at tH of this step
+    Waveform wave;
+    Ctl *ctl;
 };
 
 struct GatherArgs {
@@ -153,8 +190,9 @@ struct GatherArgs {
     const double *post;     // per-sample factor applied after the sum
     int nsamp, nip, nport, nvp;
     double *vlast;          // port V after the previous E update
-    double *row_ip, *row_vold, *row_vnew, *row_vp;   // series row of this step
+    double *ser_ip, *ser_vold, *ser_vnew, *ser_vp;   // sample series, row-major
     int mid_only;           // 1: only refresh vlast (V before the E update)
+    Ctl *ctl;
 };
 
 struct DftArgs {
@@ -171,6 +209,7 @@ struct DftArgs {
     double amp, t0, tau;    // lumped Gaussian (sim.c:834-838)
     double mt0, mtau;       // modal differentiated Gaussian (modeport.c:124-128)
     double *acc;            // nacc * 2nf
+    Ctl *ctl;               // series row counter, reset after the flush
 };
 
 // ---------------------------------------------------------------------------
@@ -345,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     e -= cbv * v;
                 }
             }
-            if (e != e0) Ex[q] = e;
+            Ex[q] = e;
         }
         if (j < g.ny) {     // Ey
             const float e0 = Ey[q], cbv = cy2.y;
@@ -366,7 +405,7 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     e += cbv * v;
                 }
             }
-            if (e != e0) Ey[q] = e;
+            Ey[q] = e;
         }
         if (k < g.nz) {     // Ez
             const float e0 = Ez[q], cbv = cz2.y;
@@ -387,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     e -= cbv * v;
                 }
             }
-            if (e != e0) Ez[q] = e;
+            Ez[q] = e;
         }
         hy_m = hy;
         hz_m = hz;
@@ -422,81 +461,103 @@ __device__ __forceinline__ float sibc_edge(const PostArgs &a, const PostParams &
     return es;
 }
 
+// Probe line and loop integrals of one step (one wavefront per sample, FP64 sums
+// reduced with fixed-order shuffles) written to the sample series row.
+__device__ void gather_samples(const GatherArgs &a, int row, int wave0, int nwave) {
+    const int lane = threadIdx.x & 63;
+    for (int w = wave0; w < a.nsamp; w += nwave) {
+        const bool is_port = w >= a.nip && w < a.nip + a.nport;
+        if (a.mid_only && !is_port) continue;
+        double acc = 0.0;
+        for (int t = a.toff[w] + lane; t < a.toff[w + 1]; t += 64) {
+            acc += (double)a.F[a.tidx[t]] * a.tw[t];
+        }
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane != 0) continue;
+        const double v = acc * a.post[w];
+        if (w < a.nip) {
+            a.ser_ip[(long long)row * a.nip + w] = v;
+        } else if (is_port) {
+            const int pp = w - a.nip;
+            if (!a.mid_only) {
+                a.ser_vold[(long long)row * a.nport + pp] = a.vlast[pp];
+                a.ser_vnew[(long long)row * a.nport + pp] = v;
+            }
+            a.vlast[pp] = v;
+        } else {
+            a.ser_vp[(long long)row * a.nvp + (w - a.nip - a.nport)] = v;
+        }
+    }
+}
+
+// Post phase of one step: Debye, SIBC and source edges.
 __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const PostParams &pp = *a.pp;
+    const long long n = a.ctl->step;
+    float vs, gm;
+    waveforms(a.wave, n, &vs, &gm);
     if (t < a.ndisp) {                       // dispersive edge
         const unsigned q = a.d_idx[t];
+        const float *mat = a.dtab + a.d_mat[t] * a.dstride;
         float e = a.E[q];
         float acc = 0.f;
         for (int p = 0; p < pp.npole; p++) acc += pp.kc[p] * a.J[(long long)p * a.ndisp + t];
-        e -= a.d_cb[t] * acc;
+        e -= mat[0] * acc;
         if (a.d_sb) {
             const int u = a.d_sb[t];
             if (u >= 0) e = sibc_edge(a, pp, u);
         }
         if (a.d_src) {
             const int x = a.d_src[t];
-            if (x >= 0) e += a.wl[x] * a.vs + a.wm[x] * a.gm;
+            if (x >= 0) e += a.wl[x] * vs + a.wm[x] * gm;
         }
         const float de = e - a.Eold[t];
         for (int p = 0; p < pp.npole; p++) {
             const long long o = (long long)p * a.ndisp + t;
             const float jv = a.J[o];
-            a.J[o] = jv + pp.km1[p] * jv + a.bdt[o] * de;
+            a.J[o] = jv + pp.km1[p] * jv + mat[1 + p] * de;
         }
         a.E[q] = e;
         a.Eold[t] = e;                       // E of this edge before the next E update
-        return;
-    }
-    int u = t - a.ndisp;
-    if (u < a.nsbo) {                        // surface edge without dispersion
-        const int slot = (int)a.sbo[u];
+    } else if (t - a.ndisp < a.nsbo) {       // surface edge without dispersion
+        const int slot = (int)a.sbo[t - a.ndisp];
         float e = sibc_edge(a, pp, slot);
         if (a.sb_src) {
             const int x = a.sb_src[slot];
-            if (x >= 0) e += a.wl[x] * a.vs + a.wm[x] * a.gm;
+            if (x >= 0) e += a.wl[x] * vs + a.wm[x] * gm;
         }
         a.E[a.sb_idx[slot]] = e;
-        return;
-    }
-    u -= a.nsbo;
-    if (u < a.nxs) {                         // source edge only
+    } else if (t - a.ndisp - a.nsbo < a.nxs) {   // source edge only
+        const int u = t - a.ndisp - a.nsbo;
         const unsigned q = a.x_idx[u];
         const int x = a.x_src[u];
-        a.E[q] += a.wl[x] * a.vs + a.wm[x] * a.gm;
+        a.E[q] += a.wl[x] * vs + a.wm[x] * gm;
     }
+}
+
+// The step's samples (one workgroup), then the advance of the device step and
+// series-row counters, once every wave has read them.
+__global__ void __launch_bounds__(kGatherBlock) k_gather(GatherArgs a) {
+    const int row = a.ctl->row;
+    gather_samples(a, row, threadIdx.x >> 6, kGatherBlock / 64);
+    if (a.mid_only) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.ctl->row = row + 1;
+        a.ctl->step += 1;
+    }
+}
+
+__global__ void k_set_ctl(Ctl *c, long long step, int row) {
+    c->step = step;
+    c->row = row;
 }
 
 // E of every dispersive edge, taken as E_old of the next E update
 __global__ void __launch_bounds__(kBlock) k_prime_eold(const float *E, const unsigned *d_idx, float *Eold, int n) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t < n) Eold[t] = E[d_idx[t]];
-}
-
-__global__ void __launch_bounds__(kBlock) k_gather(GatherArgs a) {
-    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (wave >= a.nsamp) return;
-    const bool is_port = wave >= a.nip && wave < a.nip + a.nport;
-    if (a.mid_only && !is_port) return;
-    double acc = 0.0;
-    for (int t = a.toff[wave] + lane; t < a.toff[wave + 1]; t += 64) acc += (double)a.F[a.tidx[t]] * a.tw[t];
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane != 0) return;
-    const double v = acc * a.post[wave];
-    if (wave < a.nip) {
-        a.row_ip[wave] = v;
-    } else if (is_port) {
-        const int pp = wave - a.nip;
-        if (!a.mid_only) {
-            a.row_vold[pp] = a.vlast[pp];
-            a.row_vnew[pp] = v;
-        }
-        a.vlast[pp] = v;
-    } else {
-        a.row_vp[wave - a.nip - a.nport] = v;
-    }
 }
 
 __device__ __forceinline__ void dft_acc(double *acc, double v, double pr, double pi) {
@@ -575,6 +636,7 @@ __global__ void k_dft(DftArgs a) {
         hi = ni;
     }
     if (dst) { dst[0] = acc[0]; dst[1] = acc[1]; }
+    if (t == 0) a.ctl->row = 0;   // the series has been consumed
     if (kind == 5) {
         if (a.direct) {   // leave the phasors where the next step expects them (export.c:146-148)
             const long long n = a.step0 + a.rows;
@@ -747,9 +809,14 @@ struct Device {
     DBuf<int> d_src, d_sb, sb_src, x_src;
     int nsbo = 0;
     bool has_dsb = false, has_sbsrc = false;
-    DBuf<float> d_cb, J, bdt, Eold, Y, wl, wm;
+    DBuf<float> J, dtab, Eold, Y, wl, wm;
+    DBuf<unsigned> d_mat;
+    int dstride = 1;
     DBuf<unsigned char> face;
     DBuf<PostParams> pp;
+    DBuf<Ctl> ctl;
+    long long ctl_step = -1;     // what ctl holds once the queued work has run
+    int ctl_row = -1;
     int nxs = 0;
     bool has_dsrc = false;
     bool eold_stale = false;
@@ -778,6 +845,10 @@ struct Device {
     // timing
     bool profiling = false;
     KernelTimer tH, tE, tG, tD, tP;
+    // captured chunk of kGraphSteps steps, valid while its launch arguments are unchanged
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned char> gkey;
 };
 
 }  // namespace cem
@@ -1018,20 +1089,31 @@ int upload_plan(cem_sim *s, Device *D) {
             set_error("grid too large for 32-bit edge indices");
             return -1;
         }
-        std::vector<unsigned> idx(s->ndisp);
-        std::vector<float> cbd(s->ndisp);
+        // Dispersive edges share a handful of materials: (cb, beta_p / dt) rows of
+        // a table, found by exact FP32 value, replace 4 + 4P bytes per edge.
+        const int np = s->npole;
+        std::vector<unsigned> idx(s->ndisp), mat(s->ndisp);
+        std::vector<float> tab;
+        std::map<std::vector<float>, unsigned> rows;
+        std::vector<float> key(1 + np);
         for (int e = 0; e < s->ndisp; e++) {
             idx[e] = (unsigned)(s->d_comp[e] * N + s->d_idx[e]);
-            cbd[e] = (float)s->d_cb[e];
+            key[0] = (float)s->d_cb[e];
+            for (int p = 0; p < np; p++) key[1 + p] = (float)(s->d_beta[(size_t)p * s->ndisp + e] * (1.0 / s->dt));
+            auto it = rows.find(key);
+            if (it == rows.end()) {
+                it = rows.emplace(key, (unsigned)rows.size()).first;
+                tab.insert(tab.end(), key.begin(), key.end());
+            }
+            mat[e] = it->second;
         }
+        D->dstride = 1 + np;
         if (D->d_idx.alloc(s->ndisp) || D->d_idx.upload(idx.data(), s->ndisp, st)) return -1;
-        if (D->d_cb.alloc(s->ndisp) || D->d_cb.upload(cbd.data(), s->ndisp, st)) return -1;
+        if (D->d_mat.alloc(s->ndisp) || D->d_mat.upload(mat.data(), s->ndisp, st)) return -1;
+        if (D->dtab.alloc(tab.size() + 1) || D->dtab.upload(tab.data(), tab.size(), st)) return -1;
         if (D->Eold.alloc(s->ndisp) || D->Eold.zero(st)) return -1;
-        const size_t nj = (size_t)s->npole * s->ndisp;
-        if (D->J.alloc(nj) || D->J.zero(st) || D->bdt.alloc(nj)) return -1;
-        std::vector<float> b(nj);
-        for (size_t q = 0; q < nj; q++) b[q] = (float)(s->d_beta[q] * (1.0 / s->dt));
-        if (D->bdt.upload(b.data(), nj, st)) return -1;
+        const size_t nj = (size_t)np * s->ndisp;
+        if (D->J.alloc(nj) || D->J.zero(st)) return -1;
         PostParams pp{};
         pp.ndisp = s->ndisp;
         pp.npole = s->npole;
@@ -1048,6 +1130,9 @@ int upload_plan(cem_sim *s, Device *D) {
             pp.om[t] = (float)(-std::expm1(-s->sibc_a[t] * s->dt));
         }
         if (D->pp.alloc(1) || D->pp.upload(&pp, 1, st)) return -1;
+        if (D->ctl.alloc(1) || D->ctl.zero(st)) return -1;
+        D->ctl_step = 0;
+        D->ctl_row = 0;
     }
     // SIBC tables
     {
@@ -1155,7 +1240,8 @@ int upload_plan(cem_sim *s, Device *D) {
 }
 
 HArgs h_args(cem_sim *s, Device *D) {
-    HArgs a{};
+    HArgs a;
+    std::memset(&a, 0, sizeof a);   // padding too: the bytes key the captured graph
     a.g = D->g;
     a.E = D->F.p;
     a.H = D->F.p + 3 * D->g.N;
@@ -1168,8 +1254,9 @@ HArgs h_args(cem_sim *s, Device *D) {
     return a;
 }
 
-EArgs e_args(cem_sim *s, Device *D, long step) {
-    EArgs a{};
+EArgs e_args(cem_sim *s, Device *D) {
+    EArgs a;
+    std::memset(&a, 0, sizeof a);
     a.g = D->g;
     a.E = D->F.p;
     a.H = D->F.p + 3 * D->g.N;
@@ -1185,8 +1272,14 @@ EArgs e_args(cem_sim *s, Device *D, long step) {
     return a;
 }
 
-void timed_begin(Device *D, KernelTimer &t) {
-    if (!D->profiling) return;
+// Kernel launch on the engine stream; when profiling, the launch carries a start /
+// stop event pair that the runtime stamps with the dispatch's own begin and end.
+template <typename F, typename... A>
+void launch(Device *D, KernelTimer &t, F kernel, dim3 grid, dim3 block, A... args) {
+    if (!D->profiling) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, D->st, args...);
+        return;
+    }
     if (t.used == 4096) t.fold();
     if (t.used == t.pool.size()) {
         hipEvent_t a, b;
@@ -1194,12 +1287,7 @@ void timed_begin(Device *D, KernelTimer &t) {
         HIP_CHECK_VOID(hipEventCreate(&b));
         t.pool.push_back({a, b});
     }
-    HIP_CHECK_VOID(hipEventRecord(t.pool[t.used].first, D->st));
-}
-
-void timed_end(Device *D, KernelTimer &t) {
-    if (!D->profiling) return;
-    HIP_CHECK_VOID(hipEventRecord(t.pool[t.used].second, D->st));
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, D->st, t.pool[t.used].first, t.pool[t.used].second, 0, args...);
     t.used++;
 }
 
@@ -1207,27 +1295,24 @@ void launch_h(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     HArgs a = h_args(s, D);
     const int npblk = (D->g.S + kBlock - 1) / kBlock, nseg = (i1 - i0 + D->iseg_h - 1) / D->iseg_h;
-    timed_begin(D, D->tH);
-    hipLaunchKernelGGL(k_update_h, dim3(npblk * nseg), dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_h, npblk);
-    timed_end(D, D->tH);
+    launch(D, D->tH, k_update_h, dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_h, npblk);
 }
 
 void launch_e(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
-    EArgs a = e_args(s, D, s->step);
+    EArgs a = e_args(s, D);
     const int npblk = (D->g.S + kBlock - 1) / kBlock, nseg = (i1 - i0 + D->iseg_e - 1) / D->iseg_e;
-    timed_begin(D, D->tE);
     if (D->compressed)
-        hipLaunchKernelGGL(k_update_e<true>, dim3(npblk * nseg), dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_e, npblk);
+        launch(D, D->tE, k_update_e<true>, dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_e, npblk);
     else
-        hipLaunchKernelGGL(k_update_e<false>, dim3(npblk * nseg), dim3(kBlock), 0, D->st, a, i0, i1, D->iseg_e, npblk);
-    timed_end(D, D->tE);
+        launch(D, D->tE, k_update_e<false>, dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_e, npblk);
 }
 
-void launch_post(cem_sim *s, Device *D, long step) {
-    const int total = s->ndisp + D->nsbo + D->nxs;
-    if (!total) return;
-    PostArgs a{};
+void launch_dft(cem_sim *s, Device *D, int direct);
+
+PostArgs post_args(cem_sim *s, Device *D) {
+    PostArgs a;
+    std::memset(&a, 0, sizeof a);
     a.E = D->F.p;
     a.H = D->F.p + 3 * D->g.N;
     a.N = D->g.N;
@@ -1239,13 +1324,14 @@ void launch_post(cem_sim *s, Device *D, long step) {
     a.nsbo = D->nsbo;
     a.nxs = D->nxs;
     a.d_idx = D->d_idx.p;
-    a.d_cb = D->d_cb.p;
+    a.d_mat = D->d_mat.p;
+    a.dtab = D->dtab.p;
+    a.dstride = D->dstride;
     a.d_src = D->has_dsrc ? D->d_src.p : nullptr;
     a.d_sb = D->has_dsb ? D->d_sb.p : nullptr;
     a.sbo = D->sbo.p;
     a.sb_src = D->has_sbsrc ? D->sb_src.p : nullptr;
     a.J = D->J.p;
-    a.bdt = D->bdt.p;
     a.Eold = D->Eold.p;
     a.sb_idx = D->sb_idx.p;
     a.face = D->face.p;
@@ -1254,12 +1340,69 @@ void launch_post(cem_sim *s, Device *D, long step) {
     a.x_src = D->x_src.p;
     a.wl = D->wl.p;
     a.wm = D->wm.p;
-    const double tH = (step + 0.5) * s->dt;
-    a.vs = (float)lumped_waveform(s, tH);
-    a.gm = (float)modeport_waveform(s, tH);
-    timed_begin(D, D->tP);
-    hipLaunchKernelGGL(k_post, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, a);
-    timed_end(D, D->tP);
+    a.wave.has_lumped = s->src_port >= 0;
+    a.wave.has_mode = s->msrc >= 0;
+    a.wave.amp = s->src_amp;
+    a.wave.t0 = s->src_t0;
+    a.wave.tau = s->src_tau;
+    a.wave.mt0 = s->msrc_t0;
+    a.wave.mtau = s->msrc_tau;
+    a.wave.dt = s->dt;
+    a.ctl = D->ctl.p;
+    return a;
+}
+
+GatherArgs gather_args(cem_sim *s, Device *D, int mid_only) {
+    GatherArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.F = D->F.p;
+    a.tidx = D->tidx.p;
+    a.tw = D->tw.p;
+    a.toff = D->toff.p;
+    a.post = D->post.p;
+    a.nsamp = D->nsamp;
+    a.nip = (int)s->ip.size();
+    a.nport = (int)s->port.size();
+    a.nvp = (int)s->vp.size();
+    a.vlast = D->vlast.p;
+    a.ser_ip = D->ser_ip;
+    a.ser_vold = D->ser_vold;
+    a.ser_vnew = D->ser_vnew;
+    a.ser_vp = D->ser_vp;
+    a.mid_only = mid_only;
+    a.ctl = D->ctl.p;
+    return a;
+}
+
+int post_blocks(cem_sim *s, Device *D) {
+    return (s->ndisp + D->nsbo + D->nxs + kBlock - 1) / kBlock;
+}
+
+// The device counters follow the host's step and row; after a host-side change
+// of either (resume, sample push, set-up) they are written before the next step.
+void sync_ctl(cem_sim *s, Device *D) {
+    if (D->ctl_step == s->step && D->ctl_row == D->rows) return;
+    hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, D->st, D->ctl.p, (long long)s->step, D->rows);
+    D->ctl_step = s->step;
+    D->ctl_row = D->rows;
+}
+
+// Post phase and probe samples of one step, then the host's step / row
+// bookkeeping and the DFT flush when the series is full.
+void launch_post(cem_sim *s, Device *D) {
+    sync_ctl(s, D);
+    if (const int nb = post_blocks(s, D)) {
+        PostArgs a = post_args(s, D);
+        launch(D, D->tP, k_post, dim3(nb), dim3(kBlock), a);
+    }
+    GatherArgs g = gather_args(s, D, 0);
+    launch(D, D->tG, k_gather, dim3(1), dim3(kGatherBlock), g);
+    if (D->rows == 0) D->row0 = s->step;
+    D->rows++;
+    s->step++;
+    D->ctl_step = s->step;
+    D->ctl_row = D->rows;
+    if (D->rows == kSeriesCap) launch_dft(s, D, 0);
 }
 
 void launch_dft(cem_sim *s, Device *D, int direct) {
@@ -1292,52 +1435,25 @@ void launch_dft(cem_sim *s, Device *D, int direct) {
     a.mt0 = s->msrc_t0;
     a.mtau = s->msrc_tau;
     a.acc = D->acc.p;
+    a.ctl = D->ctl.p;
     const int nt = a.nf * (a.nacc + 1);
-    timed_begin(D, D->tD);
-    hipLaunchKernelGGL(k_dft, dim3((nt + 127) / 128), dim3(128), 0, D->st, a);
-    timed_end(D, D->tD);
+    launch(D, D->tD, k_dft, dim3((nt + 127) / 128), dim3(128), a);
     D->ph_cur = 1 - D->ph_cur;
     D->row0 += D->rows;
     D->rows = 0;
+    D->ctl_row = 0;
 }
 
-void launch_gather(cem_sim *s, Device *D, int mid_only) {
+// V of every port before the E update (after a host write of E or a resume)
+void launch_gather_mid(cem_sim *s, Device *D) {
     if (!D->nsamp) return;
-    GatherArgs a{};
-    a.F = D->F.p;
-    a.tidx = D->tidx.p;
-    a.tw = D->tw.p;
-    a.toff = D->toff.p;
-    a.post = D->post.p;
-    a.nsamp = D->nsamp;
-    a.nip = (int)s->ip.size();
-    a.nport = (int)s->port.size();
-    a.nvp = (int)s->vp.size();
-    a.vlast = D->vlast.p;
-    const int r = D->rows;
-    a.row_ip = D->ser_ip + (size_t)r * a.nip;
-    a.row_vold = D->ser_vold + (size_t)r * a.nport;
-    a.row_vnew = D->ser_vnew + (size_t)r * a.nport;
-    a.row_vp = D->ser_vp + (size_t)r * a.nvp;
-    a.mid_only = mid_only;
-    const int threads = D->nsamp * 64;
-    timed_begin(D, D->tG);
-    hipLaunchKernelGGL(k_gather, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, a);
-    timed_end(D, D->tG);
-}
-
-// post phase of one step: record samples, advance the step counter, flush the DFT when full
-void finish_step(cem_sim *s, Device *D) {
-    if (D->rows == 0) D->row0 = s->step;
-    launch_gather(s, D, 0);
-    D->rows++;
-    s->step++;
-    if (D->rows == kSeriesCap) launch_dft(s, D, 0);
+    GatherArgs a = gather_args(s, D, 1);
+    hipLaunchKernelGGL(k_gather, dim3(1), dim3(kGatherBlock), 0, D->st, a);
 }
 
 int ensure_vlast(cem_sim *s, Device *D) {
     if (D->vlast_stale) {
-        launch_gather(s, D, 1);
+        launch_gather_mid(s, D);
         D->vlast_stale = false;
     }
     if (D->eold_stale && s->ndisp) {
@@ -1369,6 +1485,76 @@ void upload_f32(Device *D, float *dst, const double *src, size_t n) {
     HIP_CHECK_VOID(hipStreamSynchronize(D->st));
 }
 
+void drop_graph(Device *D) {
+    if (D->gexec) (void)hipGraphExecDestroy(D->gexec);
+    D->gexec = nullptr;
+    D->gkey.clear();
+}
+
+// kGraphSteps full steps (H, E, post + samples) replayed from one captured graph.
+// Every argument of these launches is fixed for a given plan and source set (the
+// step and series row live on the device), so the graph is captured once and
+// captured again only when an argument changes.  The DFT flush is launched by
+// the host after the chunk that fills the series.
+int run_chunk(cem_sim *s, Device *D) {
+    sync_ctl(s, D);
+    const int nx = s->n[0];
+    const HArgs ha = h_args(s, D);
+    const EArgs ea = e_args(s, D);
+    const PostArgs pa = post_args(s, D);
+    const GatherArgs ga = gather_args(s, D, 0);
+    const int geo[4] = {D->iseg_h, D->iseg_e, post_blocks(s, D), (int)D->compressed};
+    std::vector<unsigned char> key;
+    for (auto pr : {std::make_pair((const void *)&ha, sizeof ha), std::make_pair((const void *)&ea, sizeof ea),
+                    std::make_pair((const void *)&pa, sizeof pa), std::make_pair((const void *)&ga, sizeof ga),
+                    std::make_pair((const void *)geo, sizeof geo)}) {
+        const unsigned char *b = (const unsigned char *)pr.first;
+        key.insert(key.end(), b, b + pr.second);
+    }
+    if (!D->gexec || key != D->gkey) {
+        drop_graph(D);
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(D->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            D->use_graph = false;
+            return -1;
+        }
+        const int npblk = (D->g.S + kBlock - 1) / kBlock;
+        const int gh = npblk * ((nx + 1 + D->iseg_h - 1) / D->iseg_h);
+        const int ge = npblk * ((nx + D->iseg_e - 1) / D->iseg_e);
+        for (int k = 0; k < kGraphSteps; k++) {
+            hipLaunchKernelGGL(k_update_h, dim3(gh), dim3(kBlock), 0, D->st, ha, 0, nx + 1, D->iseg_h, npblk);
+            if (D->compressed)
+                hipLaunchKernelGGL(k_update_e<true>, dim3(ge), dim3(kBlock), 0, D->st, ea, 0, nx, D->iseg_e, npblk);
+            else
+                hipLaunchKernelGGL(k_update_e<false>, dim3(ge), dim3(kBlock), 0, D->st, ea, 0, nx, D->iseg_e, npblk);
+            if (geo[2]) hipLaunchKernelGGL(k_post, dim3(geo[2]), dim3(kBlock), 0, D->st, pa);
+            hipLaunchKernelGGL(k_gather, dim3(1), dim3(kGatherBlock), 0, D->st, ga);
+        }
+        const hipError_t ce = hipStreamEndCapture(D->st, &graph);
+        if (ce != hipSuccess || !graph || hipGraphInstantiate(&D->gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            D->gexec = nullptr;
+            D->use_graph = false;   // fall back to direct launches for this simulation
+            return -1;
+        }
+        (void)hipGraphDestroy(graph);
+        D->gkey.swap(key);
+    }
+    if (hipGraphLaunch(D->gexec, D->st) != hipSuccess) {
+        set_error("graph launch failed");
+        drop_graph(D);
+        D->use_graph = false;
+        return -1;
+    }
+    if (D->rows == 0) D->row0 = s->step;
+    D->rows += kGraphSteps;
+    s->step += kGraphSteps;
+    D->ctl_step = s->step;
+    D->ctl_row = D->rows;
+    if (D->rows == kSeriesCap) launch_dft(s, D, 0);
+    return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1396,6 +1582,7 @@ int cem::dev_ensure(cem_sim *s) {
         return -1;
     }
     D->profiling = std::getenv("COURANT_PROFILE") != nullptr;
+    if (const char *v = std::getenv("COURANT_GRAPH")) D->use_graph = std::atoi(v) != 0;
     if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("COURANT_ISEG_E")) D->iseg_e = std::max(1, std::atoi(v));
     s->spec_src.assign(2 * s->nf, 0.0);
@@ -1424,8 +1611,11 @@ void cem::dev_destroy(cem_sim *s) {
     D->axis1d.release();
     D->slots.release();
     D->psi.release();
+    drop_graph(D);
     D->J.release();
-    D->bdt.release();
+    D->dtab.release();
+    D->d_mat.release();
+    D->ctl.release();
     D->Y.release();
     D->wl.release();
     D->wm.release();
@@ -1436,7 +1626,6 @@ void cem::dev_destroy(cem_sim *s) {
     D->d_sb.release();
     D->sb_src.release();
     D->x_src.release();
-    D->d_cb.release();
     D->Eold.release();
     D->tidx.release();
     D->tw.release();
@@ -1466,11 +1655,18 @@ void cem::dev_step(cem_sim *s, long nsteps) {
     Device *D = s->dev;
     ensure_vlast(s, D);
     const int nx = s->n[0];
-    for (long it = 0; it < nsteps; it++) {
+    long it = 0;
+    while (it < nsteps) {
+        // whole chunks from a captured graph once the series row is chunk-aligned
+        if (D->use_graph && !D->profiling && D->rows % kGraphSteps == 0 && nsteps - it >= kGraphSteps &&
+            run_chunk(s, D) == 0) {
+            it += kGraphSteps;
+            continue;
+        }
         launch_h(s, D, 0, nx + 1);
         launch_e(s, D, 0, nx);
-        launch_post(s, D, s->step);
-        finish_step(s, D);
+        launch_post(s, D);
+        it++;
     }
 }
 
@@ -1485,9 +1681,7 @@ void cem::dev_mt_E(cem_sim *s, int i0, int i1) {
 }
 
 void cem::dev_mt_post(cem_sim *s) {
-    Device *D = s->dev;
-    launch_post(s, D, s->step);
-    finish_step(s, D);
+    launch_post(s, s->dev);
 }
 
 double cem::dev_energy(cem_sim *s) {

@@ -26,7 +26,8 @@ CEM_API int cem_amd_set_profiling(cem_sim *s, int on);
 
 /* Total GPU time (ms) and launch count of one kernel since profiling was
  * switched on: which = 0 H update, 1 E update, 2 probe gather, 3 DFT,
- * 4 post-phase edge physics (Debye, SIBC, sources). */
+ * 4 post-phase edge physics (Debye, SIBC, sources).  Profiling launches
+ * every kernel directly (no graph replay). */
 CEM_API int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count);
 
 /* Stream-copy bandwidth of the current GPU in GB/s (read + write bytes), best
