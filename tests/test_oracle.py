@@ -197,3 +197,41 @@ def test_oracle_renormalize(ref):
         c = v.copy()
         orc.orc_s2_renormalize(c.ctypes.data_as(C.POINTER(C.c_double)), z1, z1, z1)
         assert np.allclose(c, v, atol=1e-12)
+
+
+@pytest.mark.parametrize("direction", [1, -1])
+def test_oracle_mode_waves_closed_form(direction):
+    """Wave separation (modeport.c:203-227) of a pure travelling wave on a lossy line with
+    complex Z0 and gamma: node voltages at the monitor cell's two faces and the current at its
+    centre must give the launched wave at the reference plane and nothing in the other wave.
+    A sign slip in the current, the cosh correction or the shift to the reference plane breaks
+    this by O(1)."""
+    import ctypes as C
+    from oracle_binding import _load
+    orc = _load(ORACLE_F64)
+    P = C.POINTER(C.c_double)
+    rng = np.random.default_rng(5)
+    nf = 7
+    Z = rng.uniform(30, 70, nf) + 1j * rng.uniform(-4, 4, nf)
+    gam = rng.uniform(5, 40, nf) + 1j * rng.uniform(200, 3000, nf)
+    dx, xm, xref = 37e-6, 1.1e-3, 0.4e-3
+    d = direction * (xref - xm)
+    for inward in (True, False):
+        amp = 0.3 - 0.7j
+        # travelling into the device means +x for dir=+1 and -x for dir=-1
+        s = -gam * direction if inward else gam * direction
+        V = lambda x: np.sqrt(Z) * amp * np.exp(s * x)
+        Iplus = V(xm) / Z * (1 if inward else -1) * direction   # +x current of the wave at the centre
+        V0, V1 = V(xm - dx / 2), V(xm + dx / 2)
+        il = lambda z: np.ascontiguousarray(np.stack([z.real, z.imag], -1).ravel())
+        a, b = np.zeros(2 * nf), np.zeros(2 * nf)
+        orc.orc_mode_waves(nf, il(V0).ctypes.data_as(P), il(V1).ctypes.data_as(P), il(Iplus).ctypes.data_as(P),
+                           il(Z).ctypes.data_as(P), np.ascontiguousarray(gam.real).ctypes.data_as(P),
+                           np.ascontiguousarray(gam.imag).ctypes.data_as(P), dx, d, direction,
+                           a.ctypes.data_as(P), b.ctypes.data_as(P))
+        A, B = a[0::2] + 1j * a[1::2], b[0::2] + 1j * b[1::2]
+        # the wave's amplitude at the reference plane
+        want = amp * np.exp(s * xref)
+        got, other = (A, B) if inward else (B, A)
+        np.testing.assert_allclose(got, want, rtol=1e-12)
+        assert np.max(np.abs(other)) <= 1e-12 * np.max(np.abs(want))
