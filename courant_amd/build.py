@@ -24,8 +24,12 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
               "-Wno-unused-parameter", "-fvisibility=hidden", "-I" + INCLUDE, "-I" + CSRC]
+# -ffp-contract=off: every kernel rounds each operation as written (the FP32 CPU
+# emulation of the oracle is built the same way), independent of the compiler's
+# multiply-add fusion choices.
 DEVICE_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
-                "-packed-fp32-ops", "-fvisibility=hidden", "-Wno-unused-result", "-I" + INCLUDE, "-I" + CSRC]
+                "-packed-fp32-ops", "-ffp-contract=off", "-fvisibility=hidden", "-Wno-unused-result",
+                "-I" + INCLUDE, "-I" + CSRC]
 
 
 def _run(cmd):
