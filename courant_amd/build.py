@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcourant.so")
 
 HOST_SOURCES = ["setup.cpp", "materials.cpp", "mode2d.cpp", "modeport.cpp"]
-DEVICE_SOURCES = ["engine.hip"]
+DEVICE_SOURCES = ["engine.hip", "mode_gpu.hip"]
 
 CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -11,6 +11,7 @@
 // band), which performs exactly the same floating-point operations per entry as
 // the reference's row-outer loop, so results are unchanged.
 #include "sim.hpp"
+#include "mode_gpu.hpp"
 
 #include <cmath>
 #include <cstdlib>
@@ -35,6 +36,7 @@ struct cem_mode2d {
     double freq = 0;
     cplx beta = 0;
     std::vector<cplx> eu, ev, hu, hv;
+    cem::ModeGpu *gpu = nullptr;   // device context of the GPU solve path, kept across frequencies
 };
 
 namespace {
@@ -274,7 +276,11 @@ extern "C" cem_mode2d *cem_mode2d_create(int ny, int nz, const double *y, const 
     return m;
 }
 
-extern "C" void cem_mode2d_destroy(cem_mode2d *m) { delete m; }
+extern "C" void cem_mode2d_destroy(cem_mode2d *m) {
+    if (!m) return;
+    cem::mode_gpu_release(m->gpu);
+    delete m;
+}
 
 extern "C" void cem_mode2d_set_debye(cem_mode2d *m, int npole, const double *f_p, const double *deps_u,
                                      const double *deps_v, const double *deps_w) {
@@ -323,14 +329,10 @@ extern "C" int cem_mode2d_solve(cem_mode2d *m, double f, double neff_guess, cons
     Order o;
     make_order(m, o);
     const int n = o.n, bw = 2 * (nz + 2) + 2;
-    Band B{n, bw, bw, 3 * bw + 1, {}, {}};
-    std::vector<cplx> x, y;
+    std::vector<cplx> x;
     Work W;
     try {
-        B.ab.assign((size_t)B.ldab * n, cplx(0.0));
-        B.ipiv.assign(n, 0);
         x.assign(n, cplx(0.0));
-        y.assign(n, cplx(0.0));
         W.eu.assign((size_t)ny * (nz + 1), cplx(0.0));
         W.ev.assign((size_t)(ny + 1) * nz, cplx(0.0));
         W.hu.assign((size_t)(ny + 1) * nz, cplx(0.0));
@@ -340,24 +342,6 @@ extern "C" int cem_mode2d_solve(cem_mode2d *m, double f, double neff_guess, cons
         set_error("out of memory");
         return -1;
     }
-    // assemble A into band storage by probing with coloured unit vectors
-    const int Wc = 2 * bw + 1;
-    for (int col = 0; col < Wc; col++) {
-        std::fill(x.begin(), x.end(), cplx(0.0));
-        for (int g = col; g < n; g += Wc) x[g] = 1.0;
-        apply_A(m, c, o, w, x.data(), y.data(), W);
-        for (int g = col; g < n; g += Wc) {
-            int lo = g - bw < 0 ? 0 : g - bw, hi = g + bw > n - 1 ? n - 1 : g + bw;
-            for (int r = lo; r <= hi; r++) B.at(r, g) = y[r];
-        }
-    }
-    // PEC unknowns are pushed far from the shift; then A - sigma I
-    const double big = -1e6 * (std::abs(sigma) + 1.0);
-    for (int g = 0; g < n; g++) {
-        if (order_pec(m, o, g)) B.at(g, g) = big;
-        B.at(g, g) -= sigma;
-    }
-    if (band_lu(B)) return -1;
     if (start) {
         const cplx *st = reinterpret_cast<const cplx *>(start);
         for (int g = 0; g < n; g++) x[g] = o.kind[g] == 0 ? st[o.idx[g]] : st[ny * (nz + 1) + o.idx[g]];
@@ -368,39 +352,95 @@ extern "C" int cem_mode2d_solve(cem_mode2d *m, double f, double neff_guess, cons
             x[g] = ((sd >> 8) & 0xffff) / 65536.0 - 0.5;
         }
     }
-    cplx lam = sigma, lam_old = 0.0;
-    std::vector<cplx> xo(n, cplx(0.0));
+    // PEC unknowns are pushed far from the shift (then A - sigma I)
+    const double big = -1e6 * (std::abs(sigma) + 1.0);
+    cplx lam = sigma;
     int rc = 1;
-    for (int it = 0; it < maxit; it++) {
-        double nx2 = 0;
-        for (int g = 0; g < n; g++) nx2 += (x[g] * std::conj(x[g])).real();
-        double inv = 1 / std::sqrt(nx2);
-        for (int g = 0; g < n; g++) x[g] *= inv;
-        // convergence of the phase-aligned vector as well as of lambda
-        cplx ph = 0.0;
-        for (int g = 0; g < n; g++) ph += std::conj(xo[g]) * x[g];
-        double dv = 0;
-        if (std::abs(ph) > 0) {
-            ph = std::conj(ph) / std::abs(ph);
-            for (int g = 0; g < n; g++) {
-                cplx d = x[g] * ph - xo[g];
-                dv += (d * std::conj(d)).real();
+    if (cem::mode_gpu_enabled()) {
+        // assembly, band LU and the inverse iteration on the GPU (FP64)
+        cem::ModeGpuProblem P;
+        P.ny = ny;
+        P.nz = nz;
+        P.w = w;
+        P.iyc = c.iyc.data();
+        P.iyn = c.iyn.data();
+        P.izc = c.izc.data();
+        P.izn = c.izn.data();
+        P.eu = c.eu.data();
+        P.ev = c.ev.data();
+        P.iw = c.iw.data();
+        P.pec_u = m->pec_u.data();
+        P.pec_v = m->pec_v.data();
+        P.sigma = sigma;
+        P.big = big;
+        P.x0 = x.data();
+        P.maxit = maxit;
+        P.tol = tol;
+        std::vector<cplx> xg(n);
+        rc = cem::mode_gpu_solve(&m->gpu, P, xg.data(), &lam);
+        if (rc < 0) return -1;
+        x.swap(xg);
+    } else {
+        Band B{n, bw, bw, 3 * bw + 1, {}, {}};
+        std::vector<cplx> y;
+        try {
+            B.ab.assign((size_t)B.ldab * n, cplx(0.0));
+            B.ipiv.assign(n, 0);
+            y.assign(n, cplx(0.0));
+        } catch (const std::bad_alloc &) {
+            set_error("out of memory");
+            return -1;
+        }
+        // assemble A into band storage by probing with coloured unit vectors
+        const int Wc = 2 * bw + 1;
+        std::vector<cplx> e(n, cplx(0.0));
+        for (int col = 0; col < Wc; col++) {
+            std::fill(e.begin(), e.end(), cplx(0.0));
+            for (int g = col; g < n; g += Wc) e[g] = 1.0;
+            apply_A(m, c, o, w, e.data(), y.data(), W);
+            for (int g = col; g < n; g += Wc) {
+                int lo = g - bw < 0 ? 0 : g - bw, hi = g + bw > n - 1 ? n - 1 : g + bw;
+                for (int r = lo; r <= hi; r++) B.at(r, g) = y[r];
             }
-        } else {
-            dv = 1;
         }
-        xo = x;
-        y = x;
-        band_solve(B, y.data());
-        cplx mu = 0.0;
-        for (int g = 0; g < n; g++) mu += std::conj(x[g]) * y[g];
-        lam = sigma + 1.0 / mu;
-        x = y;
-        if (it > 2 && std::abs(lam - lam_old) < tol * std::abs(lam) && std::sqrt(dv) < 1e-10) {
-            rc = 0;
-            break;
+        for (int g = 0; g < n; g++) {
+            if (order_pec(m, o, g)) B.at(g, g) = big;
+            B.at(g, g) -= sigma;
         }
-        lam_old = lam;
+        if (band_lu(B)) return -1;
+        cplx lam_old = 0.0;
+        std::vector<cplx> xo(n, cplx(0.0));
+        for (int it = 0; it < maxit; it++) {
+            double nx2 = 0;
+            for (int g = 0; g < n; g++) nx2 += (x[g] * std::conj(x[g])).real();
+            double inv = 1 / std::sqrt(nx2);
+            for (int g = 0; g < n; g++) x[g] *= inv;
+            // convergence of the phase-aligned vector as well as of lambda
+            cplx ph = 0.0;
+            for (int g = 0; g < n; g++) ph += std::conj(xo[g]) * x[g];
+            double dv = 0;
+            if (std::abs(ph) > 0) {
+                ph = std::conj(ph) / std::abs(ph);
+                for (int g = 0; g < n; g++) {
+                    cplx d = x[g] * ph - xo[g];
+                    dv += (d * std::conj(d)).real();
+                }
+            } else {
+                dv = 1;
+            }
+            xo = x;
+            y = x;
+            band_solve(B, y.data());
+            cplx mu = 0.0;
+            for (int g = 0; g < n; g++) mu += std::conj(x[g]) * y[g];
+            lam = sigma + 1.0 / mu;
+            x = y;
+            if (it > 2 && std::abs(lam - lam_old) < tol * std::abs(lam) && std::sqrt(dv) < 1e-10) {
+                rc = 0;
+                break;
+            }
+            lam_old = lam;
+        }
     }
     // normalise; global phase so the largest component is real
     double nx2 = 0, amax = 0;

@@ -1,0 +1,727 @@
+// GPU (gfx950) FP64 shift-invert eigen-solver of the 2D port-mode operator
+// (the device half of cem_mode2d_solve, mode2d.c:307-436).
+//
+//   k_probe_*   : A e_c for coloured unit vectors e_c (the host's probing), the
+//                 operator A = P Q applied matrix-free (mode2d.c:181-217), a
+//                 batch of colours per launch; the results are scattered into
+//                 the complex band (kl = ku = bw, ldab = 3 bw + 1, column-major)
+//   k_diag      : PEC unknowns pushed to -1e6 (|sigma| + 1), then - sigma
+//   k_lu_panel  : one panel of nb columns factorised in LDS by one workgroup:
+//                 partial pivoting over the kl rows below the diagonal, row
+//                 interchanges within the panel, scaling, rank-1 updates
+//   k_lu_trail  : the panel's interchanges and rank-1 updates applied to the
+//                 trailing columns (one wavefront per column, column in
+//                 registers) in the order of the unblocked column loop, so
+//                 every entry sees the host's update sequence
+//   k_inverse   : the whole inverse iteration in one workgroup: normalisation,
+//                 phase-aligned change, band forward / back solves (one
+//                 wavefront walks the columns through an LDS window of the
+//                 right-hand side while the others stream the next band
+//                 columns into a double-buffered LDS ring), Rayleigh quotient,
+//                 the host's convergence test; reductions in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "mode_gpu.hpp"
+#include "sim.hpp"
+
+namespace cem {
+
+namespace {
+
+constexpr double kMu0 = CEM_MU0;
+constexpr int kLuThreads = 256;
+constexpr int kTrailRows = 8;          // rows per lane of a trailing column (panel rows <= 512)
+constexpr int kInvThreads = 1024;
+constexpr size_t kRingBytes = 40 * 1024;   // LDS ring of band columns for the triangular solves
+constexpr int kProbeBatch = 64;        // colours per probing launch
+
+struct zc {
+    double r, i;
+};
+__device__ __forceinline__ zc mk(double r, double i) {
+    zc z;
+    z.r = r;
+    z.i = i;
+    return z;
+}
+__device__ __forceinline__ zc zadd(zc a, zc b) { return mk(a.r + b.r, a.i + b.i); }
+__device__ __forceinline__ zc zsub(zc a, zc b) { return mk(a.r - b.r, a.i - b.i); }
+__device__ __forceinline__ zc zmul(zc a, zc b) { return mk(a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r); }
+__device__ __forceinline__ zc zrs(double s, zc a) { return mk(s * a.r, s * a.i); }
+__device__ __forceinline__ zc zconj(zc a) { return mk(a.r, -a.i); }
+// Smith's method (R. L. Smith, CACM 5(8), 1962)
+__device__ __forceinline__ zc zdiv(zc a, zc b) {
+    if (fabs(b.r) < fabs(b.i)) {
+        const double ratio = b.r / b.i, den = b.r * ratio + b.i;
+        return mk((a.r * ratio + a.i) / den, (a.i * ratio - a.r) / den);
+    }
+    const double ratio = b.i / b.r, den = b.i * ratio + b.r;
+    return mk((a.i * ratio + a.r) / den, (a.i - a.r * ratio) / den);
+}
+__device__ __forceinline__ double zabs(zc a) { return hypot(a.r, a.i); }
+__device__ __forceinline__ zc shfl(zc v, int src) { return mk(__shfl(v.r, src), __shfl(v.i, src)); }
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+struct Geo {
+    int ny, nz, nu, nv, nn, nc, n, bw, ldab, wc;
+};
+
+// band entry (i, j) (requires j - 2 bw <= i <= j + bw)
+__device__ __forceinline__ long long bidx(const Geo &g, int i, int j) {
+    return (long long)j * g.ldab + (2 * g.bw + i - j);
+}
+
+// unknown order: j outer, k inner, E_u then E_v (mode2d.c ordering, 2 nz + 1 per j < ny)
+__device__ __forceinline__ int gu(const Geo &g, int j, int k) { return j * (2 * g.nz + 1) + (k < g.nz ? 2 * k : 2 * g.nz); }
+__device__ __forceinline__ int gv(const Geo &g, int j, int k) {
+    return j < g.ny ? j * (2 * g.nz + 1) + 2 * k + 1 : g.ny * (2 * g.nz + 1) + k;
+}
+
+struct Coef {
+    const zc *iyc, *iyn, *izc, *izn, *eu, *ev, *iw;
+    const unsigned char *pec_u, *pec_v;
+    double w;
+};
+
+__device__ __forceinline__ zc unit_u(const Geo &g, const Coef &c, int col, int j, int k) {
+    return (gu(g, j, k) % g.wc == col && !c.pec_u[j * (g.nz + 1) + k]) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+}
+__device__ __forceinline__ zc unit_v(const Geo &g, const Coef &c, int col, int j, int k) {
+    return (gv(g, j, k) % g.wc == col && !c.pec_v[j * g.nz + k]) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+}
+
+// hw = curl_x of the coloured input (cells)
+__global__ void k_probe_hw(Geo g, Coef c, int col0, int nb, zc *hw) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nb * g.nc) return;
+    const int b = (int)(t / g.nc), q = (int)(t % g.nc);
+    const int j = q / g.nz, k = q % g.nz, col = col0 + b;
+    const zc dv = zsub(unit_v(g, c, col, j + 1, k), unit_v(g, c, col, j, k));
+    const zc du = zsub(unit_u(g, c, col, j, k + 1), unit_u(g, c, col, j, k));
+    hw[t] = zsub(zmul(dv, c.iyc[j]), zmul(du, c.izc[k]));
+}
+
+// h = Q e: hu at E_v positions, hv at E_u positions (mode2d.c:181-199)
+__global__ void k_probe_h(Geo g, Coef c, int col0, int nb, const zc *hw, zc *hu, zc *hv) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = g.nv + g.nu;
+    if (t >= (long long)nb * per) return;
+    const int b = (int)(t / per), q = (int)(t % per), col = col0 + b;
+    const zc *w = hw + (size_t)b * g.nc;
+    const double cc = 1.0 / (c.w * kMu0);
+    const zc zero = mk(0.0, 0.0);
+    if (q < g.nv) {
+        const int j = q / g.nz, k = q % g.nz;
+        const zc r = zsub(j < g.ny ? w[j * g.nz + k] : zero, j > 0 ? w[(j - 1) * g.nz + k] : zero);
+        const zc a = zmul(zrs(-c.w, c.ev[q]), unit_v(g, c, col, j, k));
+        hu[(size_t)b * g.nv + q] = zsub(a, zmul(zrs(cc, r), c.iyn[j]));
+    } else {
+        const int q2 = q - g.nv, j = q2 / (g.nz + 1), k = q2 % (g.nz + 1);
+        const zc r = zsub(k < g.nz ? w[j * g.nz + k] : zero, k > 0 ? w[j * g.nz + k - 1] : zero);
+        const zc a = zmul(zrs(c.w, c.eu[q2]), unit_u(g, c, col, j, k));
+        hv[(size_t)b * g.nu + q2] = zsub(a, zmul(zrs(cc, r), c.izn[k]));
+    }
+}
+
+// ew at nodes (mode2d.c:203-209)
+__global__ void k_probe_ew(Geo g, Coef c, int nb, const zc *hu, const zc *hv, zc *ew) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nb * g.nn) return;
+    const int b = (int)(t / g.nn), q = (int)(t % g.nn);
+    const int j = q / (g.nz + 1), k = q % (g.nz + 1);
+    const zc *u = hu + (size_t)b * g.nv, *v = hv + (size_t)b * g.nu;
+    const zc zero = mk(0.0, 0.0);
+    const zc du = zmul(zsub(j < g.ny ? v[j * (g.nz + 1) + k] : zero, j > 0 ? v[(j - 1) * (g.nz + 1) + k] : zero),
+                       c.iyn[j]);
+    const zc dv = zmul(zsub(k < g.nz ? u[j * g.nz + k] : zero, k > 0 ? u[j * g.nz + k - 1] : zero), c.izn[k]);
+    ew[t] = zmul(zsub(du, dv), c.iw[q]);
+}
+
+// e = P h at every unknown (mode2d.c:210-217), PEC rows zero, scattered into band
+// column g of the colour's unit within bw of the row
+__global__ void k_probe_out(Geo g, Coef c, int col0, int nb, const zc *hu, const zc *hv, const zc *ew, zc *band) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nb * g.n) return;
+    const int b = (int)(t / g.n), r = (int)(t % g.n), col = col0 + b;
+    const int tt = r - col + g.bw;
+    if (tt < 0) return;
+    const int gcol = col + (tt / g.wc) * g.wc;
+    if (gcol >= g.n) return;
+    const zc *u = hu + (size_t)b * g.nv, *v = hv + (size_t)b * g.nu, *e = ew + (size_t)b * g.nn;
+    const double cc = 1.0 / c.w;
+    zc y = mk(0.0, 0.0);
+    const int per = 2 * g.nz + 1;
+    if (r < g.ny * per) {
+        const int j = r / per, s = r % per;
+        if (s == 2 * g.nz || (s & 1) == 0) {   // E_u(j, k)
+            const int k = s == 2 * g.nz ? g.nz : s / 2;
+            if (!c.pec_u[j * (g.nz + 1) + k])
+                y = zadd(zrs(c.w * kMu0, v[j * (g.nz + 1) + k]),
+                         zmul(zrs(cc, zsub(e[(j + 1) * (g.nz + 1) + k], e[j * (g.nz + 1) + k])), c.iyc[j]));
+        } else {                                 // E_v(j, k)
+            const int k = s / 2;
+            if (!c.pec_v[j * g.nz + k])
+                y = zadd(zrs(-c.w * kMu0, u[j * g.nz + k]),
+                         zmul(zrs(cc, zsub(e[j * (g.nz + 1) + k + 1], e[j * (g.nz + 1) + k])), c.izc[k]));
+        }
+    } else {
+        const int j = g.ny, k = r - g.ny * per;
+        if (!c.pec_v[j * g.nz + k])
+            y = zadd(zrs(-c.w * kMu0, u[j * g.nz + k]),
+                     zmul(zrs(cc, zsub(e[j * (g.nz + 1) + k + 1], e[j * (g.nz + 1) + k])), c.izc[k]));
+    }
+    band[bidx(g, r, gcol)] = y;
+}
+
+__global__ void k_diag(Geo g, Coef c, zc sigma, double big, zc *band) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= g.n) return;
+    const int per = 2 * g.nz + 1;
+    bool pec;
+    if (r < g.ny * per) {
+        const int j = r / per, s = r % per;
+        if (s == 2 * g.nz || (s & 1) == 0) pec = c.pec_u[j * (g.nz + 1) + (s == 2 * g.nz ? g.nz : s / 2)];
+        else pec = c.pec_v[j * g.nz + s / 2];
+    } else {
+        pec = c.pec_v[g.ny * g.nz + (r - g.ny * per)];
+    }
+    zc &d = band[bidx(g, r, r)];
+    if (pec) d = mk(big, 0.0);
+    d = zsub(d, sigma);
+}
+
+// ---- band LU -------------------------------------------------------------
+// Panel of columns [j0, j1), rows [j0, j0 + R), R = min(j1 - 1 + kl, n - 1) - j0 + 1,
+// column-major in LDS.  Entries below row c + kl of column c are zero and stay so.
+__global__ void __launch_bounds__(kLuThreads) k_lu_panel(Geo g, zc *band, int *ipiv, int *info, int j0, int j1) {
+    extern __shared__ zc P[];
+    __shared__ int piv_row;
+    __shared__ double piv_abs;
+    const int kl = g.bw, nbp = j1 - j0;
+    const int R = min(j1 - 1 + kl, g.n - 1) - j0 + 1;
+    for (int t = threadIdx.x; t < R * nbp; t += kLuThreads) {
+        const int cc = t / R, rr = t % R, col = j0 + cc, row = j0 + rr;
+        P[t] = row <= col + kl ? band[bidx(g, row, col)] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int j = j0; j < j1; j++) {
+        const int last = min(j + kl, g.n - 1);
+        const int cj = j - j0;
+        zc *colj = P + (size_t)cj * R;
+        if (threadIdx.x < 64) {   // pivot: largest |.|, first one on ties (rows ascending)
+            double best = -1.0;
+            int bi = j;
+            for (int i = j + (int)threadIdx.x; i <= last; i += 64) {
+                const double v = zabs(colj[i - j0]);
+                if (v > best) {
+                    best = v;
+                    bi = i;
+                }
+            }
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (threadIdx.x == 0) {
+                piv_row = bi;
+                piv_abs = best;
+            }
+        }
+        __syncthreads();
+        const int p = piv_row;
+        if (threadIdx.x == 0) {
+            ipiv[j] = p;
+            if (piv_abs == 0.0 && *info == 0) *info = j + 1;
+        }
+        if (p != j)
+            for (int cc = cj + (int)threadIdx.x; cc < nbp; cc += kLuThreads) {
+                zc *col = P + (size_t)cc * R;
+                const zc a = col[j - j0];
+                col[j - j0] = col[p - j0];
+                col[p - j0] = a;
+            }
+        __syncthreads();
+        const zc pv = colj[cj];
+        for (int i = j + 1 + (int)threadIdx.x; i <= last; i += kLuThreads) colj[i - j0] = zdiv(colj[i - j0], pv);
+        __syncthreads();
+        const int nr = last - j, ncol = nbp - cj - 1;
+        for (int t = threadIdx.x; t < nr * ncol; t += kLuThreads) {
+            const int cc = cj + 1 + t / nr, i = j + 1 + t % nr;
+            zc *col = P + (size_t)cc * R;
+            col[i - j0] = zsub(col[i - j0], zmul(colj[i - j0], col[cj]));
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < R * nbp; t += kLuThreads) {
+        const int cc = t / R, rr = t % R, col = j0 + cc, row = j0 + rr;
+        if (row <= col + kl) band[bidx(g, row, col)] = P[t];
+    }
+}
+
+// Trailing columns c in [j1, min(j1 - 1 + kv, n - 1)], one wavefront each; the
+// panel's multipliers are shared through LDS.
+__global__ void __launch_bounds__(kLuThreads) k_lu_trail(Geo g, zc *band, const int *ipiv, int j0, int j1) {
+    extern __shared__ zc L[];
+    const int kl = g.bw, kv = 2 * g.bw, nbp = j1 - j0;
+    const int R = min(j1 - 1 + kl, g.n - 1) - j0 + 1;
+    for (int t = threadIdx.x; t < R * nbp; t += kLuThreads) {
+        const int cc = t / R, rr = t % R, col = j0 + cc, row = j0 + rr;
+        L[t] = row > col && row <= col + kl ? band[bidx(g, row, col)] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int c = j1 + (int)(blockIdx.x * (kLuThreads / 64) + (threadIdx.x >> 6));
+    const int cend = min(j1 - 1 + kv, g.n - 1);
+    if (c > cend) return;
+    const int rlo = max(j0, c - kv);
+    zc v[kTrailRows];
+#pragma unroll
+    for (int m = 0; m < kTrailRows; m++) {
+        const int row = j0 + lane + 64 * m;
+        v[m] = row < j0 + R && row >= rlo ? band[bidx(g, row, c)] : mk(0.0, 0.0);
+    }
+    for (int j = j0; j < j1; j++) {
+        if (c > j + kv) continue;
+        const int p = ipiv[j], last = min(j + kl, g.n - 1);
+        // value of rows j and p
+        zc aj = mk(0.0, 0.0), ap = mk(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < kTrailRows; m++) {
+            if (m == (j - j0) / 64) aj = v[m];
+            if (m == (p - j0) / 64) ap = v[m];
+        }
+        aj = shfl(aj, (j - j0) % 64);
+        ap = shfl(ap, (p - j0) % 64);
+        const zc x = ap;   // row j after the interchange
+#pragma unroll
+        for (int m = 0; m < kTrailRows; m++) {
+            const int row = j0 + lane + 64 * m;
+            if (p != j) {
+                if (row == j) v[m] = ap;
+                else if (row == p) v[m] = aj;
+            }
+            if (row > j && row <= last) v[m] = zsub(v[m], zmul(L[(size_t)(j - j0) * R + (row - j0)], x));
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kTrailRows; m++) {
+        const int row = j0 + lane + 64 * m;
+        if (row < j0 + R && row >= rlo) band[bidx(g, row, c)] = v[m];
+    }
+}
+
+// ---- inverse iteration -------------------------------------------------
+struct InvArgs {
+    Geo g;
+    const zc *band;
+    const int *ipiv;
+    zc *x, *xo, *y;
+    zc sigma;
+    int maxit;
+    double tol;
+    double *out;   // lam (2), rc, iterations
+    int chunk_f, chunk_b;   // band columns per ring half (forward, back)
+    int ring;               // ring size in entries
+    int wmask;              // window size - 1 (power of two)
+};
+
+__device__ double block_sum(double v, double *red) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < kInvThreads / 64; i++) s += red[i];
+    return s;
+}
+
+// In-place solve of (LU) y = y with the band factors (the host's band_solve):
+// forward with the interchanges, then back substitution.  Wavefront 0 walks the
+// columns through an LDS window of y; the other wavefronts load the next chunk
+// of band columns (the entries the walk needs) into the other half of a ring.
+__device__ void band_solve(const InvArgs &a, zc *win, zc *ring) {
+    const Geo &g = a.g;
+    const int chunk_f = a.chunk_f, chunk_b = a.chunk_b, wm = a.wmask;
+    const int n = g.n, kl = g.bw, kv = 2 * g.bw;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    zc *y = a.y;
+    // ---- forward: for j: swap y[j], y[p]; y[j+1..j+kl] -= L(:, j) y[j]
+    {
+        const int K = kl, nchunk = (n + chunk_f - 1) / chunk_f;
+        // window holds y[lo, hi) at win[i & wm]
+        for (int i = threadIdx.x; i < min(n, kl + chunk_f); i += kInvThreads) win[i & wm] = y[i];
+        int hi = min(n, kl + chunk_f);
+        // preload chunk 0 multipliers
+        for (int t = threadIdx.x; t < chunk_f * K; t += kInvThreads) {
+            const int jj = t / K, rr = t % K, j = jj, i = j + 1 + rr;
+            ring[t] = (j < n && i < n) ? a.band[bidx(g, i, j)] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int ch = 0; ch < nchunk; ch++) {
+            const int cj0 = ch * chunk_f, cj1 = min(n, cj0 + chunk_f);
+            zc *cur = ring + (size_t)(ch & 1) * chunk_f * K;
+            zc *nxt = ring + (size_t)((ch + 1) & 1) * chunk_f * K;
+            if (wv == 0) {
+                for (int j = cj0; j < cj1; j++) {
+                    const int p = a.ipiv[j], last = min(j + kl, n - 1);
+                    if (lane == 0 && p != j) {
+                        const zc t = win[j & wm];
+                        win[j & wm] = win[p & wm];
+                        win[p & wm] = t;
+                    }
+                    wave_sync_lds();
+                    const zc xj = win[j & wm];
+                    const zc *l = cur + (size_t)(j - cj0) * K;
+                    for (int i = j + 1 + lane; i <= last; i += 64)
+                        win[i & wm] = zsub(win[i & wm], zmul(l[i - j - 1], xj));
+                    wave_sync_lds();
+                }
+            } else {
+                // next chunk's multipliers, and the y entries entering the window
+                const int nj0 = cj1;
+                for (int t = threadIdx.x - 64; t < chunk_f * K; t += kInvThreads - 64) {
+                    const int j = nj0 + t / K, i = j + 1 + t % K;
+                    nxt[t] = (j < n && i < n) ? a.band[bidx(g, i, j)] : mk(0.0, 0.0);
+                }
+            }
+            __syncthreads();
+            // y[cj0, cj1) is final: store; bring y[hi, min(n, cj1 + chunk_f + kl)) in
+            const int nhi = min(n, cj1 + chunk_f + kl);
+            for (int i = cj0 + (int)threadIdx.x; i < cj1; i += kInvThreads) y[i] = win[i & wm];
+            for (int i = hi + (int)threadIdx.x; i < nhi; i += kInvThreads) win[i & wm] = y[i];
+            hi = max(hi, nhi);
+            __syncthreads();
+        }
+    }
+    // ---- back: for j = n-1..0: y[j] /= U(j, j); y[j-kv..j-1] -= U(:, j) y[j]
+    {
+        const int K = kv + 1, nchunk = (n + chunk_b - 1) / chunk_b;
+        // chunk ch covers columns [n - (ch+1) chunk_b, n - ch chunk_b) walked downwards
+        const int lo0 = max(0, n - kv - chunk_b);
+        for (int i = lo0 + (int)threadIdx.x; i < n; i += kInvThreads) win[i & wm] = y[i];
+        int lo = lo0;
+        for (int t = threadIdx.x; t < chunk_b * K; t += kInvThreads) {
+            const int jj = t / K, rr = t % K, j = n - 1 - jj, i = j - kv + rr;
+            ring[t] = (j >= 0 && i >= 0) ? a.band[bidx(g, i, j)] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int ch = 0; ch < nchunk; ch++) {
+            const int cj1 = n - ch * chunk_b, cj0 = max(0, cj1 - chunk_b);
+            zc *cur = ring + (size_t)(ch & 1) * chunk_b * K;
+            zc *nxt = ring + (size_t)((ch + 1) & 1) * chunk_b * K;
+            if (wv == 0) {
+                for (int j = cj1 - 1; j >= cj0; j--) {
+                    const zc *u = cur + (size_t)(cj1 - 1 - j) * K;   // u[rr] = U(j - kv + rr, j)
+                    if (lane == 0) win[j & wm] = zdiv(win[j & wm], u[kv]);
+                    wave_sync_lds();
+                    const zc xj = win[j & wm];
+                    const int first = max(0, j - kv);
+                    for (int i = first + lane; i < j; i += 64)
+                        win[i & wm] = zsub(win[i & wm], zmul(u[i - (j - kv)], xj));
+                    wave_sync_lds();
+                }
+            } else {
+                const int nj1 = cj0;   // next chunk: columns [nj1 - chunk_b, nj1) downwards
+                for (int t = threadIdx.x - 64; t < chunk_b * K; t += kInvThreads - 64) {
+                    const int j = nj1 - 1 - t / K, i = j - kv + t % K;
+                    nxt[t] = (j >= 0 && i >= 0) ? a.band[bidx(g, i, j)] : mk(0.0, 0.0);
+                }
+            }
+            __syncthreads();
+            const int nlo = max(0, cj0 - chunk_b - kv);
+            for (int i = cj0 + (int)threadIdx.x; i < cj1; i += kInvThreads) y[i] = win[i & wm];
+            for (int i = nlo + (int)threadIdx.x; i < lo; i += kInvThreads) win[i & wm] = y[i];
+            lo = min(lo, nlo);
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
+    extern __shared__ zc ring[];   // ring (a.ring entries), then the window (wmask + 1 entries)
+    zc *win = ring + a.ring;
+    __shared__ double red[kInvThreads / 64];
+    const int n = a.g.n;
+    zc lam = a.sigma, lam_old = mk(0.0, 0.0);
+    int rc = 1, it = 0;
+    for (it = 0; it < a.maxit; it++) {
+        double s = 0.0;
+        for (int q = threadIdx.x; q < n; q += kInvThreads) s += a.x[q].r * a.x[q].r + a.x[q].i * a.x[q].i;
+        const double nx2 = block_sum(s, red);
+        const double inv = 1.0 / sqrt(nx2);
+        double pr = 0.0, pi = 0.0;
+        for (int q = threadIdx.x; q < n; q += kInvThreads) {
+            const zc v = zrs(inv, a.x[q]);
+            a.x[q] = v;
+            const zc t = zmul(zconj(a.xo[q]), v);
+            pr += t.r;
+            pi += t.i;
+        }
+        zc ph = mk(block_sum(pr, red), block_sum(pi, red));
+        double dv = 1.0;
+        const double aph = zabs(ph);
+        if (aph > 0.0) {
+            ph = mk(ph.r / aph, -ph.i / aph);
+            double d = 0.0;
+            for (int q = threadIdx.x; q < n; q += kInvThreads) {
+                const zc e = zsub(zmul(a.x[q], ph), a.xo[q]);
+                d += e.r * e.r + e.i * e.i;
+            }
+            dv = block_sum(d, red);
+        }
+        for (int q = threadIdx.x; q < n; q += kInvThreads) {
+            a.xo[q] = a.x[q];
+            a.y[q] = a.x[q];
+        }
+        __syncthreads();
+        band_solve(a, win, ring);
+        __syncthreads();
+        double mr = 0.0, mi = 0.0;
+        for (int q = threadIdx.x; q < n; q += kInvThreads) {
+            const zc t = zmul(zconj(a.x[q]), a.y[q]);
+            mr += t.r;
+            mi += t.i;
+        }
+        const zc mu = mk(block_sum(mr, red), block_sum(mi, red));
+        lam = zadd(a.sigma, zdiv(mk(1.0, 0.0), mu));
+        for (int q = threadIdx.x; q < n; q += kInvThreads) a.x[q] = a.y[q];
+        __syncthreads();
+        if (it > 2 && zabs(zsub(lam, lam_old)) < a.tol * zabs(lam) && sqrt(dv) < 1e-10) {
+            rc = 0;
+            break;
+        }
+        lam_old = lam;
+    }
+    if (threadIdx.x == 0) {
+        a.out[0] = lam.r;
+        a.out[1] = lam.i;
+        a.out[2] = rc;
+        a.out[3] = it;
+    }
+}
+
+template <typename T>
+struct DMem {
+    T *p = nullptr;
+    size_t n = 0;
+    bool get(size_t count) {
+        if (count <= n) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        n = count;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct ModeGpu {
+    hipStream_t st = nullptr;
+    int ny = -1, nz = -1;
+    DMem<zc> band, coef, x, xo, y, hw, hu, hv, ew;
+    DMem<unsigned char> pec;
+    DMem<int> ipiv, info;
+    DMem<double> out;
+    std::vector<zc> hcoef;
+};
+
+bool mode_gpu_enabled() {
+    static int have_gpu = -1;
+    if (have_gpu < 0) {
+        int ndev = 0;
+        have_gpu = hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0 ? 1 : 0;
+    }
+    const char *v = std::getenv("COURANT_MODE_HOST");
+    return have_gpu == 1 && !(v && std::atoi(v) != 0);
+}
+
+void mode_gpu_release(ModeGpu *m) {
+    if (!m) return;
+    if (m->st) (void)hipStreamSynchronize(m->st);
+    for (auto *b : {&m->band, &m->coef, &m->x, &m->xo, &m->y, &m->hw, &m->hu, &m->hv, &m->ew}) b->release();
+    m->pec.release();
+    m->ipiv.release();
+    m->info.release();
+    m->out.release();
+    if (m->st) (void)hipStreamDestroy(m->st);
+    delete m;
+}
+
+#define MG_CHECK(expr)                                                                  \
+    do {                                                                                \
+        if ((expr) != hipSuccess) {                                                     \
+            set_error("GPU mode solver: HIP error at " #expr);                          \
+            return -1;                                                                  \
+        }                                                                               \
+    } while (0)
+
+int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> *xout, std::complex<double> *lam) {
+    if (!*ctx) {
+        *ctx = new ModeGpu();
+        if (hipStreamCreateWithFlags(&(*ctx)->st, hipStreamNonBlocking) != hipSuccess) {
+            delete *ctx;
+            *ctx = nullptr;
+            set_error("GPU mode solver: cannot create stream");
+            return -1;
+        }
+    }
+    ModeGpu &M = **ctx;
+    Geo g;
+    g.ny = p.ny;
+    g.nz = p.nz;
+    g.nu = p.ny * (p.nz + 1);
+    g.nv = (p.ny + 1) * p.nz;
+    g.nn = (p.ny + 1) * (p.nz + 1);
+    g.nc = p.ny * p.nz;
+    g.n = g.nu + g.nv;
+    g.bw = 2 * (p.nz + 2) + 2;
+    g.ldab = 3 * g.bw + 1;
+    g.wc = 2 * g.bw + 1;
+    const int n = g.n, kl = g.bw, kv = 2 * g.bw;
+    // panel width: the panel (kl + nb rows x nb columns) fits 60 KiB of LDS
+    int nb = 32;
+    while (nb > 1 && (size_t)(kl + nb) * nb * sizeof(zc) > 60 * 1024) nb /= 2;
+    // triangular solves: two halves of chunk columns in the ring; the window of the
+    // right-hand side spans at most 2 chunk + kl (forward) or 2 chunk + kv + 1 (back)
+    const int chunk_f = std::max(1, (int)(kRingBytes / (2 * sizeof(zc) * kl)));
+    const int chunk_b = std::max(1, (int)(kRingBytes / (2 * sizeof(zc) * (kv + 1))));
+    const int ring = std::max(2 * chunk_f * kl, 2 * chunk_b * (kv + 1));
+    int wn = 64;
+    while (wn < std::max(2 * chunk_f + kl, 2 * chunk_b + kv + 1)) wn *= 2;
+    const size_t inv_lds = (size_t)(ring + wn) * sizeof(zc);
+    if (kl + nb > 64 * kTrailRows || inv_lds > 60 * 1024) {
+        set_error("GPU mode solver: port window too wide (nz)");
+        return -1;
+    }
+    const int ncoef = p.ny + (p.ny + 1) + p.nz + (p.nz + 1) + g.nu + g.nv + g.nn;
+    const int nbatch = std::min(kProbeBatch, g.wc);
+    if (!M.band.get((size_t)g.ldab * n) || !M.coef.get(ncoef) || !M.x.get(n) || !M.xo.get(n) || !M.y.get(n) ||
+        !M.hw.get((size_t)nbatch * g.nc) || !M.hu.get((size_t)nbatch * g.nv) || !M.hv.get((size_t)nbatch * g.nu) ||
+        !M.ew.get((size_t)nbatch * g.nn) || !M.pec.get(g.nu + g.nv) || !M.ipiv.get(n) || !M.info.get(1) ||
+        !M.out.get(4)) {
+        set_error("GPU mode solver: device out of memory");
+        return -1;
+    }
+    hipStream_t st = M.st;
+    // coefficients (host-built, FP64 complex) and PEC masks
+    M.hcoef.resize(ncoef);
+    size_t o = 0;
+    auto put = [&](const std::complex<double> *src, int len) {
+        for (int i = 0; i < len; i++) M.hcoef[o + i] = zc{src[i].real(), src[i].imag()};
+        o += len;
+    };
+    put(p.iyc, p.ny);
+    put(p.iyn, p.ny + 1);
+    put(p.izc, p.nz);
+    put(p.izn, p.nz + 1);
+    put(p.eu, g.nu);
+    put(p.ev, g.nv);
+    put(p.iw, g.nn);
+    MG_CHECK(hipMemcpyAsync(M.coef.p, M.hcoef.data(), ncoef * sizeof(zc), hipMemcpyHostToDevice, st));
+    std::vector<unsigned char> pm(g.nu + g.nv);
+    std::memcpy(pm.data(), p.pec_u, g.nu);
+    std::memcpy(pm.data() + g.nu, p.pec_v, g.nv);
+    MG_CHECK(hipMemcpyAsync(M.pec.p, pm.data(), pm.size(), hipMemcpyHostToDevice, st));
+    std::vector<zc> x0(n);
+    for (int i = 0; i < n; i++) x0[i] = zc{p.x0[i].real(), p.x0[i].imag()};
+    MG_CHECK(hipMemcpyAsync(M.x.p, x0.data(), n * sizeof(zc), hipMemcpyHostToDevice, st));
+    MG_CHECK(hipMemsetAsync(M.xo.p, 0, n * sizeof(zc), st));
+    MG_CHECK(hipMemsetAsync(M.band.p, 0, (size_t)g.ldab * n * sizeof(zc), st));
+    MG_CHECK(hipMemsetAsync(M.info.p, 0, sizeof(int), st));
+    Coef c;
+    c.iyc = M.coef.p;
+    c.iyn = c.iyc + p.ny;
+    c.izc = c.iyn + p.ny + 1;
+    c.izn = c.izc + p.nz;
+    c.eu = c.izn + p.nz + 1;
+    c.ev = c.eu + g.nu;
+    c.iw = c.ev + g.nv;
+    c.pec_u = M.pec.p;
+    c.pec_v = M.pec.p + g.nu;
+    c.w = p.w;
+    // assembly by coloured probing
+    const int T = 256;
+    for (int col0 = 0; col0 < g.wc && col0 < n; col0 += nbatch) {
+        const int nbt = std::min(nbatch, std::min(g.wc, n) - col0);
+        hipLaunchKernelGGL(k_probe_hw, dim3((unsigned)(((long long)nbt * g.nc + T - 1) / T)), dim3(T), 0, st, g, c,
+                           col0, nbt, M.hw.p);
+        hipLaunchKernelGGL(k_probe_h, dim3((unsigned)(((long long)nbt * (g.nu + g.nv) + T - 1) / T)), dim3(T), 0, st,
+                           g, c, col0, nbt, M.hw.p, M.hu.p, M.hv.p);
+        hipLaunchKernelGGL(k_probe_ew, dim3((unsigned)(((long long)nbt * g.nn + T - 1) / T)), dim3(T), 0, st, g, c,
+                           nbt, M.hu.p, M.hv.p, M.ew.p);
+        hipLaunchKernelGGL(k_probe_out, dim3((unsigned)(((long long)nbt * n + T - 1) / T)), dim3(T), 0, st, g, c,
+                           col0, nbt, M.hu.p, M.hv.p, M.ew.p, M.band.p);
+    }
+    hipLaunchKernelGGL(k_diag, dim3((n + T - 1) / T), dim3(T), 0, st, g, c, zc{p.sigma.real(), p.sigma.imag()}, p.big,
+                       M.band.p);
+    // band LU, panel by panel
+    for (int j0 = 0; j0 < n; j0 += nb) {
+        const int j1 = std::min(n, j0 + nb);
+        const int R = std::min(j1 - 1 + kl, n - 1) - j0 + 1;
+        const size_t lds = (size_t)R * (j1 - j0) * sizeof(zc);
+        hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(kLuThreads), lds, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+        const int ncol = std::min(j1 - 1 + kv, n - 1) - j1 + 1;
+        if (ncol > 0) {
+            const int wpb = kLuThreads / 64;
+            hipLaunchKernelGGL(k_lu_trail, dim3((ncol + wpb - 1) / wpb), dim3(kLuThreads), lds, st, g, M.band.p,
+                               M.ipiv.p, j0, j1);
+        }
+    }
+    InvArgs ia;
+    ia.g = g;
+    ia.band = M.band.p;
+    ia.ipiv = M.ipiv.p;
+    ia.x = M.x.p;
+    ia.xo = M.xo.p;
+    ia.y = M.y.p;
+    ia.sigma = zc{p.sigma.real(), p.sigma.imag()};
+    ia.maxit = p.maxit;
+    ia.tol = p.tol;
+    ia.out = M.out.p;
+    ia.chunk_f = chunk_f;
+    ia.chunk_b = chunk_b;
+    ia.ring = ring;
+    ia.wmask = wn - 1;
+    hipLaunchKernelGGL(k_inverse, dim3(1), dim3(kInvThreads), inv_lds, st, ia);
+    MG_CHECK(hipGetLastError());
+    int info = 0;
+    double out[4];
+    std::vector<zc> xr(n);
+    MG_CHECK(hipMemcpyAsync(&info, M.info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    MG_CHECK(hipMemcpyAsync(out, M.out.p, sizeof out, hipMemcpyDeviceToHost, st));
+    MG_CHECK(hipMemcpyAsync(xr.data(), M.x.p, n * sizeof(zc), hipMemcpyDeviceToHost, st));
+    MG_CHECK(hipStreamSynchronize(st));
+    if (info) return -1;   // singular pivot: the host path reports the same as a failed LU
+    for (int i = 0; i < n; i++) xout[i] = std::complex<double>(xr[i].r, xr[i].i);
+    *lam = std::complex<double>(out[0], out[1]);
+    return out[2] == 0.0 ? 0 : 1;
+}
+
+}  // namespace cem

@@ -378,3 +378,76 @@ def test_graph_replay_matches_direct_launches(cem, gpu_available, monkeypatch, s
         for p in range(nport):
             assert np.array_equal(a.port_v(p), b.port_v(p)), n
         assert np.array_equal(a.source_spectrum(), b.source_spectrum()), n
+
+
+def _solve_mode(lib, m, f, neff, start=None):
+    L_ = lib.lib
+    rc = L_.cem_mode2d_solve(m, f, neff, start, 300, 1e-13)
+    ny_nz = _solve_mode.sizes[m]
+    nu, nv = ny_nz
+    e = np.concatenate([np.ctypeslib.as_array(L_.cem_mode2d_eu(m), shape=(2 * nu,)),
+                        np.ctypeslib.as_array(L_.cem_mode2d_ev(m), shape=(2 * nv,))]).copy()
+    return rc, L_.cem_mode2d_beta(m), L_.cem_mode2d_alpha(m), L_.cem_mode2d_power(m), e
+
+
+_solve_mode.sizes = {}
+
+
+def _compare_gpu_host(cem, monkeypatch, m, nu, nv, f, neff):
+    """Solve the same cross-section on the GPU and on the host (COURANT_MODE_HOST=1): the
+    eigenvalue to 1e-10 and the normalised, phase-fixed eigenvector to 1e-8 of its largest
+    entry, from the fixed start vector (when the host converges from it) and from a seeded
+    start at a nearby frequency, as the frequency sweep runs it."""
+    _solve_mode.sizes[m] = (nu, nv)
+    monkeypatch.setenv("COURANT_MODE_HOST", "1")
+    h = _solve_mode(cem, m, f, neff)
+    monkeypatch.delenv("COURANT_MODE_HOST")
+    g = _solve_mode(cem, m, f, neff)
+    if h[0] == 0:
+        assert g[0] == 0
+        assert abs(g[1] - h[1]) <= 1e-10 * abs(h[1]), (g[1], h[1])
+        assert abs(g[2] - h[2]) <= 1e-10 * abs(h[1]), (g[2], h[2])
+        assert abs(g[3] - h[3]) <= 1e-8 * abs(h[3])
+        assert np.max(np.abs(g[4] - h[4])) <= 1e-8 * np.max(np.abs(h[4]))
+    seed = h[4].copy()
+    f2, ng = f * 1.01, h[1] / (2 * np.pi * f / L.C0)
+    monkeypatch.setenv("COURANT_MODE_HOST", "1")
+    hs = _solve_mode(cem, m, f2, ng, seed.ctypes.data_as(L._DP))
+    monkeypatch.delenv("COURANT_MODE_HOST")
+    gs = _solve_mode(cem, m, f2, ng, seed.ctypes.data_as(L._DP))
+    assert hs[0] == 0 and gs[0] == 0
+    assert abs(gs[1] - hs[1]) <= 1e-10 * abs(hs[1]), (gs[1], hs[1])
+    assert abs(gs[2] - hs[2]) <= 1e-10 * abs(hs[1]), (gs[2], hs[2])
+    assert abs(gs[3] - hs[3]) <= 1e-8 * abs(hs[3])
+    assert np.max(np.abs(gs[4] - hs[4])) <= 1e-8 * np.max(np.abs(hs[4]))
+    return hs, gs
+
+
+def test_mode2d_gpu_matches_host_waveguide(cem, gpu_available, monkeypatch):
+    """WR-28 cross-section (test_mode2d.c geometry): the GPU band LU + inverse iteration gives
+    the host solver's TE10 eigenpair, from the fixed start vector and from a seeded start."""
+    print_case("mode2d GPU vs host: WR-28")
+    a, b, NY, NZ = 7.112e-3, 3.556e-3, 24, 12
+    y = np.array([a * j / NY for j in range(NY + 1)])
+    z = np.array([b * k / NZ for k in range(NZ + 1)])
+    m = cem.lib.cem_mode2d_create(NY, NZ, y.ctypes.data_as(L._DP), z.ctypes.data_as(L._DP))
+    try:
+        h, g = _compare_gpu_host(cem, monkeypatch, m, NY * (NZ + 1), (NY + 1) * NZ, 35e9, 0.7)
+        print("beta host %.12g gpu %.12g" % (h[1], g[1]))
+    finally:
+        cem.lib.cem_mode2d_destroy(m)
+
+
+def test_mode2d_gpu_matches_host_lossy_port(cem, gpu_available, monkeypatch):
+    """The lossy GCPW port cross-section (Debye substrate, conductor loss, PML window) solved
+    on the GPU and on the host: beta, alpha, power and eigenvector agree."""
+    print_case("mode2d GPU vs host: lossy GCPW port")
+    s, info = S.gcpw(cem, **MODE_MINI)
+    nx, ny, nz = info["cells"]
+    m = cem.lib.cem_cross_section_x(s.h, 3, 0, ny, 0, nz)
+    assert m
+    try:
+        h, g = _compare_gpu_host(cem, monkeypatch, m, ny * (nz + 1), (ny + 1) * nz, 20e9, 1.8)
+        print("beta host %.12g gpu %.12g alpha %.6g / %.6g" % (h[1], g[1], h[2], g[2]))
+    finally:
+        cem.lib.cem_mode2d_destroy(m)
