@@ -470,6 +470,23 @@ extern "C" int cem_mode2d_solve(cem_mode2d *m, double f, double neff_guess, cons
     return rc;
 }
 
+cem_mode2d *cem::mode2d_clone(const cem_mode2d *m) {
+    cem_mode2d *c = new cem_mode2d(*m);
+    c->gpu = nullptr;
+    return c;
+}
+
+void cem::mode2d_copy_state(cem_mode2d *dst, const cem_mode2d *src) {
+    dst->beta = src->beta;
+    dst->freq = src->freq;
+    dst->eu = src->eu;
+    dst->ev = src->ev;
+    dst->hu = src->hu;
+    dst->hv = src->hv;
+}
+
+bool cem::mode2d_on_gpu() { return cem::mode_gpu_enabled(); }
+
 extern "C" double cem_mode2d_beta(const cem_mode2d *m) { return m->beta.real(); }
 extern "C" double cem_mode2d_alpha(const cem_mode2d *m) { return -m->beta.imag(); }
 double cem::mode_freq_hint(const cem_mode2d *m) { return m->freq; }

@@ -38,8 +38,8 @@ namespace {
 constexpr double kMu0 = CEM_MU0;
 constexpr int kLuThreads = 256;
 constexpr int kTrailRows = 8;          // rows per lane of a trailing column (panel rows <= 512)
-constexpr int kInvThreads = 1024;
-constexpr size_t kRingBytes = 40 * 1024;   // LDS ring of band columns for the triangular solves
+constexpr int kInvThreads = 512;
+constexpr size_t kRingBytes = 150 * 1024;  // LDS ring of band columns for the triangular solves (gfx950: 160 KiB per workgroup)
 constexpr int kProbeBatch = 64;        // colours per probing launch
 
 struct zc {
@@ -66,7 +66,19 @@ __device__ __forceinline__ zc zdiv(zc a, zc b) {
     return mk((a.i * ratio + a.r) / den, (a.i - a.r * ratio) / den);
 }
 __device__ __forceinline__ double zabs(zc a) { return hypot(a.r, a.i); }
+// c - a b with four fused multiply-adds (the triangular solves)
+__device__ __forceinline__ zc zfms(zc c, zc a, zc b) {
+    return mk(fma(-a.r, b.r, fma(a.i, b.i, c.r)), fma(-a.r, b.i, fma(-a.i, b.r, c.i)));
+}
 __device__ __forceinline__ zc shfl(zc v, int src) { return mk(__shfl(v.r, src), __shfl(v.i, src)); }
+// value of lane src (uniform) to every lane, through scalar registers
+__device__ __forceinline__ double bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ zc bcast(zc v, int src) { return mk(bcast(v.r, src), bcast(v.i, src)); }
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
@@ -77,6 +89,26 @@ __device__ __forceinline__ void wave_sync_lds() {
 struct Geo {
     int ny, nz, nu, nv, nn, nc, n, bw, ldab, wc;
 };
+
+// dst[t] = src(t) for t = t0, t0 + nt, ... < total: each thread issues U loads
+// before it stores any of them, so a copy costs one memory latency per U
+// elements instead of one per element.
+template <int U, typename F>
+__device__ __forceinline__ void copy_batched(zc *dst, int total, int t0, int nt, F src) {
+    for (int base = t0; base < total; base += nt * U) {
+        zc tmp[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = base + u * nt;
+            tmp[u] = t < total ? src(t) : zc{0.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = base + u * nt;
+            if (t < total) dst[t] = tmp[u];
+        }
+    }
+}
 
 // band entry (i, j) (requires j - 2 bw <= i <= j + bw)
 __device__ __forceinline__ long long bidx(const Geo &g, int i, int j) {
@@ -211,10 +243,10 @@ __global__ void __launch_bounds__(kLuThreads) k_lu_panel(Geo g, zc *band, int *i
     __shared__ double piv_abs;
     const int kl = g.bw, nbp = j1 - j0;
     const int R = min(j1 - 1 + kl, g.n - 1) - j0 + 1;
-    for (int t = threadIdx.x; t < R * nbp; t += kLuThreads) {
+    copy_batched<16>(P, R * nbp, threadIdx.x, kLuThreads, [=](int t) {
         const int cc = t / R, rr = t % R, col = j0 + cc, row = j0 + rr;
-        P[t] = row <= col + kl ? band[bidx(g, row, col)] : mk(0.0, 0.0);
-    }
+        return row <= col + kl ? band[bidx(g, row, col)] : mk(0.0, 0.0);
+    });
     __syncthreads();
     for (int j = j0; j < j1; j++) {
         const int last = min(j + kl, g.n - 1);
@@ -280,10 +312,10 @@ __global__ void __launch_bounds__(kLuThreads) k_lu_trail(Geo g, zc *band, const 
     extern __shared__ zc L[];
     const int kl = g.bw, kv = 2 * g.bw, nbp = j1 - j0;
     const int R = min(j1 - 1 + kl, g.n - 1) - j0 + 1;
-    for (int t = threadIdx.x; t < R * nbp; t += kLuThreads) {
+    copy_batched<16>(L, R * nbp, threadIdx.x, kLuThreads, [=](int t) {
         const int cc = t / R, rr = t % R, col = j0 + cc, row = j0 + rr;
-        L[t] = row > col && row <= col + kl ? band[bidx(g, row, col)] : mk(0.0, 0.0);
-    }
+        return row > col && row <= col + kl ? band[bidx(g, row, col)] : mk(0.0, 0.0);
+    });
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int c = j1 + (int)(blockIdx.x * (kLuThreads / 64) + (threadIdx.x >> 6));
@@ -336,9 +368,9 @@ struct InvArgs {
     int maxit;
     double tol;
     double *out;   // lam (2), rc, iterations
-    int chunk_f, chunk_b;   // band columns per ring half (forward, back)
+    int chunk_f, chunk_b;   // band columns per ring slot (forward, back)
     int ring;               // ring size in entries
-    int wmask;              // window size - 1 (power of two)
+    int timing;             // accumulate cycle counts of the solve phases into out[4..9]
 };
 
 __device__ double block_sum(double v, double *red) {
@@ -352,125 +384,223 @@ __device__ double block_sum(double v, double *red) {
     return s;
 }
 
-// In-place solve of (LU) y = y with the band factors (the host's band_solve):
-// forward with the interchanges, then back substitution.  Wavefront 0 walks the
-// columns through an LDS window of y; the other wavefronts load the next chunk
-// of band columns (the entries the walk needs) into the other half of a ring.
-__device__ void band_solve(const InvArgs &a, zc *win, zc *ring) {
-    const Geo &g = a.g;
-    const int chunk_f = a.chunk_f, chunk_b = a.chunk_b, wm = a.wmask;
-    const int n = g.n, kl = g.bw, kv = 2 * g.bw;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    zc *y = a.y;
-    // ---- forward: for j: swap y[j], y[p]; y[j+1..j+kl] -= L(:, j) y[j]
-    {
-        const int K = kl, nchunk = (n + chunk_f - 1) / chunk_f;
-        // window holds y[lo, hi) at win[i & wm]
-        for (int i = threadIdx.x; i < min(n, kl + chunk_f); i += kInvThreads) win[i & wm] = y[i];
-        int hi = min(n, kl + chunk_f);
-        // preload chunk 0 multipliers
-        for (int t = threadIdx.x; t < chunk_f * K; t += kInvThreads) {
-            const int jj = t / K, rr = t % K, j = jj, i = j + 1 + rr;
-            ring[t] = (j < n && i < n) ? a.band[bidx(g, i, j)] : mk(0.0, 0.0);
-        }
-        __syncthreads();
-        for (int ch = 0; ch < nchunk; ch++) {
-            const int cj0 = ch * chunk_f, cj1 = min(n, cj0 + chunk_f);
-            zc *cur = ring + (size_t)(ch & 1) * chunk_f * K;
-            zc *nxt = ring + (size_t)((ch + 1) & 1) * chunk_f * K;
-            if (wv == 0) {
-                for (int j = cj0; j < cj1; j++) {
-                    const int p = a.ipiv[j], last = min(j + kl, n - 1);
-                    if (lane == 0 && p != j) {
-                        const zc t = win[j & wm];
-                        win[j & wm] = win[p & wm];
-                        win[p & wm] = t;
-                    }
-                    wave_sync_lds();
-                    const zc xj = win[j & wm];
-                    const zc *l = cur + (size_t)(j - cj0) * K;
-                    for (int i = j + 1 + lane; i <= last; i += 64)
-                        win[i & wm] = zsub(win[i & wm], zmul(l[i - j - 1], xj));
-                    wave_sync_lds();
-                }
-            } else {
-                // next chunk's multipliers, and the y entries entering the window
-                const int nj0 = cj1;
-                for (int t = threadIdx.x - 64; t < chunk_f * K; t += kInvThreads - 64) {
-                    const int j = nj0 + t / K, i = j + 1 + t % K;
-                    nxt[t] = (j < n && i < n) ? a.band[bidx(g, i, j)] : mk(0.0, 0.0);
-                }
-            }
-            __syncthreads();
-            // y[cj0, cj1) is final: store; bring y[hi, min(n, cj1 + chunk_f + kl)) in
-            const int nhi = min(n, cj1 + chunk_f + kl);
-            for (int i = cj0 + (int)threadIdx.x; i < cj1; i += kInvThreads) y[i] = win[i & wm];
-            for (int i = hi + (int)threadIdx.x; i < nhi; i += kInvThreads) win[i & wm] = y[i];
-            hi = max(hi, nhi);
-            __syncthreads();
-        }
+// Window of the right-hand side held in registers of one wavefront: slot k of
+// lane l holds entry 64 (b0 + k) + l, b0 the current block; the slots rotate
+// by one at every block change.  A uniform slot s is selected through a bit
+// mask, so every access stays a register access.
+template <int MAXS>
+__device__ __forceinline__ zc slot_get(const zc (&v)[MAXS], unsigned m) {
+    zc r = mk(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < MAXS; k++) {
+        const bool on = (m >> k) & 1u;
+        r.r = on ? v[k].r : r.r;
+        r.i = on ? v[k].i : r.i;
     }
-    // ---- back: for j = n-1..0: y[j] /= U(j, j); y[j-kv..j-1] -= U(:, j) y[j]
-    {
-        const int K = kv + 1, nchunk = (n + chunk_b - 1) / chunk_b;
-        // chunk ch covers columns [n - (ch+1) chunk_b, n - ch chunk_b) walked downwards
-        const int lo0 = max(0, n - kv - chunk_b);
-        for (int i = lo0 + (int)threadIdx.x; i < n; i += kInvThreads) win[i & wm] = y[i];
-        int lo = lo0;
-        for (int t = threadIdx.x; t < chunk_b * K; t += kInvThreads) {
-            const int jj = t / K, rr = t % K, j = n - 1 - jj, i = j - kv + rr;
-            ring[t] = (j >= 0 && i >= 0) ? a.band[bidx(g, i, j)] : mk(0.0, 0.0);
-        }
-        __syncthreads();
-        for (int ch = 0; ch < nchunk; ch++) {
-            const int cj1 = n - ch * chunk_b, cj0 = max(0, cj1 - chunk_b);
-            zc *cur = ring + (size_t)(ch & 1) * chunk_b * K;
-            zc *nxt = ring + (size_t)((ch + 1) & 1) * chunk_b * K;
-            if (wv == 0) {
-                for (int j = cj1 - 1; j >= cj0; j--) {
-                    const zc *u = cur + (size_t)(cj1 - 1 - j) * K;   // u[rr] = U(j - kv + rr, j)
-                    if (lane == 0) win[j & wm] = zdiv(win[j & wm], u[kv]);
-                    wave_sync_lds();
-                    const zc xj = win[j & wm];
-                    const int first = max(0, j - kv);
-                    for (int i = first + lane; i < j; i += 64)
-                        win[i & wm] = zsub(win[i & wm], zmul(u[i - (j - kv)], xj));
-                    wave_sync_lds();
-                }
-            } else {
-                const int nj1 = cj0;   // next chunk: columns [nj1 - chunk_b, nj1) downwards
-                for (int t = threadIdx.x - 64; t < chunk_b * K; t += kInvThreads - 64) {
-                    const int j = nj1 - 1 - t / K, i = j - kv + t % K;
-                    nxt[t] = (j >= 0 && i >= 0) ? a.band[bidx(g, i, j)] : mk(0.0, 0.0);
-                }
-            }
-            __syncthreads();
-            const int nlo = max(0, cj0 - chunk_b - kv);
-            for (int i = cj0 + (int)threadIdx.x; i < cj1; i += kInvThreads) y[i] = win[i & wm];
-            for (int i = nlo + (int)threadIdx.x; i < lo; i += kInvThreads) win[i & wm] = y[i];
-            lo = min(lo, nlo);
-            __syncthreads();
-        }
+    return r;
+}
+template <int MAXS>
+__device__ __forceinline__ void slot_put(zc (&v)[MAXS], unsigned m, zc x) {
+#pragma unroll
+    for (int k = 0; k < MAXS; k++) {
+        const bool on = (m >> k) & 1u;
+        v[k].r = on ? x.r : v[k].r;
+        v[k].i = on ? x.i : v[k].i;
     }
 }
 
+// Ring fills: multipliers L(j+1 .. j+kl, j) and pivots of the columns of forward
+// chunk ch; U(j-kv .. j, j) of the columns of back chunk ch (walked downwards).
+__device__ __forceinline__ void fill_fwd(const Geo g, const zc *__restrict__ band, const int *__restrict__ ipiv,
+                                         zc *ring, int *rpiv, int CH, int ch, int t0, int nt) {
+    const int K = g.bw, n = g.n;
+    zc *dst = ring + (size_t)(ch % 3) * CH * K;
+    int *pdst = rpiv + (ch % 3) * CH;
+    copy_batched<8>(dst, CH * K, t0, nt, [=](int t) {
+        const int j = ch * CH + t / K, i = j + 1 + t % K;
+        return (j < n && i < n) ? band[bidx(g, i, j)] : mk(0.0, 0.0);
+    });
+    for (int t = t0; t < CH; t += nt) pdst[t] = ch * CH + t < n ? ipiv[ch * CH + t] : 0;
+}
+
+__device__ __forceinline__ void fill_back(const Geo g, const zc *__restrict__ band, zc *ring, int CH, int ch, int t0,
+                                          int nt) {
+    const int kv = 2 * g.bw, K = kv + 1, n = g.n;
+    zc *dst = ring + (size_t)(ch % 3) * CH * K;
+    copy_batched<8>(dst, CH * K, t0, nt, [=](int t) {
+        const int j = n - 1 - ch * CH - t / K, i = j - kv + t % K;
+        if (j < 0 || i < 0) return mk(0.0, 0.0);
+        const zc u = band[bidx(g, i, j)];
+        return i == j ? zdiv(mk(1.0, 0.0), u) : u;   // the diagonal as its reciprocal
+    });
+}
+
+// In-place solve of (LU) y = y with the band factors (the host's band_solve):
+// forward with the interchanges, then back substitution.  Wavefront 0 walks
+// the columns with the live part of y in registers (one 64-entry block moved
+// out and the next loaded one block ahead); the other wavefronts stream the
+// band columns the walk needs into a three-deep LDS ring, two chunks ahead,
+// with one workgroup barrier per chunk.
+template <int SF, int SB>
+__device__ void band_solve(const Geo g, const zc *__restrict__ band, const int *__restrict__ ipiv, zc *y,
+                           int chunk_f, int chunk_b, zc *ring, int *rpiv, long long *tm) {
+    const int n = g.n, kl = g.bw, kv = 2 * g.bw;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const zc zero = mk(0.0, 0.0);
+    // ---- forward: for j: swap y[j], y[p]; y[j+1..j+kl] -= L(:, j) y[j]
+    {
+        constexpr int MAXS = SF, S = SF;
+        const int K = kl, CH = chunk_f, nchunk = (n + CH - 1) / CH;
+        fill_fwd(g, band, ipiv, ring, rpiv, CH, 0, threadIdx.x, kInvThreads);
+        if (nchunk > 1) fill_fwd(g, band, ipiv, ring, rpiv, CH, 1, threadIdx.x, kInvThreads);
+        __syncthreads();
+        zc v[MAXS], pend = zero;   // slot k: block (j >> 6) + k
+        const int nblk = (n + 63) / 64;
+        if (wv == 0) {
+#pragma unroll
+            for (int k = 0; k < MAXS; k++) {
+                const int i = 64 * k + lane;
+                v[k] = k < S && i < n ? y[i] : zero;
+            }
+            const int i = 64 * S + lane;
+            pend = i < n ? y[i] : zero;
+        }
+        for (int ch = 0; ch < nchunk; ch++) {
+            const int cj0 = ch * CH, cj1 = min(n, cj0 + CH);
+            const long long c0 = tm ? clock64() : 0;
+            if (wv == 0) {
+                const zc *cur = ring + (size_t)(ch % 3) * CH * K;
+                const int *cp = rpiv + (ch % 3) * CH;
+                for (int j = cj0; j < cj1; j++) {
+                    const int jb = j >> 6;
+                    if ((j & 63) == 0 && jb > 0) {
+                        // block jb-1 is final: out to y; slots move down, block jb-1+S enters
+                        if (64 * (jb - 1) + lane < n) y[64 * (jb - 1) + lane] = v[0];
+#pragma unroll
+                        for (int k = 0; k + 1 < MAXS; k++) v[k] = v[k + 1];
+                        slot_put(v, 1u << (S - 1), pend);
+                        const int i = 64 * (jb + S) + lane;
+                        pend = jb + S < nblk && i < n ? y[i] : zero;
+                    }
+                    const int p = cp[j - cj0], last = min(j + kl, n - 1);
+                    zc vj = bcast(v[0], j & 63);
+                    if (p != j) {
+                        const unsigned mp = 1u << ((p >> 6) - jb);
+                        const zc vp = bcast(slot_get(v, mp), p & 63);
+                        if (lane == (j & 63)) v[0] = vp;
+                        if (lane == (p & 63)) slot_put(v, mp, vj);
+                        vj = vp;
+                    }
+                    const zc *l = cur + (size_t)(j - cj0) * K;
+#pragma unroll
+                    for (int k = 0; k < MAXS; k++) {
+                        const int i = 64 * (jb + k) + lane;
+                        if (k < S && i > j && i <= last) v[k] = zfms(v[k], l[i - j - 1], vj);
+                    }
+                }
+            } else if (ch + 2 < nchunk) {
+                fill_fwd(g, band, ipiv, ring, rpiv, CH, ch + 2, threadIdx.x - 64, kInvThreads - 64);
+            }
+            const long long c1 = tm ? clock64() : 0;
+            __syncthreads();
+            if (tm && threadIdx.x == 0) {
+                tm[0] += c1 - c0;
+                tm[1] += clock64() - c1;
+            }
+            if (tm && threadIdx.x == 64) tm[4] += c1 - c0;
+        }
+        if (wv == 0) {   // the last block (every earlier one went out at a block start)
+            const int jb = (n - 1) >> 6;
+            if (64 * jb + lane < n) y[64 * jb + lane] = v[0];
+        }
+        __syncthreads();
+    }
+    // ---- back: for j = n-1..0: y[j] /= U(j, j); y[j-kv..j-1] -= U(:, j) y[j]
+    {
+        constexpr int MAXS = SB, S = SB;
+        const int K = kv + 1, CH = chunk_b, nchunk = (n + CH - 1) / CH;
+        fill_back(g, band, ring, CH, 0, threadIdx.x, kInvThreads);
+        if (nchunk > 1) fill_back(g, band, ring, CH, 1, threadIdx.x, kInvThreads);
+        __syncthreads();
+        zc v[MAXS], pend = zero;   // slot k: block (j >> 6) - k
+        const int top = (n - 1) >> 6;
+        if (wv == 0) {
+#pragma unroll
+            for (int k = 0; k < MAXS; k++) {
+                const int b = top - k, i = 64 * b + lane;
+                v[k] = k < S && b >= 0 && i < n ? y[i] : zero;
+            }
+            pend = top - S >= 0 ? y[64 * (top - S) + lane] : zero;
+        }
+        for (int ch = 0; ch < nchunk; ch++) {
+            const int cj1 = n - ch * CH, cj0 = max(0, cj1 - CH);
+            const long long c0 = tm ? clock64() : 0;
+            if (wv == 0) {
+                const zc *cur = ring + (size_t)(ch % 3) * CH * K;
+                for (int j = cj1 - 1; j >= cj0; j--) {
+                    const int jb = j >> 6;
+                    if ((j & 63) == 63 && jb < top) {
+                        // block jb+1 is final: out to y; slots move up, block jb+1-S enters
+                        if (64 * (jb + 1) + lane < n) y[64 * (jb + 1) + lane] = v[0];
+#pragma unroll
+                        for (int k = 0; k + 1 < MAXS; k++) v[k] = v[k + 1];
+                        slot_put(v, 1u << (S - 1), pend);
+                        pend = jb - S >= 0 ? y[64 * (jb - S) + lane] : zero;
+                    }
+                    const zc *u = cur + (size_t)(cj1 - 1 - j) * K;   // u[rr] = U(j - kv + rr, j), u[kv] = 1/U(j, j)
+                    const zc xj = zmul(bcast(v[0], j & 63), u[kv]);
+                    if (lane == (j & 63)) v[0] = xj;
+                    const int first = max(0, j - kv);
+#pragma unroll
+                    for (int k = 0; k < MAXS; k++) {
+                        const int i = 64 * (jb - k) + lane;
+                        if (k < S && i >= first && i < j) v[k] = zfms(v[k], u[i - (j - kv)], xj);
+                    }
+                }
+            } else if (ch + 2 < nchunk) {
+                fill_back(g, band, ring, CH, ch + 2, threadIdx.x - 64, kInvThreads - 64);
+            }
+            const long long c1 = tm ? clock64() : 0;
+            __syncthreads();
+            if (tm && threadIdx.x == 0) {
+                tm[2] += c1 - c0;
+                tm[3] += clock64() - c1;
+            }
+            if (tm && threadIdx.x == 64) tm[5] += c1 - c0;
+        }
+        if (wv == 0) {   // block 0 (every later one went out at a block change)
+            if (lane < n) y[lane] = v[0];
+        }
+        __syncthreads();
+    }
+}
+
+template <int SF, int SB>
 __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
-    extern __shared__ zc ring[];   // ring (a.ring entries), then the window (wmask + 1 entries)
-    zc *win = ring + a.ring;
+    extern __shared__ zc ring[];   // three chunks of band columns, then their pivots
+    int *rpiv = reinterpret_cast<int *>(ring + a.ring);
     __shared__ double red[kInvThreads / 64];
+    __shared__ long long tmr[8];
+    if (threadIdx.x < 8) tmr[threadIdx.x] = 0;
+    __syncthreads();
     const int n = a.g.n;
+    zc *__restrict__ X = a.x;
+    zc *__restrict__ XO = a.xo;
+    zc *__restrict__ Y = a.y;
     zc lam = a.sigma, lam_old = mk(0.0, 0.0);
     int rc = 1, it = 0;
     for (it = 0; it < a.maxit; it++) {
         double s = 0.0;
-        for (int q = threadIdx.x; q < n; q += kInvThreads) s += a.x[q].r * a.x[q].r + a.x[q].i * a.x[q].i;
+        #pragma unroll 8
+        for (int q = threadIdx.x; q < n; q += kInvThreads) s += X[q].r * X[q].r + X[q].i * X[q].i;
         const double nx2 = block_sum(s, red);
         const double inv = 1.0 / sqrt(nx2);
         double pr = 0.0, pi = 0.0;
+        #pragma unroll 8
         for (int q = threadIdx.x; q < n; q += kInvThreads) {
-            const zc v = zrs(inv, a.x[q]);
-            a.x[q] = v;
-            const zc t = zmul(zconj(a.xo[q]), v);
+            const zc v = zrs(inv, X[q]);
+            X[q] = v;
+            const zc t = zmul(zconj(XO[q]), v);
             pr += t.r;
             pi += t.i;
         }
@@ -480,28 +610,32 @@ __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
         if (aph > 0.0) {
             ph = mk(ph.r / aph, -ph.i / aph);
             double d = 0.0;
+            #pragma unroll 8
             for (int q = threadIdx.x; q < n; q += kInvThreads) {
-                const zc e = zsub(zmul(a.x[q], ph), a.xo[q]);
+                const zc e = zsub(zmul(X[q], ph), XO[q]);
                 d += e.r * e.r + e.i * e.i;
             }
             dv = block_sum(d, red);
         }
+        #pragma unroll 8
         for (int q = threadIdx.x; q < n; q += kInvThreads) {
-            a.xo[q] = a.x[q];
-            a.y[q] = a.x[q];
+            XO[q] = X[q];
+            Y[q] = X[q];
         }
         __syncthreads();
-        band_solve(a, win, ring);
+        band_solve<SF, SB>(a.g, a.band, a.ipiv, a.y, a.chunk_f, a.chunk_b, ring, rpiv, a.timing ? tmr : nullptr);
         __syncthreads();
         double mr = 0.0, mi = 0.0;
+        #pragma unroll 8
         for (int q = threadIdx.x; q < n; q += kInvThreads) {
-            const zc t = zmul(zconj(a.x[q]), a.y[q]);
+            const zc t = zmul(zconj(X[q]), Y[q]);
             mr += t.r;
             mi += t.i;
         }
         const zc mu = mk(block_sum(mr, red), block_sum(mi, red));
         lam = zadd(a.sigma, zdiv(mk(1.0, 0.0), mu));
-        for (int q = threadIdx.x; q < n; q += kInvThreads) a.x[q] = a.y[q];
+        #pragma unroll 8
+        for (int q = threadIdx.x; q < n; q += kInvThreads) X[q] = Y[q];
         __syncthreads();
         if (it > 2 && zabs(zsub(lam, lam_old)) < a.tol * zabs(lam) && sqrt(dv) < 1e-10) {
             rc = 0;
@@ -514,6 +648,7 @@ __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
         a.out[1] = lam.i;
         a.out[2] = rc;
         a.out[3] = it;
+        for (int k = 0; k < 6; k++) a.out[4 + k] = (double)tmr[k];
     }
 }
 
@@ -539,6 +674,16 @@ struct DMem {
         n = 0;
     }
 };
+
+// dynamic LDS beyond 64 KiB needs the per-kernel attribute (gfx950: up to 160 KiB)
+template <int SF, int SB>
+int run_inverse(size_t lds, hipStream_t st, const InvArgs &ia) {
+    if (hipFuncSetAttribute((const void *)k_inverse<SF, SB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess)
+        return -1;
+    hipLaunchKernelGGL((k_inverse<SF, SB>), dim3(1), dim3(kInvThreads), lds, st, ia);
+    return 0;
+}
 
 }  // namespace
 
@@ -605,18 +750,18 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     g.ldab = 3 * g.bw + 1;
     g.wc = 2 * g.bw + 1;
     const int n = g.n, kl = g.bw, kv = 2 * g.bw;
-    // panel width: the panel (kl + nb rows x nb columns) fits 60 KiB of LDS
+    // panel width: the panel (kl + nb rows x nb columns) fits 60 KiB of LDS (wider
+    // panels measured slower: the trailing walk per column grows with nb)
     int nb = 32;
     while (nb > 1 && (size_t)(kl + nb) * nb * sizeof(zc) > 60 * 1024) nb /= 2;
-    // triangular solves: two halves of chunk columns in the ring; the window of the
-    // right-hand side spans at most 2 chunk + kl (forward) or 2 chunk + kv + 1 (back)
-    const int chunk_f = std::max(1, (int)(kRingBytes / (2 * sizeof(zc) * kl)));
-    const int chunk_b = std::max(1, (int)(kRingBytes / (2 * sizeof(zc) * (kv + 1))));
-    const int ring = std::max(2 * chunk_f * kl, 2 * chunk_b * (kv + 1));
-    int wn = 64;
-    while (wn < std::max(2 * chunk_f + kl, 2 * chunk_b + kv + 1)) wn *= 2;
-    const size_t inv_lds = (size_t)(ring + wn) * sizeof(zc);
-    if (kl + nb > 64 * kTrailRows || inv_lds > 60 * 1024) {
+    // triangular solves: three slots of chunk columns in the LDS ring; register
+    // window of S = ceil(K / 64) + 1 blocks of 64 entries
+    const int chunk_f = std::max(1, (int)(kRingBytes / (3 * sizeof(zc) * kl + 3 * sizeof(int))));
+    const int chunk_b = std::max(1, (int)(kRingBytes / (3 * sizeof(zc) * (kv + 1))));
+    const int ring = std::max(3 * chunk_f * kl, 3 * chunk_b * (kv + 1));
+    const int sf = (kl + 63) / 64 + 1, sb = (kv + 63) / 64 + 1;
+    const size_t inv_lds = (size_t)ring * sizeof(zc) + (size_t)3 * chunk_f * sizeof(int);
+    if (kl + nb > 64 * kTrailRows || sf > 8 || inv_lds > kRingBytes + 4096) {
         set_error("GPU mode solver: port window too wide (nz)");
         return -1;
     }
@@ -625,11 +770,16 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     if (!M.band.get((size_t)g.ldab * n) || !M.coef.get(ncoef) || !M.x.get(n) || !M.xo.get(n) || !M.y.get(n) ||
         !M.hw.get((size_t)nbatch * g.nc) || !M.hu.get((size_t)nbatch * g.nv) || !M.hv.get((size_t)nbatch * g.nu) ||
         !M.ew.get((size_t)nbatch * g.nn) || !M.pec.get(g.nu + g.nv) || !M.ipiv.get(n) || !M.info.get(1) ||
-        !M.out.get(4)) {
+        !M.out.get(10)) {
         set_error("GPU mode solver: device out of memory");
         return -1;
     }
     hipStream_t st = M.st;
+    const bool debug = std::getenv("COURANT_MODE_DEBUG") != nullptr;
+    hipEvent_t ev[4] = {};
+    if (debug)
+        for (auto &e : ev) MG_CHECK(hipEventCreate(&e));
+    if (debug) MG_CHECK(hipEventRecord(ev[0], st));
     // coefficients (host-built, FP64 complex) and PEC masks
     M.hcoef.resize(ncoef);
     size_t o = 0;
@@ -681,11 +831,18 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     }
     hipLaunchKernelGGL(k_diag, dim3((n + T - 1) / T), dim3(T), 0, st, g, c, zc{p.sigma.real(), p.sigma.imag()}, p.big,
                        M.band.p);
+    if (debug) MG_CHECK(hipEventRecord(ev[1], st));
     // band LU, panel by panel
+    size_t lu_lds_set = 0;
     for (int j0 = 0; j0 < n; j0 += nb) {
         const int j1 = std::min(n, j0 + nb);
         const int R = std::min(j1 - 1 + kl, n - 1) - j0 + 1;
         const size_t lds = (size_t)R * (j1 - j0) * sizeof(zc);
+        if (lds > lu_lds_set) {   // dynamic LDS beyond 64 KiB needs the per-kernel attribute
+            MG_CHECK(hipFuncSetAttribute((const void *)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            MG_CHECK(hipFuncSetAttribute((const void *)k_lu_trail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            lu_lds_set = lds;
+        }
         hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(kLuThreads), lds, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
         const int ncol = std::min(j1 - 1 + kv, n - 1) - j1 + 1;
         if (ncol > 0) {
@@ -694,6 +851,7 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
                                M.ipiv.p, j0, j1);
         }
     }
+    if (debug) MG_CHECK(hipEventRecord(ev[2], st));
     InvArgs ia;
     ia.g = g;
     ia.band = M.band.p;
@@ -708,16 +866,36 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     ia.chunk_f = chunk_f;
     ia.chunk_b = chunk_b;
     ia.ring = ring;
-    ia.wmask = wn - 1;
-    hipLaunchKernelGGL(k_inverse, dim3(1), dim3(kInvThreads), inv_lds, st, ia);
+    ia.timing = debug ? 1 : 0;
+    // the walk is compiled for the exact register-window sizes; sb is 2 sf - 2 or 2 sf - 1
+    int lr = 1;
+#define MG_INV(SFV, SBV) \
+    if (sf == SFV && sb == SBV) lr = run_inverse<SFV, SBV>(inv_lds, st, ia);
+    MG_INV(2, 2) MG_INV(2, 3) MG_INV(3, 4) MG_INV(3, 5) MG_INV(4, 6) MG_INV(4, 7) MG_INV(5, 8) MG_INV(5, 9)
+    MG_INV(6, 10) MG_INV(6, 11) MG_INV(7, 12) MG_INV(7, 13) MG_INV(8, 14) MG_INV(8, 15)
+#undef MG_INV
+    if (lr) {
+        set_error(lr > 0 ? "GPU mode solver: port window too wide (nz)" : "GPU mode solver: launch failed");
+        return -1;
+    }
     MG_CHECK(hipGetLastError());
+    if (debug) MG_CHECK(hipEventRecord(ev[3], st));
     int info = 0;
-    double out[4];
+    double out[10];
     std::vector<zc> xr(n);
     MG_CHECK(hipMemcpyAsync(&info, M.info.p, sizeof(int), hipMemcpyDeviceToHost, st));
     MG_CHECK(hipMemcpyAsync(out, M.out.p, sizeof out, hipMemcpyDeviceToHost, st));
     MG_CHECK(hipMemcpyAsync(xr.data(), M.x.p, n * sizeof(zc), hipMemcpyDeviceToHost, st));
     MG_CHECK(hipStreamSynchronize(st));
+    if (debug) {
+        float t[3];
+        for (int k = 0; k < 3; k++) MG_CHECK(hipEventElapsedTime(&t[k], ev[k], ev[k + 1]));
+        std::fprintf(stderr, "mode_gpu: n %d bw %d nb %d chunks %d/%d | assembly %.2f ms, LU %.2f ms, %d iterations %.2f ms\n",
+                     n, kl, nb, chunk_f, chunk_b, t[0], t[1], (int)out[3] + 1, t[2]);
+        std::fprintf(stderr, "mode_gpu: cycles fwd walk %.3g wait %.3g | back walk %.3g wait %.3g | fill fwd %.3g back %.3g\n",
+                     out[4], out[5], out[6], out[7], out[8], out[9]);
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
     if (info) return -1;   // singular pivot: the host path reports the same as a failed LU
     for (int i = 0; i < n; i++) xout[i] = std::complex<double>(xr[i].r, xr[i].i);
     *lam = std::complex<double>(out[0], out[1]);

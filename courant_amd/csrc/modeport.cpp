@@ -10,6 +10,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 using namespace cem;
@@ -144,56 +146,117 @@ void cem::modeports_free(cem_sim *s) {
 
 // Per output frequency: solve the port mode, walking outward from the frequency
 // nearest f0 and seeding each solve with the previous mode (modeport.c:146-201).
-static int solve_port_modes(cem_sim *s, ModePort &p) {
+// One sweep of the output frequencies outward from q0 (step +1 or -1), each solve
+// seeded with the previous mode (modeport.c:160-168), on the cross-section xs.
+static int sweep_port(cem_sim *s, ModePort &p, cem_mode2d *xs, int q0, int step, bool write_q0) {
     const int nf = s->nf;
     const cem_mode_port_def &d = p.def;
+    const int ny = d.j1 - d.j0, nz = d.k1 - d.k0;
+    const int nst = ny * (nz + 1) + (ny + 1) * nz;
+    std::vector<double> start(2 * nst);
+    const double dx = s->d[0][d.i_mon];
+    cem_mode2d_solve(xs, warp(s, s->freq[q0]), d.neff0, nullptr, 300, 1e-13);
+    for (int q = q0; q >= 0 && q < nf; q += step) {
+        double fw = warp(s, s->freq[q]);
+        double ng = cem_mode2d_beta(xs) / (2 * M_PI * mode_freq_hint(xs) / CEM_C0);
+        std::memcpy(start.data(), cem_mode2d_eu(xs), 2 * ny * (nz + 1) * sizeof(double));
+        std::memcpy(start.data() + 2 * ny * (nz + 1), cem_mode2d_ev(xs), 2 * (ny + 1) * nz * sizeof(double));
+        if (cem_mode2d_solve(xs, fw, ng, start.data(), 300, 1e-13)) {
+            set_error("mode port: eigenmode did not converge at an output frequency");
+            return -1;
+        }
+        if (q == q0 && !write_q0) continue;
+        cplx kc(cem_mode2d_beta(xs), -cem_mode2d_alpha(xs));
+        cplx b3 = 2 / dx * std::asin(kc * dx / 2.0);
+        p.kappa[2 * q] = kc.real();
+        p.kappa[2 * q + 1] = kc.imag();
+        p.beta[q] = b3.real();
+        p.alpha[q] = -b3.imag();
+        if (s->sibc_on) {
+            double zr, zi;
+            cem_conductor_zs(s, s->freq[q], &zr, &zi);
+            p.alpha_c[q] = cem_mode2d_conductor_alpha(xs, zr);
+            p.alpha[q] += p.alpha_c[q];
+        }
+        double V[2], Iv[2];
+        cem_mode2d_vi(xs, d.vj - d.j0, d.vk0 - d.k0, d.vk1 - d.k0, d.ja - d.j0, d.jb - d.j0, d.ka - d.k0,
+                      d.kb - d.k0, V, Iv);
+        // cem_mode2d_vi returns -sum Ez dz; the port voltage is vsign * sum Ez dz
+        cplx Z = (double)(-d.vsign) * cplx(V[0], V[1]) / cplx(Iv[0], Iv[1]);
+        p.z0[2 * q] = Z.real();
+        p.z0[2 * q + 1] = Z.imag();
+    }
+    return 0;
+}
+
+static int nearest_freq(const cem_sim *s, double f0) {
+    int q0 = 0;
+    for (int q = 0; q < s->nf; q++)
+        if (std::fabs(s->freq[q] - f0) < std::fabs(s->freq[q0] - f0)) q0 = q;
+    return q0;
+}
+
+static void reset_port(cem_sim *s, ModePort &p) {
+    const int nf = s->nf;
     p.z0.assign(2 * nf, 0.0);
     p.beta.assign(nf, 0.0);
     p.alpha.assign(nf, 0.0);
     p.kappa.assign(2 * nf, 0.0);
     p.alpha_c.assign(nf, 0.0);
-    const int ny = d.j1 - d.j0, nz = d.k1 - d.k0;
-    const int nst = ny * (nz + 1) + (ny + 1) * nz;
-    std::vector<double> start(2 * nst);
-    const double dx = s->d[0][d.i_mon];
-    int q0 = 0;
-    for (int q = 0; q < nf; q++)
-        if (std::fabs(s->freq[q] - d.f0) < std::fabs(s->freq[q0] - d.f0)) q0 = q;
-    for (int pass = 0; pass < 2; pass++) {
-        cem_mode2d_solve(p.xs, warp(s, s->freq[q0]), d.neff0, nullptr, 300, 1e-13);
-        int step = pass == 0 ? 1 : -1;
-        for (int q = q0; q >= 0 && q < nf; q += step) {
-            double fw = warp(s, s->freq[q]);
-            double ng = cem_mode2d_beta(p.xs) / (2 * M_PI * mode_freq_hint(p.xs) / CEM_C0);
-            std::memcpy(start.data(), cem_mode2d_eu(p.xs), 2 * ny * (nz + 1) * sizeof(double));
-            std::memcpy(start.data() + 2 * ny * (nz + 1), cem_mode2d_ev(p.xs), 2 * (ny + 1) * nz * sizeof(double));
-            if (cem_mode2d_solve(p.xs, fw, ng, start.data(), 300, 1e-13)) {
-                set_error("mode port: eigenmode did not converge at an output frequency");
-                return -1;
-            }
-            cplx kc(cem_mode2d_beta(p.xs), -cem_mode2d_alpha(p.xs));
-            cplx b3 = 2 / dx * std::asin(kc * dx / 2.0);
-            p.kappa[2 * q] = kc.real();
-            p.kappa[2 * q + 1] = kc.imag();
-            p.beta[q] = b3.real();
-            p.alpha[q] = -b3.imag();
-            if (s->sibc_on) {
-                double zr, zi;
-                cem_conductor_zs(s, s->freq[q], &zr, &zi);
-                p.alpha_c[q] = cem_mode2d_conductor_alpha(p.xs, zr);
-                p.alpha[q] += p.alpha_c[q];
-            }
-            double V[2], Iv[2];
-            cem_mode2d_vi(p.xs, d.vj - d.j0, d.vk0 - d.k0, d.vk1 - d.k0, d.ja - d.j0, d.jb - d.j0, d.ka - d.k0,
-                          d.kb - d.k0, V, Iv);
-            // cem_mode2d_vi returns -sum Ez dz; the port voltage is vsign * sum Ez dz
-            cplx Z = (double)(-d.vsign) * cplx(V[0], V[1]) / cplx(Iv[0], Iv[1]);
-            p.z0[2 * q] = Z.real();
-            p.z0[2 * q + 1] = Z.imag();
-        }
-    }
+}
+
+// Port quantities at every output frequency: two sweeps from the frequency nearest
+// f0, upwards then downwards (modeport.c:146-201).
+static int solve_port_modes(cem_sim *s, ModePort &p) {
+    reset_port(s, p);
+    const int q0 = nearest_freq(s, p.def.f0);
+    for (int pass = 0; pass < 2; pass++)
+        if (sweep_port(s, p, p.xs, q0, pass == 0 ? 1 : -1, true)) return -1;
     p.solved = true;
     return 0;
+}
+
+// With the GPU solver, the sweeps of every unsolved port run at once, one host
+// thread and one device context each (a clone of the port's cross-section for the
+// downward sweep).  Each sweep is the sequence the sequential solve runs, and the
+// device solve is deterministic, so the results are the same; the downward sweep
+// leaves q0 to the upward one, which computes the same value.
+static int solve_all_ports(cem_sim *s) {
+    struct Job {
+        ModePort *p;
+        cem_mode2d *xs;
+        int step;
+        int rc = 0;
+        std::string err;
+    };
+    std::vector<Job> jobs;
+    for (auto &p : s->mport)
+        if (!p.solved && p.xs) {
+            reset_port(s, p);
+            jobs.push_back({&p, p.xs, 1});
+            jobs.push_back({&p, mode2d_clone(p.xs), -1});
+        }
+    std::vector<std::thread> th;
+    for (auto &jb : jobs)
+        th.emplace_back([s, &jb]() {
+            const int q0 = nearest_freq(s, jb.p->def.f0);
+            jb.rc = sweep_port(s, *jb.p, jb.xs, q0, jb.step, jb.step == 1);
+            if (jb.rc) jb.err = cem_last_error();
+        });
+    for (auto &t : th) t.join();
+    int rc = 0;
+    for (size_t k = 0; k < jobs.size(); k += 2) {
+        Job &up = jobs[k], &down = jobs[k + 1];
+        mode2d_copy_state(up.p->xs, down.xs);   // the cross-section ends where the sequential solve ends
+        cem_mode2d_destroy(down.xs);
+        if (up.rc || down.rc) {
+            set_error((up.rc ? up.err : down.err).c_str());
+            rc = -1;
+        } else {
+            up.p->solved = true;
+        }
+    }
+    return rc;
 }
 
 // Incident and outgoing waves at the reference plane (modeport.c:203-227)
@@ -203,7 +266,11 @@ extern "C" int cem_mode_port_waves(cem_sim *s, int port, double *a, double *b, d
         return -1;
     }
     ModePort &p = s->mport[port];
-    if (!p.solved && solve_port_modes(s, p)) return -1;
+    if (!p.solved) {
+        const int rc = mode2d_on_gpu() ? solve_all_ports(s) : solve_port_modes(s, p);
+        if (!p.solved) return -1;
+        (void)rc;
+    }
     const double *V0 = cem_vprobe(s, p.vp0), *V1 = cem_vprobe(s, p.vp1), *Ix = cem_iprobe(s, p.ip);
     if (!V0 || !V1 || !Ix) return -1;
     const cem_mode_port_def &d = p.def;

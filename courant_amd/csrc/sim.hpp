@@ -148,6 +148,12 @@ void modeports_free(cem_sim *s);
 double modeport_waveform(const cem_sim *s, double t);
 double lumped_waveform(const cem_sim *s, double t);
 double mode_freq_hint(const cem_mode2d *m);
+// independent copy of a cross-section (own solve state, no device context)
+cem_mode2d *mode2d_clone(const cem_mode2d *m);
+// solved mode of src (beta, frequency, E, H) into dst
+void mode2d_copy_state(cem_mode2d *dst, const cem_mode2d *src);
+// true when cem_mode2d_solve runs on the GPU
+bool mode2d_on_gpu();
 void nnls(int m, int n, const double *A, const double *b, double *x);
 
 // device engine (engine.hip)
