@@ -12,6 +12,8 @@
           meshed with 3 cells, SIBC sigma 5.8e7 with Huray roughness, 6-pole
           Djordjevic-Sarkar substrate (4.1, 0.02 at 1 GHz), 25 um solder mask,
           2D eigenmode ports, 401 frequencies 0.1-40 GHz
+* board4           - large single board: four copper layers, three Debye dielectrics,
+                     seeded meanders and vias on 600x400x160 non-uniform cells
 """
 import math
 
@@ -261,3 +263,158 @@ def gcpw_sparams(s, info):
 def cells(info):
     nx, ny, nz = info["cells"]
     return nx * ny * nz
+
+
+# ---------------------------------------------------------------------------
+# Large single board: 4-layer stackup on 600 x 400 x 160 cells (38.8 M grid points)
+BOARD4 = dict(
+    nx=600, ny=400, dx=0.1e-3, dy=0.1e-3, npml=8,
+    layers=(0.2e-3, 0.3e-3, 0.2e-3),    # dielectric thicknesses between copper layers 1-2, 2-3, 3-4
+    zair=3.0e-3, hd=25e-6, nz=160,
+    meanders=5, width=(0.1e-3, 0.4e-3), length=(10e-3, 40e-3), seed_layer=3000,
+    vias=40, via_r=0.15e-3, seed_via=3100,
+    ds=(4.1, 0.02, 1e9), npole=6, sigma_cu=5.8e7, rough=2, rough_p1=0.5e-6,
+    rough_p2=2.0 / (4 * math.pi * 0.25e-12), fmax=40e9, nf=801, cfl=0.99,
+)
+
+
+def _z_stack(p):
+    """z lines: uniform cells of hd through the dielectric stack (copper planes at the layer
+    interfaces), air on both sides growing geometrically from hd to fill zair, the growth
+    ratio chosen so the domain has exactly p["nz"] cells with npml CPML cells per side at
+    the outermost air spacing."""
+    hd, npml = p["hd"], p["npml"]
+    zc = [0.0]
+    zd = [0.0]
+    for t in p["layers"]:
+        m = max(1, int(round(t / hd)))
+        zd += [zd[-1] + t * (i + 1) / m for i in range(m)]
+        zc.append(zd[-1])
+    nair = p["nz"] - 2 * npml - (len(zd) - 1)
+    if nair < 4:
+        raise ValueError("board4: not enough cells for the air layers")
+
+    def side(n):
+        lo, hi = 1.0, 2.0   # geometric growth ratio with hd * (r^n - 1) / (r - 1) = zair
+        if hd * n >= p["zair"]:
+            return [hd * p["zair"] / (hd * n)] * n
+        for _ in range(200):
+            r = 0.5 * (lo + hi)
+            if hd * (r ** n - 1) / (r - 1) > p["zair"]:
+                hi = r
+            else:
+                lo = r
+        r = 0.5 * (lo + hi)
+        h = [hd * r ** i for i in range(n)]
+        f = p["zair"] / sum(h)
+        return [v * f for v in h]
+
+    lo_h, hi_h = side(nair // 2), side(nair - nair // 2)
+    Z = [0.0]
+    for h in lo_h:
+        Z.insert(0, Z[0] - h)
+    Z += zd[1:]
+    for h in hi_h:
+        Z.append(Z[-1] + h)
+    Z = [Z[0] - lo_h[-1] * i for i in range(npml, 0, -1)] + Z + [Z[-1] + hi_h[-1] * i for i in range(1, npml + 1)]
+    return np.array(Z), zc
+
+
+def _meander(rng, xlo, xhi, ylo, yhi, w, length):
+    """Axis-aligned serpentine of total centre-line length `length`, width w: a list of
+    rectangles (x0, x1, y0, y1)."""
+    amp = rng.uniform(1.0e-3, 3.0e-3)
+    pitch = 4 * w
+    x = rng.uniform(xlo + 1e-3, xhi - amp - 1e-3)
+    y = rng.uniform(ylo + 1e-3, yhi - 1e-3)
+    rects, left, dirn, pts = [], length, 1, [(x, y)]
+    horiz = True
+    while left > 0:
+        if horiz:
+            run = min(amp, left)
+            nxp = x + dirn * run
+            rects.append((min(x, nxp) - w / 2, max(x, nxp) + w / 2, y - w / 2, y + w / 2))
+            x = nxp
+            dirn = -dirn
+        else:
+            run = min(pitch, left)
+            if y + run > yhi - 1e-3:
+                run = -run
+            nyp = y + run
+            rects.append((x - w / 2, x + w / 2, min(y, nyp) - w / 2, max(y, nyp) + w / 2))
+            y = nyp
+        left -= abs(run)
+        horiz = not horiz
+        pts.append((x, y))
+    return rects, pts
+
+
+def board4(cem, **over):
+    """Large single board: copper layers 1-4 at the dielectric interfaces, each with seeded
+    meander traces (PEC edges on the layer plane), PEC via cylinders through the stack,
+    Djordjevic-Sarkar Debye dielectrics, SIBC on every conductor, two lumped ports on the
+    first meander of layer 1, CPML on all faces."""
+    p = dict(BOARD4)
+    p.update(over)
+    npml = p["npml"]
+    nxi, nyi = p["nx"] - 2 * npml, p["ny"] - 2 * npml
+    X = np.array([(i - npml) * p["dx"] for i in range(p["nx"] + 1)])
+    Y = np.array([(j - npml) * p["dy"] for j in range(p["ny"] + 1)])
+    Z, zc = _z_stack(p)
+    nx, ny, nz = len(X) - 1, len(Y) - 1, len(Z) - 1
+    s = cem.create_sim(X, Y, Z)
+    for f in range(6):
+        s.set_boundary(f, L.CPML, npml)
+    xlo, xhi, ylo, yhi = X[0] - 1, X[-1] + 1, Y[0] - 1, Y[-1] + 1
+    bx0, bx1, by0, by1 = 0.0, nxi * p["dx"], 0.0, nyi * p["dy"]
+    fmax = p["fmax"]
+    freqs = np.linspace(fmax / p["nf"], fmax, p["nf"])
+    dk, tand, fref = p["ds"]
+    m = cem.fit_djordjevic_sarkar(dk, tand, fref, freqs[0], fmax, p["npole"])
+    s.set_debye_poles(m["fp"])
+    s.material_box_debye(bx0, bx1, by0, by1, zc[0], zc[-1], m["eps_inf"], 0.0, m["deps"])
+    # meanders on each copper plane
+    XX, YY = np.meshgrid(X, Y, indexing="ij")      # nodes (nx+1, ny+1)
+    ks = [nearest(Z, z) for z in zc]
+    first = None
+    for lay, k in enumerate(ks):
+        rng = np.random.default_rng(p["seed_layer"] + lay)
+        inside = np.zeros((nx + 1, ny + 1), dtype=bool)
+        for t in range(p["meanders"]):
+            w = rng.uniform(*p["width"])
+            rects, pts = _meander(rng, bx0, bx1, by0, by1, w, rng.uniform(*p["length"]))
+            for (x0, x1, y0, y1) in rects:
+                inside |= (XX >= x0 - 1e-12) & (XX <= x1 + 1e-12) & (YY >= y0 - 1e-12) & (YY <= y1 + 1e-12)
+            if lay == 0 and t == 0:
+                first = pts
+        mu = inside[:-1, :] & inside[1:, :]      # x edges (nx, ny+1)
+        mv = inside[:, :-1] & inside[:, 1:]      # y edges (nx+1, ny)
+        s.pec_edges_plane(L.Z, k, mu.astype(np.uint8), mv.astype(np.uint8))
+    # vias: PEC cells whose centres lie within the cylinder, through the dielectric stack
+    rng = np.random.default_rng(p["seed_via"])
+    xc = 0.5 * (X[:-1] + X[1:])
+    yc = 0.5 * (Y[:-1] + Y[1:])
+    k0, k1 = ks[0], ks[-1]
+    r = p["via_r"]
+    for _ in range(p["vias"]):
+        vx, vy = rng.uniform(bx0 + 1e-3, bx1 - 1e-3), rng.uniform(by0 + 1e-3, by1 - 1e-3)
+        i0, i1 = int(np.searchsorted(xc, vx - r)), int(np.searchsorted(xc, vx + r))
+        j0, j1 = int(np.searchsorted(yc, vy - r)), int(np.searchsorted(yc, vy + r))
+        if i1 <= i0 or j1 <= j0:
+            continue
+        cx, cy = np.meshgrid(xc[i0:i1], yc[j0:j1], indexing="ij")
+        disk = (cx - vx) ** 2 + (cy - vy) ** 2 <= r * r
+        mask = np.repeat(disk[:, :, None], k1 - k0, axis=2).astype(np.uint8)
+        s.pec_cells(i0, j0, k0, mask)
+    s.set_conductor_loss(p["sigma_cu"], p["rough"], p["rough_p1"], p["rough_p2"], freqs[0], fmax)
+    # lumped ports from layer 1 down to layer 2 at the two ends of the first meander
+    (xa, ya), (xb, yb) = first[0], first[-1]
+    ia, ja, ib, jb = nearest(X, xa), nearest(Y, ya), nearest(X, xb), nearest(Y, yb)
+    p1 = s.add_lumped_port(L.Z, ia, ja, ks[0], ks[1], -1, 50.0)
+    p2 = s.add_lumped_port(L.Z, ib, jb, ks[0], ks[1], -1, 50.0)
+    tau = math.sqrt(6.9) / (math.pi * 1.25 * fmax)
+    s.set_gaussian(p1, 1.0, 4.5 * tau, tau)
+    s.set_frequencies(freqs)
+    s.finalize(p["cfl"])
+    info = dict(cells=(nx, ny, nz), ports=(p1, p2), freqs=freqs, params=p, layers=ks, tau=tau)
+    return s, info
