@@ -451,3 +451,22 @@ def test_mode2d_gpu_matches_host_lossy_port(cem, gpu_available, monkeypatch):
         print("beta host %.12g gpu %.12g alpha %.6g / %.6g" % (h[1], g[1], h[2], g[2]))
     finally:
         cem.lib.cem_mode2d_destroy(m)
+
+
+def test_four_layer_board_matches_reference(cem, ref, gpu_available):
+    """The four-layer board generator at a reduced size (meanders on every copper layer, vias,
+    6-pole Debye dielectrics, Huray SIBC, lumped ports): fields after 800 steps within 1e-4 of the
+    largest field value, port spectra within 1e-4."""
+    from test_plan import SMALL_BOARD4
+    print_case("four-layer board, reduced, 800 steps")
+    a, _ = S.board4(ref, **SMALL_BOARD4)
+    b, _ = S.board4(cem, **SMALL_BOARD4)
+    a.step(800)
+    b.step(800)
+    err = max(np.max(np.abs(x - y)) for x, y in zip(fields(a), fields(b)))
+    scale = max(np.max(np.abs(x)) for x in fields(a))
+    print("800 steps: max |dF| / max |F| = %.2e" % (err / scale))
+    assert err <= FIELD_TOL * scale
+    for p in (0, 1):
+        assert rel(b.port_v(p), a.port_v(p)) < SPEC_TOL
+        assert rel(b.port_i(p), a.port_i(p)) < SPEC_TOL

@@ -19,11 +19,16 @@ SMALL_LUMPED = dict(hmin=40e-6, hmax=100e-6, dx=200e-6, length=2e-3, yair=0.3e-3
 SMALL_MODE = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=1e-3, feed=1.2e-3, mon=0.6e-3, yair=0.3e-3,
                   zair=0.4e-3, ports="mode", lossy=True, tcu=35e-6, mask=25e-6, dc=False, df=2e9)
 
+# the four-layer board generator at a reduced size (meanders, vias, Debye, SIBC, lumped ports)
+SMALL_BOARD4 = dict(nx=80, ny=60, nz=56, zair=0.4e-3, meanders=2, vias=4, nf=11, length=(2e-3, 4e-3),
+                    width=(0.15e-3, 0.3e-3))
+
 SCENES = {
     "parity": lambda c: S.parity_scene(c)[0],
     "checkpoint": S.checkpoint_scene,
     "gcpw_lumped": lambda c: S.gcpw(c, **SMALL_LUMPED)[0],
     "gcpw_mode_lossy": lambda c: S.gcpw(c, **SMALL_MODE)[0],
+    "board4_small": lambda c: S.board4(c, **SMALL_BOARD4)[0],
 }
 
 
