@@ -10,7 +10,11 @@ The timed steps run as the engine runs them (chunks replayed from a captured
 HIP graph); per-kernel durations come from a second, event-bracketed pass of
 direct launches of the same kernels.
 
-Prints one JSON line; `value` is Mcell-updates/s = nx*ny*nz*steps / wall.
+Prints one JSON line; `value` is Mcell-updates/s = nx*ny*nz*steps / wall.  The
+same line carries `batched` (seeded 2-port boards, both excitations, one at a
+time versus concurrent streams) and `served` (a Poisson stream of seeded boards
+through the admission loop: solves/s, p50/p99 time to S-parameters, against the
+same boards run one at a time); `--no-batch` leaves them out.
 """
 import argparse
 import csv
@@ -128,6 +132,50 @@ def pmc_traffic(workload):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def batched_and_served(cem, nboards, workers, nreq, rate, budget_gb):
+    """Batched: the first `nboards` of the 64 seeded GCPW / microstrip boards (seed 4000 + b),
+    both excitations each, one at a time and then on `workers` concurrent streams; the two
+    runs must give bit-identical S matrices.  Served: `nreq` seeded request boards (seed
+    5000 + r) arriving as a Poisson stream at `rate` boards/s, admitted by estimated
+    footprint under `budget_gb`, against the same boards run one at a time."""
+    from courant_amd import serve
+    boards = [serve.random_board(4000 + b) for b in range(nboards)]
+    serve.run_batch(cem, [dict(boards[0], nx=80, ny=80, nz=60, steps=64)], 1)   # context warm-up
+    s1, w1, u1 = serve.run_batch(cem, boards, 1)
+    print(f"batched: one at a time {w1:.2f} s", file=sys.stderr, flush=True)
+    sb, wb, ub = serve.run_batch(cem, boards, workers)
+    print(f"batched: {workers} workers {wb:.2f} s", file=sys.stderr, flush=True)
+    nsol = len(serve.jobs_of(boards))
+    batched = {
+        "workload": "seeded GCPW/microstrip boards 80..256 x 80..192 x 60..96, Debye 2-6 poles + SIBC, "
+                    "both excitations per board (seed 4000+b)",
+        "boards": nboards, "solves": nsol, "workers": workers,
+        "solves_per_s": round(nsol / wb, 3), "one_at_a_time_solves_per_s": round(nsol / w1, 3),
+        "speedup": round(w1 / wb, 3), "mcell_updates_per_s": round(ub / wb / 1e6, 1),
+        "one_at_a_time_mcell_updates_per_s": round(u1 / w1 / 1e6, 1),
+        "bitwise_same_as_one_at_a_time": all(np.array_equal(a, b) for a, b in zip(s1, sb)),
+    }
+    reqs = [serve.random_request(5000 + r) for r in range(nreq)]
+    nsol = len(serve.jobs_of(reqs))
+    t0 = time.perf_counter()
+    seq = [serve.run_batch(cem, [b], 1)[0][0] for b in reqs]
+    wseq = time.perf_counter() - t0
+    print(f"served: one at a time {wseq:.2f} s", file=sys.stderr, flush=True)
+    lat, wall, got, upd = serve.serve(cem, reqs, rate, workers, budget_gb * 1e9)
+    print(f"served: {wall:.2f} s", file=sys.stderr, flush=True)
+    served = {
+        "workload": "Poisson arrivals of seeded boards, cells log-normal (median 3 M, sigma 0.8) in "
+                    "[0.5 M, 20 M], half lossy, half 2-port (seed 5000+r)",
+        "rate_per_s": rate, "boards": nreq, "solves": nsol, "workers": workers, "budget_gb": budget_gb,
+        "mean_mcells": round(float(np.mean([serve.cells(b) for b in reqs])) / 1e6, 2),
+        "solves_per_s": round(nsol / wall, 3), "one_at_a_time_solves_per_s": round(nsol / wseq, 3),
+        "speedup": round(wseq / wall, 3), "mcell_updates_per_s": round(upd / wall / 1e6, 1),
+        "p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
+        "bitwise_same_as_one_at_a_time": all(np.array_equal(a, b) for a, b in zip(seq, got)),
+    }
+    return batched, served
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +190,12 @@ def main():
                          "board4: four-layer board, 600x400x160 (38.8 M grid points)")
     ap.add_argument("--copy-only", type=float, default=0,
                     help="only run the stream-copy kernel on this many GB (byte-counter calibration)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--batch-boards", type=int, default=8, help="boards of the batched set (of 64)")
+    ap.add_argument("--serve-requests", type=int, default=12, help="boards of the served stream")
+    ap.add_argument("--serve-rate", type=float, default=5.0, help="Poisson arrival rate, boards/s")
+    ap.add_argument("--workers", type=int, default=4, help="concurrent solves (host threads, one stream each)")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="device footprint admitted at once")
     args = ap.parse_args()
     if args.copy_only:
         from courant_amd import lib
@@ -234,6 +288,9 @@ def main():
             out["roofline"]["traffic"] = round(tr["kernels"].get(kname, 0.0))
             out["roofline"]["traffic_by_kernel"] = {k: round(v) for k, v in tr["kernels"].items()}
             out["roofline"]["pmc_scale"] = {"FETCH_SIZE": tr["fetch_scale"], "WRITE_SIZE": tr["write_scale"]}
+    if not args.no_batch:
+        out["batched"], out["served"] = batched_and_served(cem, args.batch_boards, args.workers,
+                                                           args.serve_requests, args.serve_rate, args.budget_gb)
     if not args.no_cpu and args.workload == "cfg2":
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "courant_amd.h"
+#include "devmem.hpp"
 #include "sim.hpp"
 
 using namespace cem;
@@ -742,10 +743,12 @@ template <class T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
-    int alloc(size_t count) {
+    hipStream_t owner = nullptr;   // stream the block was allocated on and is freed on
+    int alloc(size_t count, hipStream_t st) {
         n = count;
+        owner = st;
         if (!count) return 0;
-        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+        if (dev_alloc((void **)&p, count * sizeof(T), st) != hipSuccess) {
             p = nullptr;
             set_error("device out of memory");
             return -1;
@@ -766,7 +769,7 @@ struct DBuf {
         return hipMemsetAsync(p, 0, n * sizeof(T), st) == hipSuccess ? 0 : -1;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        dev_free(p, owner);
         p = nullptr;
         n = 0;
     }
@@ -941,15 +944,15 @@ int upload_sources(cem_sim *s, Device *D) {
     D->x_src.release();
     D->d_src.release();
     D->sb_src.release();
-    if (D->wl.alloc(wl.size()) || D->wm.alloc(wm.size()) || D->x_idx.alloc(x_idx.size()) ||
-        D->x_src.alloc(x_src.size()))
+    if (D->wl.alloc(wl.size(), D->st) || D->wm.alloc(wm.size(), D->st) || D->x_idx.alloc(x_idx.size(), D->st) ||
+        D->x_src.alloc(x_src.size(), D->st))
         return -1;
     if (D->wl.upload(wl.data(), wl.size(), D->st) || D->wm.upload(wm.data(), wm.size(), D->st) ||
         D->x_idx.upload(x_idx.data(), x_idx.size(), D->st) || D->x_src.upload(x_src.data(), x_src.size(), D->st))
         return -1;
-    if (D->has_dsrc && (D->d_src.alloc(d_src.size()) || D->d_src.upload(d_src.data(), d_src.size(), D->st)))
+    if (D->has_dsrc && (D->d_src.alloc(d_src.size(), D->st) || D->d_src.upload(d_src.data(), d_src.size(), D->st)))
         return -1;
-    if (D->has_sbsrc && (D->sb_src.alloc(sb_src.size()) || D->sb_src.upload(sb_src.data(), sb_src.size(), D->st)))
+    if (D->has_sbsrc && (D->sb_src.alloc(sb_src.size(), D->st) || D->sb_src.upload(sb_src.data(), sb_src.size(), D->st)))
         return -1;
     D->sources_dirty = false;
     return 0;
@@ -970,7 +973,7 @@ int upload_plan(cem_sim *s, Device *D) {
     hipStream_t st = D->st;
 
     // fields (zero) and coefficients
-    if (D->F.alloc(6 * N) || D->F.zero(st)) return -1;
+    if (D->F.alloc(6 * N, D->st) || D->F.zero(st)) return -1;
     // Coefficient compression: the (ca, cb) pairs of each component take few
     // distinct values (one per material mix), so each point stores three 8-bit
     // indices into a 256-entry table; more than 256 pairs falls back to arrays.
@@ -1007,11 +1010,11 @@ int upload_plan(cem_sim *s, Device *D) {
         }
         D->compressed = ok;
         if (ok) {
-            if (D->cidx.alloc(N) || D->cidx.upload(idx.data(), N, st)) return -1;
-            if (D->ctab.alloc(3 * 256) || D->ctab.upload(tab.data(), 3 * 256, st)) return -1;
+            if (D->cidx.alloc(N, D->st) || D->cidx.upload(idx.data(), N, st)) return -1;
+            if (D->ctab.alloc(3 * 256, D->st) || D->ctab.upload(tab.data(), 3 * 256, st)) return -1;
         }
     }
-    if (!D->compressed && (D->ca.alloc(3 * N) || D->cb.alloc(3 * N))) return -1;
+    if (!D->compressed && (D->ca.alloc(3 * N, D->st) || D->cb.alloc(3 * N, D->st))) return -1;
     if (!D->compressed) {
         std::vector<float> tmp(N);
         for (int c = 0; c < 3; c++) {
@@ -1041,7 +1044,7 @@ int upload_plan(cem_sim *s, Device *D) {
         off_ce[a] = ax.size();
         for (int i = 0; i <= n; i++) ax.push_back((float)(s->dd[a][i] > 0 ? s->ce[a][i] * (1.0 / s->dd[a][i]) : 0.0));
     }
-    if (D->axis1d.alloc(ax.size()) || D->axis1d.upload(ax.data(), ax.size(), st)) return -1;
+    if (D->axis1d.alloc(ax.size(), D->st) || D->axis1d.upload(ax.data(), ax.size(), st)) return -1;
     for (int a = 0; a < 3; a++) {
         D->hs[a] = D->axis1d.p + off_hs[a];
         D->es[a] = D->axis1d.p + off_es[a];
@@ -1071,7 +1074,7 @@ int upload_plan(cem_sim *s, Device *D) {
         }
         D->pmlH.per[a] = D->pmlE.per[a] = (long long)pml_per(s, a);
     }
-    if (D->slots.alloc(slots.size()) || D->slots.upload(slots.data(), slots.size(), st)) return -1;
+    if (D->slots.alloc(slots.size(), D->st) || D->slots.upload(slots.data(), slots.size(), st)) return -1;
     for (int a = 0; a < 3; a++) {
         D->pmlH.slot[a] = D->slots.p + off_sh[a];
         D->pmlE.slot[a] = D->slots.p + off_se[a];
@@ -1083,7 +1086,7 @@ int upload_plan(cem_sim *s, Device *D) {
             D->psiH_len[c][w] = psi_len(s, 0, c, w);
             psi_total += D->psiE_len[c][w] + D->psiH_len[c][w];
         }
-    if (D->psi.alloc(psi_total + 1) || D->psi.zero(st)) return -1;
+    if (D->psi.alloc(psi_total + 1, D->st) || D->psi.zero(st)) return -1;
     {
         size_t o = 0;
         for (int c = 0; c < 3; c++)
@@ -1120,12 +1123,12 @@ int upload_plan(cem_sim *s, Device *D) {
             mat[e] = it->second;
         }
         D->dstride = 1 + np;
-        if (D->d_idx.alloc(s->ndisp) || D->d_idx.upload(idx.data(), s->ndisp, st)) return -1;
-        if (D->d_mat.alloc(s->ndisp) || D->d_mat.upload(mat.data(), s->ndisp, st)) return -1;
-        if (D->dtab.alloc(tab.size() + 1) || D->dtab.upload(tab.data(), tab.size(), st)) return -1;
-        if (D->Eold.alloc(s->ndisp) || D->Eold.zero(st)) return -1;
+        if (D->d_idx.alloc(s->ndisp, D->st) || D->d_idx.upload(idx.data(), s->ndisp, st)) return -1;
+        if (D->d_mat.alloc(s->ndisp, D->st) || D->d_mat.upload(mat.data(), s->ndisp, st)) return -1;
+        if (D->dtab.alloc(tab.size() + 1, D->st) || D->dtab.upload(tab.data(), tab.size(), st)) return -1;
+        if (D->Eold.alloc(s->ndisp, D->st) || D->Eold.zero(st)) return -1;
         const size_t nj = (size_t)np * s->ndisp;
-        if (D->J.alloc(nj) || D->J.zero(st)) return -1;
+        if (D->J.alloc(nj, D->st) || D->J.zero(st)) return -1;
         PostParams pp{};
         pp.ndisp = s->ndisp;
         pp.npole = s->npole;
@@ -1141,8 +1144,8 @@ int upload_plan(cem_sim *s, Device *D) {
             pp.zc[t] = (float)s->sibc_c[t];
             pp.om[t] = (float)(-std::expm1(-s->sibc_a[t] * s->dt));
         }
-        if (D->pp.alloc(1) || D->pp.upload(&pp, 1, st)) return -1;
-        if (D->ctl.alloc(1) || D->ctl.zero(st)) return -1;
+        if (D->pp.alloc(1, D->st) || D->pp.upload(&pp, 1, st)) return -1;
+        if (D->ctl.alloc(1, D->st) || D->ctl.zero(st)) return -1;
         D->ctl_step = 0;
         D->ctl_row = 0;
     }
@@ -1162,11 +1165,11 @@ int upload_plan(cem_sim *s, Device *D) {
             }
         }
         D->nsbo = (int)sbo.size();
-        if (D->sb_idx.alloc(s->nsb) || D->sb_idx.upload(idx.data(), s->nsb, st)) return -1;
-        if (D->sbo.alloc(sbo.size()) || D->sbo.upload(sbo.data(), sbo.size(), st)) return -1;
-        if (D->has_dsb && (D->d_sb.alloc(s->ndisp) || D->d_sb.upload(d_sb.data(), s->ndisp, st))) return -1;
+        if (D->sb_idx.alloc(s->nsb, D->st) || D->sb_idx.upload(idx.data(), s->nsb, st)) return -1;
+        if (D->sbo.alloc(sbo.size(), D->st) || D->sbo.upload(sbo.data(), sbo.size(), st)) return -1;
+        if (D->has_dsb && (D->d_sb.alloc(s->ndisp, D->st) || D->d_sb.upload(d_sb.data(), s->ndisp, st))) return -1;
         const size_t ny_ = (size_t)s->nsb * s->sibc_npole;
-        if (D->Y.alloc(ny_) || D->Y.zero(st) || D->face.alloc(s->nsb)) return -1;
+        if (D->Y.alloc(ny_, D->st) || D->Y.zero(st) || D->face.alloc(s->nsb, D->st)) return -1;
         if (D->face.upload(s->sb_mask.data(), s->nsb, st)) return -1;
     }
     if (upload_sources(s, D)) return -1;
@@ -1202,19 +1205,19 @@ int upload_plan(cem_sim *s, Device *D) {
         }
         toff.push_back((int)tidx.size());
         D->nsamp = (int)post.size();
-        if (D->tidx.alloc(tidx.size() + 1) || D->tw.alloc(tw.size() + 1) || D->toff.alloc(toff.size()) ||
-            D->post.alloc(post.size() + 1))
+        if (D->tidx.alloc(tidx.size() + 1, D->st) || D->tw.alloc(tw.size() + 1, D->st) || D->toff.alloc(toff.size(), D->st) ||
+            D->post.alloc(post.size() + 1, D->st))
             return -1;
         if (D->tidx.upload(tidx.data(), tidx.size(), st) || D->tw.upload(tw.data(), tw.size(), st) ||
             D->toff.upload(toff.data(), toff.size(), st) || D->post.upload(post.data(), post.size(), st))
             return -1;
-        if (D->vlast.alloc(s->port.size() + 1) || D->vlast.zero(st)) return -1;
+        if (D->vlast.alloc(s->port.size() + 1, D->st) || D->vlast.zero(st)) return -1;
     }
     // series and DFT state
     {
         const size_t nip = s->ip.size(), np = s->port.size(), nvp = s->vp.size();
         const size_t per = nip + 2 * np + nvp;
-        if (D->ser.alloc(per * kSeriesCap + 1)) return -1;
+        if (D->ser.alloc(per * kSeriesCap + 1, D->st)) return -1;
         double *b = D->ser.p;
         D->ser_ip = b;
         b += nip * kSeriesCap;
@@ -1228,13 +1231,13 @@ int upload_plan(cem_sim *s, Device *D) {
         std::vector<double> om(nf), R(np + 1, 1.0);
         for (int f = 0; f < nf; f++) om[f] = 2 * M_PI * s->freq[f];
         for (size_t p = 0; p < np; p++) R[p] = s->port[p].R;
-        if (D->omega.alloc(nf) || D->omega.upload(om.data(), nf, st)) return -1;
-        if (D->rot.alloc(2 * nf) || D->rot.upload(s->rotE.data(), 2 * nf, st)) return -1;
+        if (D->omega.alloc(nf, D->st) || D->omega.upload(om.data(), nf, st)) return -1;
+        if (D->rot.alloc(2 * nf, D->st) || D->rot.upload(s->rotE.data(), 2 * nf, st)) return -1;
         for (int b2 = 0; b2 < 2; b2++)
-            if (D->ph[b2].alloc(4 * nf) || D->ph[b2].zero(st)) return -1;
+            if (D->ph[b2].alloc(4 * nf, D->st) || D->ph[b2].zero(st)) return -1;
         D->ph_cur = 0;
-        if (D->acc.alloc((size_t)D->nacc * 2 * nf) || D->acc.zero(st)) return -1;
-        if (D->R.alloc(np + 1) || D->R.upload(R.data(), np + 1, st)) return -1;
+        if (D->acc.alloc((size_t)D->nacc * 2 * nf, D->st) || D->acc.zero(st)) return -1;
+        if (D->R.alloc(np + 1, D->st) || D->R.upload(R.data(), np + 1, st)) return -1;
         D->rows = 0;
         D->row0 = 0;
     }
@@ -1243,9 +1246,9 @@ int upload_plan(cem_sim *s, Device *D) {
         std::vector<double> d1;
         for (int a = 0; a < 3; a++) d1.insert(d1.end(), s->d[a].begin(), s->d[a].end());
         for (int a = 0; a < 3; a++) d1.insert(d1.end(), s->dd[a].begin(), s->dd[a].end());
-        if (D->d1d.alloc(d1.size()) || D->d1d.upload(d1.data(), d1.size(), st)) return -1;
+        if (D->d1d.alloc(d1.size(), D->st) || D->d1d.upload(d1.data(), d1.size(), st)) return -1;
         D->energy_blocks = 1024;
-        if (D->partial.alloc(D->energy_blocks + 1)) return -1;
+        if (D->partial.alloc(D->energy_blocks + 1, D->st)) return -1;
     }
     HIP_OK(hipStreamSynchronize(st));
     return 0;

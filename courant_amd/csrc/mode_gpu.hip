@@ -28,6 +28,7 @@
 #include <cstring>
 #include <vector>
 
+#include "devmem.hpp"
 #include "mode_gpu.hpp"
 #include "sim.hpp"
 
@@ -656,12 +657,12 @@ template <typename T>
 struct DMem {
     T *p = nullptr;
     size_t n = 0;
-    bool get(size_t count) {
+    hipStream_t owner = nullptr;   // stream the block was allocated on and is freed on
+    bool get(size_t count, hipStream_t st) {
         if (count <= n) return true;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+        release();
+        owner = st;
+        if (dev_alloc((void **)&p, count * sizeof(T), st) != hipSuccess) {
             p = nullptr;
             return false;
         }
@@ -669,7 +670,7 @@ struct DMem {
         return true;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        dev_free(p, owner);
         p = nullptr;
         n = 0;
     }
@@ -698,13 +699,12 @@ struct ModeGpu {
 };
 
 bool mode_gpu_enabled() {
-    static int have_gpu = -1;
-    if (have_gpu < 0) {
+    static const bool have_gpu = [] {
         int ndev = 0;
-        have_gpu = hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0 ? 1 : 0;
-    }
+        return hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0;
+    }();
     const char *v = std::getenv("COURANT_MODE_HOST");
-    return have_gpu == 1 && !(v && std::atoi(v) != 0);
+    return have_gpu && !(v && std::atoi(v) != 0);
 }
 
 void mode_gpu_release(ModeGpu *m) {
@@ -767,14 +767,14 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     }
     const int ncoef = p.ny + (p.ny + 1) + p.nz + (p.nz + 1) + g.nu + g.nv + g.nn;
     const int nbatch = std::min(kProbeBatch, g.wc);
-    if (!M.band.get((size_t)g.ldab * n) || !M.coef.get(ncoef) || !M.x.get(n) || !M.xo.get(n) || !M.y.get(n) ||
-        !M.hw.get((size_t)nbatch * g.nc) || !M.hu.get((size_t)nbatch * g.nv) || !M.hv.get((size_t)nbatch * g.nu) ||
-        !M.ew.get((size_t)nbatch * g.nn) || !M.pec.get(g.nu + g.nv) || !M.ipiv.get(n) || !M.info.get(1) ||
-        !M.out.get(10)) {
+    hipStream_t st = M.st;
+    if (!M.band.get((size_t)g.ldab * n, st) || !M.coef.get(ncoef, st) || !M.x.get(n, st) || !M.xo.get(n, st) ||
+        !M.y.get(n, st) || !M.hw.get((size_t)nbatch * g.nc, st) || !M.hu.get((size_t)nbatch * g.nv, st) ||
+        !M.hv.get((size_t)nbatch * g.nu, st) || !M.ew.get((size_t)nbatch * g.nn, st) || !M.pec.get(g.nu + g.nv, st) ||
+        !M.ipiv.get(n, st) || !M.info.get(1, st) || !M.out.get(10, st)) {
         set_error("GPU mode solver: device out of memory");
         return -1;
     }
-    hipStream_t st = M.st;
     const bool debug = std::getenv("COURANT_MODE_DEBUG") != nullptr;
     hipEvent_t ev[4] = {};
     if (debug)

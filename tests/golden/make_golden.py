@@ -6,6 +6,8 @@
   checkpoint_plan.npz  integer plan of the solver/tests/test_checkpoint.c scene: Debye
                        edge list, SIBC edge and face lists, port / probe lists, psi lengths
   gcpw_mini.npz        S11, S21 of a small lumped-port GCPW after 4000 steps (f64)
+  served_mini.npz      full 2 x 2 S matrix (both excitations) of a reduced served board:
+                       lossy GCPW, 3 Debye poles, SIBC copper, lumped ports, 2500 steps
 
 Run: python tests/golden/make_golden.py
 """
@@ -19,8 +21,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from courant_amd import lib, scenes  # noqa: E402
+from courant_amd import lib, scenes, serve  # noqa: E402
 
+SERVED_MINI = dict(serve.random_board(4000), nx=64, ny=48, nz=36, h_cells=6, npole=3, kind="gcpw",
+                   width=0.3e-3, er=4.1, nf=41, steps=2500)
 GCPW_MINI = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=3e-3, yair=0.4e-3, zair=0.6e-3, zbelow=0.2e-3,
                  pml_below=True, df=0.5e9)
 
@@ -56,6 +60,8 @@ def main():
     g.step(4000)
     s11, s21 = scenes.gcpw_sparams(g, gi)
     np.savez_compressed(os.path.join(HERE, "gcpw_mini.npz"), freqs=gi["freqs"], s11=s11, s21=s21, steps=4000)
+    S, _, _ = serve.run_batch(ref, [SERVED_MINI], 1)
+    np.savez_compressed(os.path.join(HERE, "served_mini.npz"), S=S[0], steps=SERVED_MINI["steps"])
     print("golden fixtures written")
 
 
