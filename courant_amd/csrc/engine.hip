@@ -736,6 +736,107 @@ __global__ void k_f64_to_f32(const double *src, float *dst, long long n) {
 }
 
 // ---------------------------------------------------------------------------
+// FP64 port post-processing on the device
+// ---------------------------------------------------------------------------
+struct zd {
+    double x, y;
+};
+__device__ __forceinline__ zd zadd(zd a, zd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ zd zsub(zd a, zd b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ zd zscale(zd a, double s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ zd zmul(zd a, zd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// Smith's quotient: no intermediate overflow for the magnitudes on this path
+__device__ __forceinline__ zd zdiv(zd a, zd b) {
+    if (fabs(b.x) >= fabs(b.y)) {
+        const double r = b.y / b.x, den = b.x + b.y * r;
+        return {(a.x + a.y * r) / den, (a.y - a.x * r) / den};
+    }
+    const double r = b.x / b.y, den = b.x * r + b.y;
+    return {(a.x * r + a.y) / den, (a.y * r - a.x) / den};
+}
+__device__ __forceinline__ zd zexp(zd z) {
+    double sn, cs;
+    sincos(z.y, &sn, &cs);
+    const double e = exp(z.x);
+    return {e * cs, e * sn};
+}
+__device__ __forceinline__ zd zcosh(zd z) {
+    double sn, cs;
+    sincos(z.y, &sn, &cs);
+    return {cosh(z.x) * cs, sinh(z.x) * sn};
+}
+// principal square root (branch cut on the negative real axis)
+__device__ __forceinline__ zd zsqrt(zd z) {
+    if (z.x == 0.0 && z.y == 0.0) return {0.0, z.y};
+    const double t = sqrt(0.5 * (fabs(z.x) + hypot(z.x, z.y)));
+    if (z.x >= 0.0) return {t, z.y / (2.0 * t)};
+    return {fabs(z.y) / (2.0 * t), copysign(t, z.y)};
+}
+
+struct ModeWaveArgs {
+    int nf, dir;
+    double dx, dist;               // monitor cell length; signed distance monitor -> reference plane
+    const double *V0, *V1, *Ix;    // DFT accumulators of the two monitor V probes and the I probe
+    const double *mode;            // per frequency: Re Z0, Im Z0, alpha, beta of the port mode
+    double *out;                   // a (2 nf) then b (2 nf)
+};
+
+// Wave separation of one mode port (modeport.c:203-227), one thread per
+// frequency: V at the monitor plane from the two staggered V probes with the
+// discrete-dispersion correction 1/cosh(gamma dx / 2), I from the loop probe,
+// a, b = (V +- Z0 I) / 2 sqrt(Z0), each moved to the reference plane by
+// exp(-+gamma d).
+__global__ void k_mode_waves(ModeWaveArgs a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.nf) return;
+    const zd Z = {a.mode[4 * q], a.mode[4 * q + 1]};
+    const zd g = {a.mode[4 * q + 2], a.mode[4 * q + 3]};
+    const zd c = zcosh({g.x * a.dx / 2.0, g.y * a.dx / 2.0});
+    const zd V = zdiv({a.V0[2 * q] + a.V1[2 * q], a.V0[2 * q + 1] + a.V1[2 * q + 1]}, zscale(c, 2.0));
+    const zd I = {(double)a.dir * a.Ix[2 * q], (double)a.dir * a.Ix[2 * q + 1]};
+    const zd sq = zscale(zsqrt(Z), 2.0);
+    const zd ZI = zmul(Z, I);
+    const zd A = zmul(zdiv(zadd(V, ZI), sq), zexp({-g.x * a.dist, -g.y * a.dist}));
+    const zd B = zmul(zdiv(zsub(V, ZI), sq), zexp({g.x * a.dist, g.y * a.dist}));
+    a.out[2 * q] = A.x;
+    a.out[2 * q + 1] = A.y;
+    a.out[2 * a.nf + 2 * q] = B.x;
+    a.out[2 * a.nf + 2 * q + 1] = B.y;
+}
+
+// 2x2 renormalisation (modeport.c:242-264) of nf frequencies in place, one
+// thread per frequency: Z = sqrt(Zd)(I + S)(I - S)^-1 sqrt(Zd),
+// S' = (Z - zref)(Z + zref)^-1 with Zd = diag(z1, z2).
+__global__ void k_s2_renormalize(double *S, int nf, const double *z1, const double *z2, double zref) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nf) return;
+    double *v = S + 8 * q;
+    const zd s00 = {v[0], v[1]}, s01 = {v[2], v[3]}, s10 = {v[4], v[5]}, s11 = {v[6], v[7]};
+    const zd one = {1.0, 0.0};
+    const zd a00 = zsub(one, s00), a01 = {-s01.x, -s01.y}, a10 = {-s10.x, -s10.y}, a11 = zsub(one, s11);
+    const zd det = zsub(zmul(a00, a11), zmul(a01, a10));
+    const zd i00 = zdiv(a11, det), i01 = zdiv({-a01.x, -a01.y}, det);
+    const zd i10 = zdiv({-a10.x, -a10.y}, det), i11 = zdiv(a00, det);
+    const zd b00 = zadd(one, s00), b01 = s01, b10 = s10, b11 = zadd(one, s11);
+    const zd n00 = zadd(zmul(b00, i00), zmul(b01, i10)), n01 = zadd(zmul(b00, i01), zmul(b01, i11));
+    const zd n10 = zadd(zmul(b10, i00), zmul(b11, i10)), n11 = zadd(zmul(b10, i01), zmul(b11, i11));
+    const double r1 = sqrt(z1[q]), r2 = sqrt(z2[q]);
+    const zd z00 = zscale(zscale(n00, r1), r1), z01 = zscale(zscale(n01, r1), r2);
+    const zd z10 = zscale(zscale(n10, r2), r1), z11 = zscale(zscale(n11, r2), r2);
+    const zd p00 = {z00.x + zref, z00.y}, p11 = {z11.x + zref, z11.y};
+    const zd m00 = {z00.x - zref, z00.y}, m11 = {z11.x - zref, z11.y};
+    const zd dp = zsub(zmul(p00, p11), zmul(z01, z10));
+    const zd q00 = zdiv(p11, dp), q01 = zdiv({-z01.x, -z01.y}, dp);
+    const zd q10 = zdiv({-z10.x, -z10.y}, dp), q11 = zdiv(p00, dp);
+    const zd r00 = zadd(zmul(m00, q00), zmul(z01, q10)), r01 = zadd(zmul(m00, q01), zmul(z01, q11));
+    const zd r10 = zadd(zmul(z10, q00), zmul(m11, q10)), r11 = zadd(zmul(z10, q01), zmul(m11, q11));
+    v[0] = r00.x; v[1] = r00.y;
+    v[2] = r01.x; v[3] = r01.y;
+    v[4] = r10.x; v[5] = r10.y;
+    v[6] = r11.x; v[7] = r11.y;
+}
+
+// ---------------------------------------------------------------------------
 // device state of one simulation
 // ---------------------------------------------------------------------------
 template <class T>
@@ -1837,6 +1938,43 @@ void cem::dev_spectra_download(cem_sim *s) {
     std::memcpy(s->spec_src.data(), blk(2 * np + nvp + nip), 2 * nf * sizeof(double));
 }
 
+// Wave separation of one mode port on the device from the DFT accumulators it
+// already holds; mode = (Re Z0, Im Z0, alpha, beta) per frequency.
+int cem::dev_mode_waves(cem_sim *s, int vp0, int vp1, int ip, const double *mode, double dx, double dist, int dir,
+                        double *a, double *b) {
+    if (dev_ensure(s)) return -1;
+    Device *D = s->dev;
+    launch_dft(s, D, 0);
+    const int nf = s->nf;
+    const size_t np = s->port.size(), nvp = s->vp.size();
+    const double *acc = D->acc.p;
+    ModeWaveArgs w{};
+    w.nf = nf;
+    w.dir = dir;
+    w.dx = dx;
+    w.dist = dist;
+    w.V0 = acc + (2 * np + vp0) * 2 * (size_t)nf;
+    w.V1 = acc + (2 * np + vp1) * 2 * (size_t)nf;
+    w.Ix = acc + (2 * np + nvp + ip) * 2 * (size_t)nf;
+    DBuf<double> md, out;
+    int rc = -1;
+    if (!md.alloc(4 * (size_t)nf, D->st) && !md.upload(mode, 4 * (size_t)nf, D->st) && !out.alloc(4 * (size_t)nf, D->st)) {
+        w.mode = md.p;
+        w.out = out.p;
+        hipLaunchKernelGGL(k_mode_waves, dim3((nf + 127) / 128), dim3(128), 0, D->st, w);
+        if (hipGetLastError() == hipSuccess &&
+            hipMemcpyAsync(a, out.p, 2 * nf * sizeof(double), hipMemcpyDeviceToHost, D->st) == hipSuccess &&
+            hipMemcpyAsync(b, out.p + 2 * nf, 2 * nf * sizeof(double), hipMemcpyDeviceToHost, D->st) == hipSuccess &&
+            hipStreamSynchronize(D->st) == hipSuccess)
+            rc = 0;
+        else
+            set_error("mode port: device wave separation failed");
+    }
+    md.release();
+    out.release();
+    return rc;
+}
+
 // host-recorded samples of an external engine (export.c:121-149): the reference
 // recomputes the phasors from cos/sin at every sample, so the DFT runs "direct"
 void cem::dev_push_samples(cem_sim *s, int n, const double *ip, const double *vold, const double *vnew,
@@ -1950,6 +2088,44 @@ extern "C" int cem_amd_step_device(cem_sim *s, long n) {
     if (!s || !s->finalized || dev_ensure(s)) return -1;
     dev_step(s, n);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// 2x2 renormalisation of nf frequencies on the device, FP64, in place.
+extern "C" int cem_amd_s2_renormalize(double *S, int nf, const double *z1, const double *z2, double zref) {
+    if (!S || !z1 || !z2 || nf < 0) {
+        set_error("s2 renormalize: bad arguments");
+        return -1;
+    }
+    if (nf == 0) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error("no GPU device available for the courant engine");
+        return -1;
+    }
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        set_error("cannot create HIP stream");
+        return -1;
+    }
+    DBuf<double> dS, d1, d2;
+    int rc = -1;
+    const size_t n = (size_t)nf;
+    if (!dS.alloc(8 * n, st) && !d1.alloc(n, st) && !d2.alloc(n, st) && !dS.upload(S, 8 * n, st) &&
+        !d1.upload(z1, n, st) && !d2.upload(z2, n, st)) {
+        hipLaunchKernelGGL(k_s2_renormalize, dim3((nf + 127) / 128), dim3(128), 0, st, dS.p, nf, d1.p, d2.p, zref);
+        if (hipGetLastError() == hipSuccess &&
+            hipMemcpyAsync(S, dS.p, 8 * n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+            hipStreamSynchronize(st) == hipSuccess)
+            rc = 0;
+        else
+            set_error("s2 renormalize: device failure");
+    }
+    dS.release();
+    d1.release();
+    d2.release();
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    return rc;
 }
 
 // Stream-copy bandwidth of this GPU: copies `bytes` (rounded down to 16 B) between

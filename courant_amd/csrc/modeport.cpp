@@ -9,6 +9,7 @@
 #include "sim.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -260,6 +261,13 @@ static int solve_all_ports(cem_sim *s) {
 }
 
 // Incident and outgoing waves at the reference plane (modeport.c:203-227)
+// COURANT_WAVES_HOST=1 runs the same FP64 formula on the host (the check path of
+// the device wave separation); by default it runs on the GPU.
+static bool waves_on_host() {
+    const char *v = std::getenv("COURANT_WAVES_HOST");
+    return v && std::atoi(v) != 0;
+}
+
 extern "C" int cem_mode_port_waves(cem_sim *s, int port, double *a, double *b, double *z0, double *beta) {
     if (port < 0 || port >= (int)s->mport.size() || !s->finalized) {
         set_error("bad mode port");
@@ -271,12 +279,28 @@ extern "C" int cem_mode_port_waves(cem_sim *s, int port, double *a, double *b, d
         if (!p.solved) return -1;
         (void)rc;
     }
-    const double *V0 = cem_vprobe(s, p.vp0), *V1 = cem_vprobe(s, p.vp1), *Ix = cem_iprobe(s, p.ip);
-    if (!V0 || !V1 || !Ix) return -1;
     const cem_mode_port_def &d = p.def;
     const double dx = s->d[0][d.i_mon];
     const double xmon = 0.5 * (s->node[0][d.i_mon] + s->node[0][d.i_mon + 1]);
     const double dist = d.dir * (d.x_ref - xmon);
+    if (!waves_on_host()) {
+        // FP64 on the GPU, next to the DFT accumulators; only a and b come back
+        std::vector<double> mode(4 * (size_t)s->nf);
+        for (int q = 0; q < s->nf; q++) {
+            mode[4 * q] = p.z0[2 * q];
+            mode[4 * q + 1] = p.z0[2 * q + 1];
+            mode[4 * q + 2] = p.alpha[q];
+            mode[4 * q + 3] = p.beta[q];
+        }
+        if (dev_mode_waves(s, p.vp0, p.vp1, p.ip, mode.data(), dx, dist, d.dir, a, b)) return -1;
+        for (int q = 0; q < s->nf; q++) {
+            if (z0) z0[q] = p.z0[2 * q];
+            if (beta) beta[q] = p.beta[q];
+        }
+        return 0;
+    }
+    const double *V0 = cem_vprobe(s, p.vp0), *V1 = cem_vprobe(s, p.vp1), *Ix = cem_iprobe(s, p.ip);
+    if (!V0 || !V1 || !Ix) return -1;
     for (int q = 0; q < s->nf; q++) {
         cplx Z(p.z0[2 * q], p.z0[2 * q + 1]);
         cplx g(p.alpha[q], p.beta[q]);

@@ -176,5 +176,7 @@ void dev_state_upload(cem_sim *s, int which, int comp, int w, const double *src)
 void dev_phasors(cem_sim *s, double *phE, double *phH, int download);
 void dev_acc_io(cem_sim *s, double *buf, int download);   // all DFT accumulators, checkpoint order
 void dev_set_step(cem_sim *s, long step);
+int dev_mode_waves(cem_sim *s, int vp0, int vp1, int ip, const double *mode, double dx, double dist, int dir,
+                   double *a, double *b);   // wave separation of one mode port on the device
 
 }  // namespace cem

@@ -130,6 +130,7 @@ AMD_SIGNATURES = {
     "cem_amd_synchronize": (_I, [_P]),
     "cem_amd_step_device": (_I, [_P, _L]),
     "cem_amd_copy_bandwidth": (_D, [_S, _I]),
+    "cem_amd_s2_renormalize": (_I, [_DP, _I, _DP, _DP, _D]),
 }
 
 
@@ -186,6 +187,17 @@ class Courant:
         deps = np.zeros(npole)
         err = self.lib.cem_fit_djordjevic_sarkar(dk, df, fref, fmin, fmax, npole, C.byref(einf), _dp(fp), _dp(deps))
         return {"eps_inf": einf.value, "fp": fp, "deps": deps, "tan_delta_error": err}
+
+    def renormalize_batch(self, S, z1, z2, zref):
+        """S: nf x 2 x 2 complex, z1, z2: nf reference impedances; renormalised on the GPU."""
+        S = np.asarray(S, complex)
+        nf = S.shape[0]
+        v = np.ascontiguousarray(np.stack([S.real, S.imag], -1).reshape(nf, 8))
+        rc = self.lib.cem_amd_s2_renormalize(_dp(v), nf, _dp(_f64(z1)), _dp(_f64(z2)), zref)
+        if rc < 0:
+            raise CourantError(self.last_error())
+        v = v.reshape(nf, 2, 2, 2)
+        return v[..., 0] + 1j * v[..., 1]
 
     def renormalize2(self, S, z1, z2, zref):
         """S: 4 complex values (S11, S12, S21, S22); returns the renormalised 4."""

@@ -30,6 +30,13 @@ CEM_API int cem_amd_set_profiling(cem_sim *s, int on);
  * every kernel directly (no graph replay). */
 CEM_API int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count);
 
+/* 2x2 S-matrix renormalisation of nf frequencies on the GPU in FP64, in place:
+ * S holds 8 doubles per frequency (S11, S12, S21, S22 as re, im), z1[q], z2[q]
+ * are the ports' reference impedances at frequency q.  Same formula as
+ * cem_s2_renormalize (solver/include/courant.h, modeport.c:242-264), batched
+ * over frequencies.  Returns 0, or -1 with cem_last_error() set. */
+CEM_API int cem_amd_s2_renormalize(double *S, int nf, const double *z1, const double *z2, double zref);
+
 /* Stream-copy bandwidth of the current GPU in GB/s (read + write bytes), best
  * of `iters` copies of `bytes` between two device buffers; -1 on error. */
 CEM_API double cem_amd_copy_bandwidth(size_t bytes, int iters);

@@ -134,6 +134,43 @@ def test_lossy_mode_port_waves(cem, ref, gpu_available):
     assert np.abs(s11a - s11b).max() <= 1e-4
 
 
+def test_device_wave_separation_matches_host_formula(cem, gpu_available, monkeypatch):
+    """The FP64 wave separation on the GPU against the same formula on the host, on the
+    same simulation (same spectra, same solved port modes): 1e-12 of max |a|."""
+    print_case("mode-port wave separation, device vs host formula, 1500 steps")
+    b, ib = S.gcpw(cem, **MODE_MINI)
+    b.step(1500)
+    for p in ib["ports"]:
+        dev = b.mode_port_waves(p)
+        monkeypatch.setenv("COURANT_WAVES_HOST", "1")
+        host = b.mode_port_waves(p)
+        monkeypatch.delenv("COURANT_WAVES_HOST")
+        scale = np.max(np.abs(host["a"]))
+        for k in ("a", "b"):
+            err = np.max(np.abs(dev[k] - host[k])) / scale
+            print("port %d %s: max err %.2e of max |a|" % (p, k, err))
+            assert err <= 1e-12
+        np.testing.assert_array_equal(dev["z0"], host["z0"])
+        np.testing.assert_array_equal(dev["beta"], host["beta"])
+
+
+def test_device_renormalize_matches_reference(cem, ref, gpu_available):
+    """Batched FP64 2x2 renormalisation on the GPU against the reference's per-frequency
+    cem_s2_renormalize (1e-13 relative), and the identity when z1 = z2 = zref."""
+    print_case("s2 renormalisation, 257 frequencies")
+    rng = np.random.default_rng(7)
+    nf = 257
+    Sm = 0.6 * (rng.normal(size=(nf, 2, 2)) + 1j * rng.normal(size=(nf, 2, 2))) / np.sqrt(2)
+    z1 = rng.uniform(20, 90, nf)
+    z2 = rng.uniform(20, 90, nf)
+    got = cem.renormalize_batch(Sm, z1, z2, 50.0)
+    for q in range(nf):
+        want = np.array(ref.renormalize2(list(Sm[q].ravel()), z1[q], z2[q], 50.0)).reshape(2, 2)
+        np.testing.assert_allclose(got[q], want, rtol=1e-13, atol=1e-14)
+    z = np.full(nf, 42.0)
+    np.testing.assert_allclose(cem.renormalize_batch(Sm, z, z, 42.0), Sm, rtol=1e-13, atol=1e-14)
+
+
 def test_checkpoint_resume_is_bit_exact(cem, gpu_available):
     """900 straight steps == 400 + save + load into a fresh sim + 500 (test_checkpoint.c:245-264)."""
     print_case("checkpoint resume")
