@@ -273,18 +273,23 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
         const float dx_ez = ez_i - ez, dx_ey = ey_i - ey;
         const float hx = iin ? a.hs[0][i] : 0.f;
         const int sx = a.has_pml && iin ? a.pml.slot[0][i] : -1;
+        // psi of the y and z slabs, loaded for every lane through clamped offsets
+        // (offset 0 lies inside every psi array) so the loads are not behind
+        // lane-divergent branches; stores stay conditional
+        const long long oy = sy >= 0 ? sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k : 0;
+        const long long oz = sz >= 0 ? psi_z(g, a.pml, i, j, sz) : 0;
+        const float v0y = a.has_pml ? a.psi[0][0][oy] : 0.f, v0z = a.has_pml ? a.psi[0][1][oz] : 0.f;
+        const float v1z = a.has_pml ? a.psi[1][0][oz] : 0.f, v2y = a.has_pml ? a.psi[2][1][oy] : 0.f;
         if (jin && kin) {   // Hx: all i
             float h = Hx[q] - (dy_ez * hy - dz_ey * hz);
             if (sy >= 0) {
-                float *ps = a.psi[0][0] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
-                const float v = by * *ps + cy * dy_ez;
-                *ps = v;
+                const float v = by * v0y + cy * dy_ez;
+                a.psi[0][0][oy] = v;
                 h -= a.dth * v;
             }
             if (sz >= 0) {
-                float *ps = a.psi[0][1] + psi_z(g, a.pml, i, j, sz);
-                const float v = bz * *ps + cz * dz_ey;
-                *ps = v;
+                const float v = bz * v0z + cz * dz_ey;
+                a.psi[0][1][oz] = v;
                 h += a.dth * v;
             }
             Hx[q] = h;
@@ -298,9 +303,8 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
                 h += a.dth * v;
             }
             if (sz >= 0) {
-                float *ps = a.psi[1][0] + psi_z(g, a.pml, i, j, sz);
-                const float v = bz * *ps + cz * dz_ex;
-                *ps = v;
+                const float v = bz * v1z + cz * dz_ex;
+                a.psi[1][0][oz] = v;
                 h -= a.dth * v;
             }
             Hy[q] = h;
@@ -314,9 +318,8 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
                 h -= a.dth * v;
             }
             if (sy >= 0) {
-                float *ps = a.psi[2][1] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
-                const float v = by * *ps + cy * dy_ex;
-                *ps = v;
+                const float v = by * v2y + cy * dy_ex;
+                a.psi[2][1][oy] = v;
                 h += a.dth * v;
             }
             Hz[q] = h;
@@ -377,6 +380,13 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
             cy2 = make_float2(j < g.ny ? ca[g.N + q] : 0.f, j < g.ny ? cb[g.N + q] : 0.f);
             cz2 = make_float2(k < g.nz ? ca[2 * g.N + q] : 0.f, k < g.nz ? cb[2 * g.N + q] : 0.f);
         }
+        // psi of the y and z slabs, loaded for every lane through clamped offsets
+        // (offset 0 lies inside every psi array) so the loads are not behind
+        // lane-divergent branches; stores stay conditional
+        const long long oy = sy >= 0 ? sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k : 0;
+        const long long oz = sz >= 0 ? psi_z(g, a.pml, i, j, sz) : 0;
+        const float v0y = a.has_pml ? a.psi[0][0][oy] : 0.f, v0z = a.has_pml ? a.psi[0][1][oz] : 0.f;
+        const float v1z = a.has_pml ? a.psi[1][0][oz] : 0.f, v2y = a.has_pml ? a.psi[2][1][oy] : 0.f;
         if (i < g.nx) {     // Ex
             const float e0 = Ex[q], cbv = cx.y;
             float e = e0;
@@ -384,15 +394,13 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                 const float dy_hz = hz - hz_j, dz_hy = hy - hy_k;
                 e = cx.x * e0 + cbv * (dy_hz * ey_s - dz_hy * ez_s);
                 if (sy >= 0) {
-                    float *ps = a.psi[0][0] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
-                    const float v = by * *ps + cy * dy_hz;
-                    *ps = v;
+                    const float v = by * v0y + cy * dy_hz;
+                    a.psi[0][0][oy] = v;
                     e += cbv * v;
                 }
                 if (sz >= 0) {
-                    float *ps = a.psi[0][1] + psi_z(g, a.pml, i, j, sz);
-                    const float v = bz * *ps + cz * dz_hy;
-                    *ps = v;
+                    const float v = bz * v0z + cz * dz_hy;
+                    a.psi[0][1][oz] = v;
                     e -= cbv * v;
                 }
             }
@@ -411,9 +419,8 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     e -= cbv * v;
                 }
                 if (sz >= 0) {
-                    float *ps = a.psi[1][0] + psi_z(g, a.pml, i, j, sz);
-                    const float v = bz * *ps + cz * dz_hx;
-                    *ps = v;
+                    const float v = bz * v1z + cz * dz_hx;
+                    a.psi[1][0][oz] = v;
                     e += cbv * v;
                 }
             }
@@ -432,9 +439,8 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     e += cbv * v;
                 }
                 if (sy >= 0) {
-                    float *ps = a.psi[2][1] + sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k;
-                    const float v = by * *ps + cy * dy_hx;
-                    *ps = v;
+                    const float v = by * v2y + cy * dy_hx;
+                    a.psi[2][1][oy] = v;
                     e -= cbv * v;
                 }
             }
