@@ -241,6 +241,7 @@ def main():
     stats = {k: sim.kernel_stats(i) for i, k in enumerate(("H", "E", "gather", "dft", "post"))}
     sim.set_profiling(False)
     ab = algorithmic_bytes(sim)
+    shape = sim.launch_shape()
     copy_gbs = cem.lib.cem_amd_copy_bandwidth(2 << 30, 10)
     value = cells * args.steps / wall / 1e6
     dom = max(("H", "E"), key=lambda k: stats[k][0])
@@ -278,7 +279,8 @@ def main():
         "kernels_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in stats.items()},
         "algorithmic_mb": {k: round(ab[k] / 1e6, 1) for k in ("H", "E", "post")},
         "launch": {"timed": "graph replay", "ms_per_step_direct_launches": round(wall_direct / nprof * 1e3, 5),
-                   "profiled_steps": nprof},
+                   "profiled_steps": nprof, "iplanes_per_thread": {"H": shape[0], "E": shape[1]},
+                   "autotuned": shape[2]},
     }
     sim.close()
     if not args.no_pmc:

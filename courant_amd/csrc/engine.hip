@@ -33,7 +33,9 @@
 #include <cstring>
 #include <algorithm>
 #include <chrono>
+#include <array>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -973,8 +975,10 @@ struct Device {
     // energy
     DBuf<double> d1d, partial;
     int energy_blocks = 0;
-    // launch shape: x planes per thread in the stencil kernels
+    // launch shape: x planes per thread in the stencil kernels, set by the autotuner
+    // on the first steps unless fixed by COURANT_ISEG_H / COURANT_ISEG_E
     int iseg_h = kIseg, iseg_e = kIseg;
+    bool tuned = false;
     // timing
     bool profiling = false;
     KernelTimer tH, tE, tG, tD, tP;
@@ -1721,8 +1725,10 @@ int cem::dev_ensure(cem_sim *s) {
     }
     D->profiling = std::getenv("COURANT_PROFILE") != nullptr;
     if (const char *v = std::getenv("COURANT_GRAPH")) D->use_graph = std::atoi(v) != 0;
-    if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v));
-    if (const char *v = std::getenv("COURANT_ISEG_E")) D->iseg_e = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v)), D->tuned = true;
+    if (const char *v = std::getenv("COURANT_ISEG_E")) D->iseg_e = std::max(1, std::atoi(v)), D->tuned = true;
+    if (const char *v = std::getenv("COURANT_AUTOTUNE"))
+        if (std::atoi(v) == 0) D->tuned = true;
     s->spec_src.assign(2 * s->nf, 0.0);
     for (size_t p = 0; p < s->port.size(); p++) {
         s->spec_port_v[p].assign(2 * s->nf, 0.0);
@@ -1789,11 +1795,76 @@ void cem::dev_sources_changed(cem_sim *s) {
     if (s->dev) s->dev->sources_dirty = true;
 }
 
+// Autotuner of the stencil launch shape (i-planes walked per thread).  Which
+// thread updates a point does not change any value, so the candidates run on
+// the simulation's own first steps: after one warm-up step each candidate runs
+// kTuneSteps steps with event-stamped launches, H and E each take their fastest
+// candidate, and the choice is kept for later simulations of the same grid
+// shape in this process (a batch of equal boards tunes once).
+constexpr int kTuneCand[] = {1, 2, 4, 8};
+constexpr int kTuneSteps = 3;
+constexpr long kTuneCost = 1 + (long)(sizeof kTuneCand / sizeof kTuneCand[0]) * kTuneSteps;
+
+std::mutex g_tune_mu;
+std::map<std::array<long long, 5>, std::pair<int, int>> g_tune;
+
+long autotune(cem_sim *s, Device *D, long budget) {
+    const std::array<long long, 5> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            D->iseg_h = it->second.first;
+            D->iseg_e = it->second.second;
+            D->tuned = true;
+            return 0;
+        }
+    }
+    if (budget < kTuneCost) return 0;   // too few steps in this call: tune on a later one
+    const int nx = s->n[0];
+    launch_h(s, D, 0, nx + 1);          // warm-up step (first launches, caches)
+    launch_e(s, D, 0, nx);
+    launch_post(s, D);
+    D->profiling = true;
+    double best_h = 1e300, best_e = 1e300;
+    int ch = D->iseg_h, ce = D->iseg_e;
+    for (int c : kTuneCand) {
+        D->iseg_h = D->iseg_e = c;
+        for (KernelTimer *t : {&D->tH, &D->tE}) {
+            t->fold();
+            t->total_ms = 0;
+            t->count = 0;
+        }
+        for (int k = 0; k < kTuneSteps; k++) {
+            launch_h(s, D, 0, nx + 1);
+            launch_e(s, D, 0, nx);
+            launch_post(s, D);
+        }
+        D->tH.fold();
+        D->tE.fold();
+        if (D->tH.total_ms < best_h) best_h = D->tH.total_ms, ch = c;
+        if (D->tE.total_ms < best_e) best_e = D->tE.total_ms, ce = c;
+    }
+    for (KernelTimer *t : {&D->tH, &D->tE, &D->tG, &D->tD, &D->tP}) {
+        t->fold();
+        t->total_ms = 0;
+        t->count = 0;
+    }
+    D->profiling = false;
+    D->iseg_h = ch;
+    D->iseg_e = ce;
+    D->tuned = true;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = {ch, ce};
+    return kTuneCost;
+}
+
 void cem::dev_step(cem_sim *s, long nsteps) {
     Device *D = s->dev;
     ensure_vlast(s, D);
     const int nx = s->n[0];
     long it = 0;
+    if (!D->tuned && !D->profiling) it += autotune(s, D, nsteps);
     while (it < nsteps) {
         // whole chunks from a captured graph once the series row is chunk-aligned
         if (D->use_graph && !D->profiling && D->rows % kGraphSteps == 0 && nsteps - it >= kGraphSteps &&
@@ -2180,6 +2251,16 @@ extern "C" int cem_amd_s2_renormalize(double *S, int nf, const double *z1, const
 
 // Stream-copy bandwidth of this GPU: copies `bytes` (rounded down to 16 B) between
 // two device buffers `iters` times and returns the best GB/s (read + write bytes).
+extern "C" int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e) {
+    if (!s || !s->dev) {
+        set_error("simulation has not run on the GPU yet");
+        return -1;
+    }
+    if (iseg_h) *iseg_h = s->dev->iseg_h;
+    if (iseg_e) *iseg_e = s->dev->iseg_e;
+    return s->dev->tuned ? 1 : 0;
+}
+
 extern "C" double cem_amd_copy_bandwidth(size_t bytes, int iters) {
     const long long n = (long long)(bytes / 16);
     if (n <= 0 || iters <= 0) return -1;

@@ -131,6 +131,7 @@ AMD_SIGNATURES = {
     "cem_amd_step_device": (_I, [_P, _L]),
     "cem_amd_copy_bandwidth": (_D, [_S, _I]),
     "cem_amd_s2_renormalize": (_I, [_DP, _I, _DP, _DP, _D]),
+    "cem_amd_launch_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 
@@ -458,6 +459,12 @@ class Sim:
 
     def set_profiling(self, on):
         return self._ok(self.lib.cem_amd_set_profiling(self.h, 1 if on else 0))
+
+    def launch_shape(self):
+        """(i-planes per thread of the H update, of the E update, fixed yet)."""
+        h, e = C.c_int(), C.c_int()
+        rc = self._ok(self.lib.cem_amd_launch_shape(self.h, C.byref(h), C.byref(e)))
+        return h.value, e.value, rc == 1
 
     def kernel_stats(self, which):
         """(total GPU ms, launches) of kernel `which`: 0 H update, 1 E update, 2 gather, 3 DFT, 4 post."""

@@ -37,6 +37,11 @@ CEM_API int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *
  * over frequencies.  Returns 0, or -1 with cem_last_error() set. */
 CEM_API int cem_amd_s2_renormalize(double *S, int nf, const double *z1, const double *z2, double zref);
 
+/* Launch shape of the stencil kernels (x planes walked per thread by the H and
+ * E updates).  Returns 1 once the autotuner (or COURANT_ISEG_H / _E) has fixed
+ * it, 0 before, -1 if the simulation has not run on the GPU. */
+CEM_API int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e);
+
 /* Stream-copy bandwidth of the current GPU in GB/s (read + write bytes), best
  * of `iters` copies of `bytes` between two device buffers; -1 on error. */
 CEM_API double cem_amd_copy_bandwidth(size_t bytes, int iters);

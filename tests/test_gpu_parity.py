@@ -390,6 +390,26 @@ def test_coefficient_table_matches_coefficient_arrays(cem, gpu_available, monkey
     assert a.energy() == b.energy()
 
 
+def test_autotuned_launch_shape_is_bit_identical(cem, gpu_available, monkeypatch):
+    """The autotuner picks the i-planes per thread on the first steps; a run with a fixed,
+    different shape gives bit-identical fields and spectra."""
+    print_case("autotuned launch shape vs fixed shape")
+    a = S.checkpoint_scene(cem)
+    a.step(300)
+    h, e, fixed = a.launch_shape()
+    assert fixed and h in (1, 2, 4, 8) and e in (1, 2, 4, 8)
+    monkeypatch.setenv("COURANT_ISEG_H", "1" if h != 1 else "4")
+    monkeypatch.setenv("COURANT_ISEG_E", "8" if e != 8 else "1")
+    b = S.checkpoint_scene(cem)
+    b.step(300)
+    monkeypatch.delenv("COURANT_ISEG_H")
+    monkeypatch.delenv("COURANT_ISEG_E")
+    assert b.launch_shape()[:2] != (h, e)
+    for fa, fb in zip(fields(a), fields(b)):
+        assert np.array_equal(fa, fb)
+    assert np.array_equal(a.port_v(1), b.port_v(1))
+
+
 @pytest.mark.parametrize("scene", ["debye_sibc", "mode_lossy"])
 def test_graph_replay_matches_direct_launches(cem, gpu_available, monkeypatch, scene):
     """Runs of steps replayed from the captured graph and the same steps launched one
