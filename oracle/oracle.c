@@ -11,6 +11,9 @@
  *         Debye update, port and voltage-probe DFTs, rotate   sim.c:894-933
  * With ORC_REAL = double every update uses the reference's expression order,
  * so results agree with the reference core to the last bit (checked in tests).
+ * ORC_REAL = float gives an FP32 emulation of the GPU step; with ORC_HALF_STORE as
+ * well, every stored state is rounded to binary16 (the accuracy check of a
+ * half-precision storage variant).
  */
 #include "oracle.h"
 
@@ -26,6 +29,45 @@ typedef ORC_REAL real;
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
+#endif
+
+#ifdef ORC_HALF_STORE
+/* FP16-storage emulation (with ORC_REAL = float): every field, psi, Debye current
+ * and SIBC state is rounded to IEEE binary16 where a storage variant would write
+ * it, while arithmetic stays FP32.  Round to nearest even on the float's bits:
+ * 10 significand bits kept for normal halves, the 2^-24 grid below 2^-14
+ * (subnormal halves), overflow to infinity from 65520 up. */
+#include <stdint.h>
+static float half_round(float x) {
+    union { float f; uint32_t u; } v = {x};
+    const uint32_t sign = v.u & 0x80000000u;
+    uint32_t a = v.u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return x;
+    if (a >= 0x477ff000u) { v.u = sign | 0x7f800000u; return v.f; }
+    if (a < 0x38800000u) {
+        v.f = nearbyintf(fabsf(x) * 16777216.0f) / 16777216.0f;
+        v.u |= sign;
+        return v.f;
+    }
+    a += 0x0fffu + ((a >> 13) & 1u);
+    v.u = sign | (a & ~0x1fffu);
+    return v.f;
+}
+/* Stored values sit on a binary16 grid scaled by a power of two per array kind (a
+ * storage variant folds the scale into its coefficients): the scale changes the
+ * representable range, not the relative precision.  CPML psi carries a 1/d factor
+ * (~1e7 for 1e3 V/m over 40 um) and Debye J a 1/dt factor, far beyond binary16's
+ * 65504 unscaled. */
+#define HALF_SCALE_E 1.0f
+#define HALF_SCALE_H 64.0f
+#define HALF_SCALE_PSI (1.0f / 65536.0f)
+#define HALF_SCALE_J (1.0f / 4294967296.0f)
+#define HALF_SCALE_Y 64.0f
+static void half_store(real *p, size_t n, float scale) {
+    for (size_t i = 0; i < n; i++) p[i] = half_round(p[i] * scale) / scale;
+}
+/* the rounding itself, exported for its check against an IEEE binary16 conversion */
+double orc_half_round(double x) { return (double)half_round((float)x); }
 #endif
 #define ORC_EPS0 8.8541878128e-12
 #define ORC_MU0 1.25663706212e-6
@@ -460,6 +502,17 @@ static double mode_wave(const orc_sim *s, double t) {
     return -2 * x * exp(-x * x);
 }
 
+#ifdef ORC_HALF_STORE
+static void half_store_psi(orc_sim *s, int is_e) {
+    if (!s->has_pml) return;
+    for (int c = 0; c < 3; c++)
+        for (int w = 0; w < 2; w++) {
+            const int a = (c + w + 1) % 3;
+            half_store(is_e ? s->psiE[c][w] : s->psiH[c][w], pml_len(s, a, is_e), HALF_SCALE_PSI);
+        }
+}
+#endif
+
 void orc_step(orc_sim *s, long nsteps) {
     const int nf = s->nf;
     const size_t blk = 2 * (size_t)nf;
@@ -474,11 +527,19 @@ void orc_step(orc_sim *s, long nsteps) {
             }
         }
         step_H(s);
+#ifdef ORC_HALF_STORE
+        half_store(s->F + 3 * s->N, 3 * s->N, HALF_SCALE_H);
+        half_store_psi(s, 0);
+#endif
         /* mid (sim.c:872-892): current probes, port V before, Debye save */
         for (int p = 0; p < s->nip; p++) dft_add(s->accIp + p * blk, s->phH, nf, sum_terms(s, &s->ip[p]) * s->dt);
         for (int p = 0; p < s->nport; p++) s->vold[p] = sum_terms(s, &s->pv[p]);
         for (int e = 0; e < s->ndisp; e++) s->d_Eold[e] = s->F[s->d_idx[e]];
         step_E(s);
+#ifdef ORC_HALF_STORE
+        half_store(s->F, 3 * s->N, HALF_SCALE_E);
+        half_store_psi(s, 1);
+#endif
         /* post (sim.c:894-933) */
         double tH = (s->step + 0.5) * s->dt;
         for (int e = 0; e < s->ndisp; e++) {     /* Debye correct (dispersive.c:93-100) */
@@ -518,6 +579,11 @@ void orc_step(orc_sim *s, long nsteps) {
                 }
             }
         }
+#ifdef ORC_HALF_STORE
+        half_store(s->F, 3 * s->N, HALF_SCALE_E);
+        half_store(s->d_J, (size_t)s->npole * s->ndisp, HALF_SCALE_J);
+        half_store(s->sb_y, (size_t)s->nsibc * s->nterm, HALF_SCALE_Y);
+#endif
         for (int p = 0; p < s->nport; p++) {
             double V = 0.5 * (sum_terms(s, &s->pv[p]) + s->vold[p]);
             double vs = p == s->src_port ? Vs : 0.0;

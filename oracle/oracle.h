@@ -78,6 +78,8 @@ double orc_energy(const orc_sim *s);
  * distance d = dir (x_ref - x_mon) and dir; outputs a and b (2*nf). */
 void orc_mode_waves(int nf, const double *V0, const double *V1, const double *Ix, const double *z0,
                     const double *alpha, const double *beta, double dx, double d, int dir, double *a, double *b);
+/* liboracle_f16s.so only (ORC_HALF_STORE): x rounded to IEEE binary16 and back */
+double orc_half_round(double x);
 /* 2x2 renormalisation (modeport.c:242-264) */
 void orc_s2_renormalize(double *S, double z1, double z2, double zref);
 
