@@ -79,10 +79,17 @@ def cpu_baseline(target_s=12.0):
     s.step(n)
     wall = time.perf_counter() - t
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    w_ref = s.energy()
     s.close()
+    # the GPU engine on the same board for the same steps: field-energy relative error
+    gpu, _ = scenes.gcpw(lib.Courant(), **dict(scenes.CFG2, ports="lumped", probes=False))
+    gpu.step(5 + n)
+    w_gpu = gpu.energy()
+    gpu.close()
     return {"value": round(cells * n / wall / 1e6, 2), "unit": "Mcell-updates/s", "cores": cores,
             "kind": "reference",
-            "sample": f"reference f64 OpenMP core, same board with lumped ports, {n} steps ({wall:.1f} s)"}
+            "sample": f"reference f64 OpenMP core, same board with lumped ports, {n} steps ({wall:.1f} s)",
+            "energy_rel_err_gpu_vs_f64": float(f"{abs(w_gpu - w_ref) / w_ref:.3e}"), "energy_steps": 5 + n}
 
 
 def _pmc_pass(counter, args_tail, tmp, tag):
@@ -139,8 +146,17 @@ def batched_and_served(cem, nboards, workers, nreq, rate, budget_gb):
     5000 + r) arriving as a Poisson stream at `rate` boards/s, admitted by estimated
     footprint under `budget_gb`, against the same boards run one at a time."""
     from courant_amd import serve
+    serve.run_batch(cem, [dict(serve.random_board(4000), nx=80, ny=80, nz=60, steps=64)], 1)   # context warm-up
+    batched = served = None
+    if nboards > 0:
+        batched = _batched(cem, serve, nboards, workers)
+    if nreq > 0:
+        served = _served(cem, serve, nreq, rate, workers, budget_gb)
+    return batched, served
+
+
+def _batched(cem, serve, nboards, workers):
     boards = [serve.random_board(4000 + b) for b in range(nboards)]
-    serve.run_batch(cem, [dict(boards[0], nx=80, ny=80, nz=60, steps=64)], 1)   # context warm-up
     s1, w1, u1 = serve.run_batch(cem, boards, 1)
     print(f"batched: one at a time {w1:.2f} s", file=sys.stderr, flush=True)
     sb, wb, ub = serve.run_batch(cem, boards, workers)
@@ -155,6 +171,10 @@ def batched_and_served(cem, nboards, workers, nreq, rate, budget_gb):
         "one_at_a_time_mcell_updates_per_s": round(u1 / w1 / 1e6, 1),
         "bitwise_same_as_one_at_a_time": all(np.array_equal(a, b) for a, b in zip(s1, sb)),
     }
+    return batched
+
+
+def _served(cem, serve, nreq, rate, workers, budget_gb):
     reqs = [serve.random_request(5000 + r) for r in range(nreq)]
     nsol = len(serve.jobs_of(reqs))
     t0 = time.perf_counter()
@@ -173,7 +193,7 @@ def batched_and_served(cem, nboards, workers, nreq, rate, budget_gb):
         "p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
         "bitwise_same_as_one_at_a_time": all(np.array_equal(a, b) for a, b in zip(seq, got)),
     }
-    return batched, served
+    return served
 
 
 def main():
@@ -191,8 +211,8 @@ def main():
     ap.add_argument("--copy-only", type=float, default=0,
                     help="only run the stream-copy kernel on this many GB (byte-counter calibration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served measurements")
-    ap.add_argument("--batch-boards", type=int, default=8, help="boards of the batched set (of 64)")
-    ap.add_argument("--serve-requests", type=int, default=12, help="boards of the served stream")
+    ap.add_argument("--batch-boards", type=int, default=8, help="boards of the batched set (of 64; 0: none)")
+    ap.add_argument("--serve-requests", type=int, default=12, help="boards of the served stream (0: none)")
     ap.add_argument("--serve-rate", type=float, default=5.0, help="Poisson arrival rate, boards/s")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solves (host threads, one stream each)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="device footprint admitted at once")
@@ -291,8 +311,12 @@ def main():
             out["roofline"]["traffic_by_kernel"] = {k: round(v) for k, v in tr["kernels"].items()}
             out["roofline"]["pmc_scale"] = {"FETCH_SIZE": tr["fetch_scale"], "WRITE_SIZE": tr["write_scale"]}
     if not args.no_batch:
-        out["batched"], out["served"] = batched_and_served(cem, args.batch_boards, args.workers,
-                                                           args.serve_requests, args.serve_rate, args.budget_gb)
+        b, v = batched_and_served(cem, args.batch_boards, args.workers, args.serve_requests, args.serve_rate,
+                                  args.budget_gb)
+        if b:
+            out["batched"] = b
+        if v:
+            out["served"] = v
     if not args.no_cpu and args.workload == "cfg2":
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
