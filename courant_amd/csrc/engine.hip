@@ -178,11 +178,14 @@ struct PostArgs {
     const PostParams *pp;
     int ndisp, nsb, nsbo, nxs;
     const unsigned *d_idx;
-    const unsigned *d_mat;  // material of each dispersive edge: row of dtab
+    const unsigned *d_mat;  // per dispersive edge: row of dtab (bits 0-29), has an SIBC slot
+                            // (bit 31), has a source slot (bit 30)
     const float *dtab;      // per material: cb, then beta/dt per pole
     int dstride;            // 1 + npole
-    const int *d_src;       // source slot of a dispersive edge, -1 if none (nullptr: none at all)
-    const int *d_sb;        // SIBC slot of a dispersive edge, -1 if none (nullptr: none at all)
+    // the few dispersive edges that are also SIBC or source edges: ascending edge
+    // numbers with their slots (found by binary search from the flag bits)
+    const int *dsb_t, *dsb_u, *dsrc_t, *dsrc_x;
+    int ndsb, ndsrc;
     float *J;               // npole * ndisp (pole-major, reference order)
     float *Eold;            // E of each dispersive edge before this step's E update
     const unsigned *sb_idx; // per SIBC slot (reference order)
@@ -511,6 +514,18 @@ __device__ void gather_samples(const GatherArgs &a, int row, int wave0, int nwav
 }
 
 // Post phase of one step: Debye, SIBC and source edges.
+constexpr unsigned kDispSibc = 0x80000000u, kDispSrc = 0x40000000u, kDispMat = 0x3fffffffu;
+
+// position of edge t in an ascending list that holds it
+__device__ __forceinline__ int find_edge(const int *list, int n, int t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (list[mid] < t) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const PostParams &pp = *a.pp;
@@ -519,18 +534,16 @@ __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
     waveforms(a.wave, n, &vs, &gm);
     if (t < a.ndisp) {                       // dispersive edge
         const unsigned q = a.d_idx[t];
-        const float *mat = a.dtab + a.d_mat[t] * a.dstride;
+        const unsigned m = a.d_mat[t];
+        const float *mat = a.dtab + (m & kDispMat) * a.dstride;
         float e = a.E[q];
         float acc = 0.f;
         for (int p = 0; p < pp.npole; p++) acc += pp.kc[p] * a.J[(long long)p * a.ndisp + t];
         e -= mat[0] * acc;
-        if (a.d_sb) {
-            const int u = a.d_sb[t];
-            if (u >= 0) e = sibc_edge(a, pp, u);
-        }
-        if (a.d_src) {
-            const int x = a.d_src[t];
-            if (x >= 0) e += a.wl[x] * vs + a.wm[x] * gm;
+        if (m & kDispSibc) e = sibc_edge(a, pp, a.dsb_u[find_edge(a.dsb_t, a.ndsb, t)]);
+        if (m & kDispSrc) {
+            const int x = a.dsrc_x[find_edge(a.dsrc_t, a.ndsrc, t)];
+            e += a.wl[x] * vs + a.wm[x] * gm;
         }
         const float de = e - a.Eold[t];
         for (int p = 0; p < pp.npole; p++) {
@@ -941,9 +954,12 @@ struct Device {
     Pml pmlH{}, pmlE{};
     // post-phase edge lists
     DBuf<unsigned> d_idx, sb_idx, sbo, x_idx;
-    DBuf<int> d_src, d_sb, sb_src, x_src;
+    DBuf<int> sb_src, x_src;
+    DBuf<int> dsb_t, dsb_u, dsrc_t, dsrc_x;   // dispersive edges with SIBC / source slots
+    std::vector<unsigned> d_mat_host;          // material rows of the dispersive edges, no flags
+    std::vector<int> dsb_t_host;               // ascending dispersive edges with an SIBC slot
     int nsbo = 0;
-    bool has_dsb = false, has_sbsrc = false;
+    bool has_sbsrc = false;
     DBuf<float> J, dtab, Eold, Y, wl, wm;
     DBuf<unsigned> d_mat;
     int dstride = 1;
@@ -953,7 +969,7 @@ struct Device {
     long long ctl_step = -1;     // what ctl holds once the queued work has run
     int ctl_row = -1;
     int nxs = 0;
-    bool has_dsrc = false;
+    int ndsb = 0, ndsrc = 0;
     bool eold_stale = false;
     bool sources_dirty = true;
     // probes
@@ -1016,7 +1032,8 @@ long surface_slot(const cem_sim *s, unsigned long long g) {
 }
 
 // Excited sources as (global E index, lumped cs, modal weight) slots; edges that
-// are also dispersive get their slot through d_src, the others go to x_idx.
+// are also dispersive are flagged in d_mat and listed in dsrc_t / dsrc_x, the
+// others go to x_idx.
 int upload_sources(cem_sim *s, Device *D) {
     std::vector<std::pair<unsigned long long, std::pair<double, double>>> src;
     if (s->src_port >= 0) {
@@ -1042,16 +1059,16 @@ int upload_sources(cem_sim *s, Device *D) {
         wl.push_back((float)e.second.first);
         wm.push_back((float)e.second.second);
     }
-    std::vector<int> d_src(s->ndisp, -1), sb_src(s->nsb, -1), x_src;
+    std::vector<int> dsrc_t, dsrc_x, sb_src(s->nsb, -1), x_src;
     std::vector<unsigned> x_idx;
-    D->has_dsrc = D->has_sbsrc = false;
-    for (size_t k = 0; k < where.size(); k++) {
+    D->has_sbsrc = false;
+    for (size_t k = 0; k < where.size(); k++) {   // ascending global index, so t ascends too
         const unsigned long long g = where[k];
         const long t = dispersive_slot(s, g);
         const long u = t < 0 ? surface_slot(s, g) : -1;
         if (t >= 0) {
-            d_src[t] = (int)k;
-            D->has_dsrc = true;
+            dsrc_t.push_back((int)t);
+            dsrc_x.push_back((int)k);
         } else if (u >= 0) {
             sb_src[u] = (int)k;
             D->has_sbsrc = true;
@@ -1064,7 +1081,8 @@ int upload_sources(cem_sim *s, Device *D) {
     for (auto *b : {&D->wl, &D->wm}) b->release();
     D->x_idx.release();
     D->x_src.release();
-    D->d_src.release();
+    D->dsrc_t.release();
+    D->dsrc_x.release();
     D->sb_src.release();
     if (D->wl.alloc(wl.size(), D->st) || D->wm.alloc(wm.size(), D->st) || D->x_idx.alloc(x_idx.size(), D->st) ||
         D->x_src.alloc(x_src.size(), D->st))
@@ -1072,8 +1090,17 @@ int upload_sources(cem_sim *s, Device *D) {
     if (D->wl.upload(wl.data(), wl.size(), D->st) || D->wm.upload(wm.data(), wm.size(), D->st) ||
         D->x_idx.upload(x_idx.data(), x_idx.size(), D->st) || D->x_src.upload(x_src.data(), x_src.size(), D->st))
         return -1;
-    if (D->has_dsrc && (D->d_src.alloc(d_src.size(), D->st) || D->d_src.upload(d_src.data(), d_src.size(), D->st)))
+    D->ndsrc = (int)dsrc_t.size();
+    if (D->dsrc_t.alloc(dsrc_t.size(), D->st) || D->dsrc_x.alloc(dsrc_x.size(), D->st) ||
+        D->dsrc_t.upload(dsrc_t.data(), dsrc_t.size(), D->st) || D->dsrc_x.upload(dsrc_x.data(), dsrc_x.size(), D->st))
         return -1;
+    // material rows of the dispersive edges with the SIBC / source flag bits
+    if (s->ndisp) {
+        std::vector<unsigned> m(D->d_mat_host);
+        for (int t : D->dsb_t_host) m[t] |= kDispSibc;
+        for (int t : dsrc_t) m[t] |= kDispSrc;
+        if (D->d_mat.upload(m.data(), m.size(), D->st)) return -1;
+    }
     if (D->has_sbsrc && (D->sb_src.alloc(sb_src.size(), D->st) || D->sb_src.upload(sb_src.data(), sb_src.size(), D->st)))
         return -1;
     D->sources_dirty = false;
@@ -1251,7 +1278,12 @@ int upload_plan(cem_sim *s, Device *D) {
         }
         D->dstride = 1 + np;
         if (D->d_idx.alloc(s->ndisp, D->st) || D->d_idx.upload(idx.data(), s->ndisp, st)) return -1;
-        if (D->d_mat.alloc(s->ndisp, D->st) || D->d_mat.upload(mat.data(), s->ndisp, st)) return -1;
+        if (rows.size() > kDispMat) {
+            set_error("too many Debye materials");
+            return -1;
+        }
+        D->d_mat_host = mat;   // uploaded with its flag bits by upload_sources
+        if (D->d_mat.alloc(s->ndisp, D->st)) return -1;
         if (D->dtab.alloc(tab.size() + 1, D->st) || D->dtab.upload(tab.data(), tab.size(), st)) return -1;
         if (D->Eold.alloc(s->ndisp, D->st) || D->Eold.zero(st)) return -1;
         const size_t nj = (size_t)np * s->ndisp;
@@ -1279,14 +1311,14 @@ int upload_plan(cem_sim *s, Device *D) {
     // SIBC tables
     {
         std::vector<unsigned> idx(s->nsb), sbo;
-        std::vector<int> d_sb(s->ndisp, -1);
-        D->has_dsb = false;
-        for (int e = 0; e < s->nsb; e++) {
+        std::vector<int> dsb_u;
+        D->dsb_t_host.clear();
+        for (int e = 0; e < s->nsb; e++) {   // ascending global index, so t ascends too
             idx[e] = (unsigned)(s->sb_comp[e] * N + s->sb_idx[e]);
             const long t = dispersive_slot(s, idx[e]);
             if (t >= 0) {
-                d_sb[t] = e;
-                D->has_dsb = true;
+                D->dsb_t_host.push_back((int)t);
+                dsb_u.push_back(e);
             } else {
                 sbo.push_back((unsigned)e);
             }
@@ -1294,7 +1326,10 @@ int upload_plan(cem_sim *s, Device *D) {
         D->nsbo = (int)sbo.size();
         if (D->sb_idx.alloc(s->nsb, D->st) || D->sb_idx.upload(idx.data(), s->nsb, st)) return -1;
         if (D->sbo.alloc(sbo.size(), D->st) || D->sbo.upload(sbo.data(), sbo.size(), st)) return -1;
-        if (D->has_dsb && (D->d_sb.alloc(s->ndisp, D->st) || D->d_sb.upload(d_sb.data(), s->ndisp, st))) return -1;
+        D->ndsb = (int)dsb_u.size();
+        if (D->dsb_t.alloc(dsb_u.size(), D->st) || D->dsb_u.alloc(dsb_u.size(), D->st) ||
+            D->dsb_t.upload(D->dsb_t_host.data(), dsb_u.size(), st) || D->dsb_u.upload(dsb_u.data(), dsb_u.size(), st))
+            return -1;
         const size_t ny_ = (size_t)s->nsb * s->sibc_npole;
         if (D->Y.alloc(ny_, D->st) || D->Y.zero(st) || D->face.alloc(s->nsb, D->st)) return -1;
         if (D->face.upload(s->sb_mask.data(), s->nsb, st)) return -1;
@@ -1469,8 +1504,12 @@ PostArgs post_args(cem_sim *s, Device *D) {
     a.d_mat = D->d_mat.p;
     a.dtab = D->dtab.p;
     a.dstride = D->dstride;
-    a.d_src = D->has_dsrc ? D->d_src.p : nullptr;
-    a.d_sb = D->has_dsb ? D->d_sb.p : nullptr;
+    a.dsb_t = D->dsb_t.p;
+    a.dsb_u = D->dsb_u.p;
+    a.ndsb = D->ndsb;
+    a.dsrc_t = D->dsrc_t.p;
+    a.dsrc_x = D->dsrc_x.p;
+    a.ndsrc = D->ndsrc;
     a.sbo = D->sbo.p;
     a.sb_src = D->has_sbsrc ? D->sb_src.p : nullptr;
     a.J = D->J.p;
@@ -1766,8 +1805,7 @@ void cem::dev_destroy(cem_sim *s) {
     D->face.release();
     D->pp.release();
     for (auto *b : {&D->d_idx, &D->sb_idx, &D->sbo, &D->x_idx}) b->release();
-    D->d_src.release();
-    D->d_sb.release();
+    for (auto *b : {&D->dsb_t, &D->dsb_u, &D->dsrc_t, &D->dsrc_x}) b->release();
     D->sb_src.release();
     D->x_src.release();
     D->Eold.release();
