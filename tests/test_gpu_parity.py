@@ -114,6 +114,26 @@ def test_gcpw_lumped_sparameters(cem, ref, gpu_available):
     assert np.abs(s11a - s11b)[band].max() <= 1e-4
 
 
+def test_run_stops_where_the_reference_stops(cem, ref, gpu_available):
+    """cem_run (sim.c:996-1012): energy checked every 200 steps after the source has ended,
+    stop below 1e-4 of the peak.  Same number of steps as the f64 core, port spectra within
+    1e-4 relative, S21 within 1e-4 dB."""
+    print_case("cem_run decay stop, lumped gcpw")
+    a, ia = S.gcpw(ref, **GCPW_MINI)
+    b, ib = S.gcpw(cem, **GCPW_MINI)
+    na = a.run(30000, 1e-4, 200)
+    nb = b.run(30000, 1e-4, 200)
+    print("reference stopped after %d steps, GPU after %d" % (na, nb))
+    assert nb == na and 0 < na < 30000
+    for p in ia["ports"]:
+        assert rel(b.port_v(p), a.port_v(p)) < SPEC_TOL
+    s11a, s21a = S.gcpw_sparams(a, ia)
+    s11b, s21b = S.gcpw_sparams(b, ib)
+    band = (ia["freqs"] > 0) & (ia["freqs"] <= 40e9)
+    ddb, _ = db_deg_error(s21a[band], s21b[band])
+    assert ddb.max() <= 1e-4
+
+
 MODE_MINI = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=1.5e-3, feed=1.2e-3, mon=0.6e-3, yair=0.3e-3,
                  zair=0.4e-3, ports="mode", lossy=True, tcu=35e-6, mask=25e-6, dc=False, df=4e9)
 
