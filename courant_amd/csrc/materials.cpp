@@ -354,49 +354,74 @@ int cem::sibc_finalize(cem_sim *s, const unsigned char *is_port) {
     s->sb_hi.clear();
     s->sb_sg.clear();
     s->sb_mask.clear();
-    for (int c = 0; c < 3; c++) {
-        int ua = (c + 1) % 3, va = (c + 2) % 3;
-        for (int i = 0; i <= nx; i++)
-            for (int j = 0; j <= ny; j++)
-                for (int k = 0; k <= nz; k++) {
-                    int ijk[3] = {i, j, k};
-                    if (ijk[c] >= s->n[c]) continue;
-                    size_t q = IDX(s, i, j, k);
-                    if (!edge_is_metal(s, c, ijk)) continue;
-                    if (is_port && is_port[c * s->N + q]) continue;
-                    bool in_pml = false;   // edges in a CPML slab stay PEC
-                    for (int a = 0; a < 3; a++) {
-                        if (a == c) {
-                            if (ijk[a] < s->npml[2 * a] || ijk[a] >= s->n[a] - s->npml[2 * a + 1]) in_pml = true;
-                        } else if (ijk[a] < s->npml[2 * a] || ijk[a] > s->n[a] - s->npml[2 * a + 1]) {
-                            in_pml = true;
+    // surface edges in (c, i, j, k) order: each task scans a contiguous run of
+    // (component, x plane) pairs into its own lists, joined in order afterwards
+    struct Part {
+        std::vector<int> comp, nf, hc;
+        std::vector<size_t> idx, hi;
+        std::vector<double> sg;
+        std::vector<uint8_t> mask;
+    };
+    const long nplanes = 3L * (nx + 1);
+    const long nt = std::max<long>(1, std::min<long>(64, nplanes));
+    std::vector<Part> parts(nt);
+    parallel_ranges(nt, [&](long t0, long t1) {
+        for (long t = t0; t < t1; t++) {
+            Part &P = parts[t];
+            for (long r = nplanes * t / nt; r < nplanes * (t + 1) / nt; r++) {
+                const int c = (int)(r / (nx + 1)), i = (int)(r % (nx + 1));
+                const int ua = (c + 1) % 3, va = (c + 2) % 3;
+                for (int j = 0; j <= ny; j++)
+                    for (int k = 0; k <= nz; k++) {
+                        int ijk[3] = {i, j, k};
+                        if (ijk[c] >= s->n[c]) continue;
+                        size_t q = IDX(s, i, j, k);
+                        if (!edge_is_metal(s, c, ijk)) continue;
+                        if (is_port && is_port[c * s->N + q]) continue;
+                        bool in_pml = false;   // edges in a CPML slab stay PEC
+                        for (int a = 0; a < 3; a++) {
+                            if (a == c) {
+                                if (ijk[a] < s->npml[2 * a] || ijk[a] >= s->n[a] - s->npml[2 * a + 1]) in_pml = true;
+                            } else if (ijk[a] < s->npml[2 * a] || ijk[a] > s->n[a] - s->npml[2 * a + 1]) {
+                                in_pml = true;
+                            }
+                        }
+                        if (in_pml) continue;
+                        int hc[4], nf = 0, mask = 0;
+                        size_t hi[4];
+                        double sg[4];
+                        int nb[3];
+                        auto probe = [&](int axis, int delta) {
+                            std::memcpy(nb, ijk, sizeof nb);
+                            nb[axis] += delta;
+                            return !edge_is_metal(s, c, nb);
+                        };
+                        if (probe(va, +1)) { hc[nf] = ua; hi[nf] = q; sg[nf] = -1; nf++; mask |= 1; }
+                        if (probe(va, -1)) { hc[nf] = ua; hi[nf] = q - stride[va]; sg[nf] = 1; nf++; mask |= 2; }
+                        if (probe(ua, +1)) { hc[nf] = va; hi[nf] = q; sg[nf] = 1; nf++; mask |= 4; }
+                        if (probe(ua, -1)) { hc[nf] = va; hi[nf] = q - stride[ua]; sg[nf] = -1; nf++; mask |= 8; }
+                        if (!nf) continue;
+                        P.comp.push_back(c);
+                        P.idx.push_back(q);
+                        P.nf.push_back(nf);
+                        P.mask.push_back((uint8_t)mask);
+                        for (int f = 0; f < 4; f++) {
+                            P.hc.push_back(f < nf ? hc[f] : 0);
+                            P.hi.push_back(f < nf ? hi[f] : 0);
+                            P.sg.push_back(f < nf ? sg[f] / nf : 0.0);
                         }
                     }
-                    if (in_pml) continue;
-                    int hc[4], nf = 0, mask = 0;
-                    size_t hi[4];
-                    double sg[4];
-                    int nb[3];
-                    auto probe = [&](int axis, int delta) {
-                        std::memcpy(nb, ijk, sizeof nb);
-                        nb[axis] += delta;
-                        return !edge_is_metal(s, c, nb);
-                    };
-                    if (probe(va, +1)) { hc[nf] = ua; hi[nf] = q; sg[nf] = -1; nf++; mask |= 1; }
-                    if (probe(va, -1)) { hc[nf] = ua; hi[nf] = q - stride[va]; sg[nf] = 1; nf++; mask |= 2; }
-                    if (probe(ua, +1)) { hc[nf] = va; hi[nf] = q; sg[nf] = 1; nf++; mask |= 4; }
-                    if (probe(ua, -1)) { hc[nf] = va; hi[nf] = q - stride[ua]; sg[nf] = -1; nf++; mask |= 8; }
-                    if (!nf) continue;
-                    s->sb_comp.push_back(c);
-                    s->sb_idx.push_back(q);
-                    s->sb_nf.push_back(nf);
-                    s->sb_mask.push_back((uint8_t)mask);
-                    for (int f = 0; f < 4; f++) {
-                        s->sb_hc.push_back(f < nf ? hc[f] : 0);
-                        s->sb_hi.push_back(f < nf ? hi[f] : 0);
-                        s->sb_sg.push_back(f < nf ? sg[f] / nf : 0.0);
-                    }
-                }
+            }
+        }
+    });
+    for (const Part &P : parts) {
+        s->sb_comp.insert(s->sb_comp.end(), P.comp.begin(), P.comp.end());
+        s->sb_idx.insert(s->sb_idx.end(), P.idx.begin(), P.idx.end());
+        s->sb_nf.insert(s->sb_nf.end(), P.nf.begin(), P.nf.end());
+        s->sb_mask.insert(s->sb_mask.end(), P.mask.begin(), P.mask.end());
+        s->sb_hc.insert(s->sb_hc.end(), P.hc.begin(), P.hc.end());
+        s->sb_hi.insert(s->sb_hi.end(), P.hi.begin(), P.hi.end());
+        s->sb_sg.insert(s->sb_sg.end(), P.sg.begin(), P.sg.end());
     }
     s->nsb = (int)s->sb_idx.size();
     for (int k = 0; k < s->sibc_npole; k++) s->sibc_e[k] = std::exp(-s->sibc_a[k] * s->dt);

@@ -14,7 +14,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <new>
+#include <thread>
+#include <vector>
 
 using namespace cem;
 
@@ -518,10 +521,11 @@ extern "C" int cem_finalize(cem_sim *s, double cfl) {
         for (int f = 0; f < 6; f++)
             if (s->npml[f]) s->has_pml = 1;
 
-        // E-edge coefficients (sim.c:449-465)
+        // E-edge coefficients (sim.c:449-465), one (component, x plane) per task
         const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
-        for (int c = 0; c < 3; c++)
-            for (int i = 0; i <= nx; i++)
+        parallel_ranges(3L * (nx + 1), [&](long r0, long r1) {
+            for (long r = r0; r < r1; r++) {
+                const int c = (int)(r / (nx + 1)), i = (int)(r % (nx + 1));
                 for (int j = 0; j <= ny; j++)
                     for (int k = 0; k <= nz; k++) {
                         int ijk[3] = {i, j, k};
@@ -539,6 +543,8 @@ extern "C" int cem_finalize(cem_sim *s, double cfl) {
                         s->ca[c][q] = (1 - h) / (1 + h);
                         s->cb[c][q] = (s->dt / eps) / (1 + h);
                     }
+            }
+        });
 
         // lumped ports: resistive voltage source per edge (sim.c:467-492)
         for (auto &pt : s->port) {
@@ -576,46 +582,87 @@ extern "C" int cem_finalize(cem_sim *s, double cfl) {
 
         // Debye edges: ca/cb with eps_inf + sum beta/2 and the ADE tables (sim.c:501-549)
         if (s->npole) {
-            double dep[kMaxPoles];
+            // dispersive edges in (c, i, j, k) order: each task scans a contiguous run
+            // of (component, x plane) pairs, the runs are joined in order
+            const long nplanes = 3L * (nx + 1);
+            const long nt = std::max<long>(1, std::min<long>(64, nplanes));
+            std::vector<std::vector<size_t>> lists(nt);
+            std::vector<std::vector<int>> comps(nt);
+            // cells with any nonzero pole strength: an edge whose four cells have none
+            // is not dispersive (edge_debye would return 0), so it is skipped unread
+            std::vector<unsigned char> dcell(s->N, 0);
+            parallel_ranges((long)s->N, [&](long q0, long q1) {
+                for (long q = q0; q < q1; q++)
+                    for (int p = 0; p < s->npole; p++)
+                        if (s->cell_deps[p][q] != 0) {
+                            dcell[q] = 1;
+                            break;
+                        }
+            });
+            auto near_dispersive = [&](int c, int i, int j, int k) {
+                const int ua = (c + 1) % 3, va = (c + 2) % 3;
+                int ijk[3] = {i, j, k};
+                for (int du = -1; du <= 0; du++)
+                    for (int dv = -1; dv <= 0; dv++) {
+                        int cc[3] = {ijk[0], ijk[1], ijk[2]};
+                        cc[ua] += du;
+                        cc[va] += dv;
+                        if (cc[ua] < 0 || cc[ua] >= s->n[ua] || cc[va] < 0 || cc[va] >= s->n[va]) continue;
+                        if (dcell[IDX(s, cc[0], cc[1], cc[2])]) return true;
+                    }
+                return false;
+            };
+            parallel_ranges(nt, [&](long t0, long t1) {
+                double dep[kMaxPoles];
+                for (long t = t0; t < t1; t++)
+                    for (long r = nplanes * t / nt; r < nplanes * (t + 1) / nt; r++) {
+                        const int c = (int)(r / (nx + 1)), i = (int)(r % (nx + 1));
+                        for (int j = 0; j <= ny; j++)
+                            for (int k = 0; k <= nz; k++) {
+                                int ijk[3] = {i, j, k};
+                                if (ijk[c] >= s->n[c]) continue;
+                                size_t q = IDX(s, i, j, k);
+                                if (s->cb[c][q] == 0 || (isp && isp[c * s->N + q])) continue;
+                                if (!near_dispersive(c, i, j, k) || !edge_debye(s, c, i, j, k, dep)) continue;
+                                comps[t].push_back(c);
+                                lists[t].push_back(q);
+                            }
+                    }
+            });
             std::vector<size_t> list;
             std::vector<int> comp;
-            for (int c = 0; c < 3; c++)
-                for (int i = 0; i <= nx; i++)
-                    for (int j = 0; j <= ny; j++)
-                        for (int k = 0; k <= nz; k++) {
-                            int ijk[3] = {i, j, k};
-                            if (ijk[c] >= s->n[c]) continue;
-                            size_t q = IDX(s, i, j, k);
-                            if (s->cb[c][q] == 0 || (isp && isp[c * s->N + q])) continue;
-                            if (!edge_debye(s, c, i, j, k, dep)) continue;
-                            comp.push_back(c);
-                            list.push_back(q);
-                        }
+            for (long t = 0; t < nt; t++) {
+                list.insert(list.end(), lists[t].begin(), lists[t].end());
+                comp.insert(comp.end(), comps[t].begin(), comps[t].end());
+            }
             s->ndisp = (int)list.size();
             s->d_comp = comp;
             s->d_idx = list;
             s->d_cb.assign(s->ndisp, 0.0);
             s->d_beta.assign((size_t)s->ndisp * s->npole, 0.0);
-            for (int e = 0; e < s->ndisp; e++) {
-                int c = s->d_comp[e];
-                size_t q = s->d_idx[e];
-                int i = (int)(q / s->si), j = (int)((q % s->si) / s->sj), k = (int)(q % s->sj);
-                edge_debye(s, c, i, j, k, dep);
-                double er, sg;
-                int pec;
-                edge_material(s, c, i, j, k, &er, &sg, &pec);
-                double einst = er * CEM_EPS0;
-                for (int p = 0; p < s->npole; p++) {
-                    double tau = 1.0 / (2 * M_PI * s->pole_f[p]);
-                    double bt = CEM_EPS0 * dep[p] * s->dt / (tau * (1 + s->dt / (2 * tau)));
-                    s->d_beta[(size_t)p * s->ndisp + e] = bt;
-                    einst += 0.5 * bt;
+            parallel_ranges(s->ndisp, [&](long e0, long e1) {
+                double dep[kMaxPoles];
+                for (long e = e0; e < e1; e++) {
+                    int c = s->d_comp[e];
+                    size_t q = s->d_idx[e];
+                    int i = (int)(q / s->si), j = (int)((q % s->si) / s->sj), k = (int)(q % s->sj);
+                    edge_debye(s, c, i, j, k, dep);
+                    double er, sg;
+                    int pec;
+                    edge_material(s, c, i, j, k, &er, &sg, &pec);
+                    double einst = er * CEM_EPS0;
+                    for (int p = 0; p < s->npole; p++) {
+                        double tau = 1.0 / (2 * M_PI * s->pole_f[p]);
+                        double bt = CEM_EPS0 * dep[p] * s->dt / (tau * (1 + s->dt / (2 * tau)));
+                        s->d_beta[(size_t)p * s->ndisp + e] = bt;
+                        einst += 0.5 * bt;
+                    }
+                    double h = sg * s->dt / (2 * einst);
+                    s->ca[c][q] = (1 - h) / (1 + h);
+                    s->cb[c][q] = (s->dt / einst) / (1 + h);
+                    s->d_cb[e] = s->cb[c][q];
                 }
-                double h = sg * s->dt / (2 * einst);
-                s->ca[c][q] = (1 - h) / (1 + h);
-                s->cb[c][q] = (s->dt / einst) / (1 + h);
-                s->d_cb[e] = s->cb[c][q];
-            }
+            });
             for (int p = 0; p < s->npole; p++) {
                 double tau = 1.0 / (2 * M_PI * s->pole_f[p]);
                 s->d_k[p] = (1 - s->dt / (2 * tau)) / (1 + s->dt / (2 * tau));

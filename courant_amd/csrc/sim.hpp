@@ -7,9 +7,11 @@
 
 #include "courant.h"
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <memory>
+#include <thread>
 #include <vector>
 
 namespace cem {
@@ -131,6 +133,22 @@ struct cem_sim {
 };
 
 namespace cem {
+
+// fn(lo, hi) over [0, n) in contiguous ranges on up to 16 host threads; set-up
+// loops use it where every result is computed by the same code whichever
+// thread runs it (the plan stays bit-identical)
+template <class F>
+void parallel_ranges(long n, F fn) {
+    long nt = std::min<long>(16, std::max(1u, std::thread::hardware_concurrency()));
+    nt = std::min(nt, n);
+    if (nt <= 1) {
+        fn(0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (long t = 0; t < nt; t++) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+    for (auto &x : th) x.join();
+}
 
 inline size_t IDX(const cem_sim *s, int i, int j, int k) {
     return (size_t)i * s->si + (size_t)j * s->sj + (size_t)k;
