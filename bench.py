@@ -92,7 +92,7 @@ def cpu_baseline(target_s=12.0):
             "energy_rel_err_gpu_vs_f64": float(f"{abs(w_gpu - w_ref) / w_ref:.3e}"), "energy_steps": 5 + n}
 
 
-def _pmc_pass(counter, args_tail, tmp, tag):
+def _pmc_pass(counter, args_tail, tmp, tag, env=None):
     """One rocprofv3 counter pass over a child run of this script; mean counter value per
     dispatch, by kernel name (template arguments and parameter lists dropped)."""
     exe = shutil.which("rocprofv3")
@@ -101,7 +101,7 @@ def _pmc_pass(counter, args_tail, tmp, tag):
     d = os.path.join(tmp, f"{tag}_{counter}")
     cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc",
            "--", sys.executable, os.path.abspath(__file__)] + args_tail
-    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=tmp)
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=tmp, env=env)
     path = os.path.join(d, "pmc_counter_collection.csv")
     if r.returncode != 0 or not os.path.exists(path):
         return None
@@ -112,16 +112,18 @@ def _pmc_pass(counter, args_tail, tmp, tag):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def pmc_traffic(workload):
+def pmc_traffic(workload, shape):
     """HBM bytes per dispatch of each stepper kernel: FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch, one counter per rocprofv3 run), each scaled by what the same counter reads for
-    the plain copy kernel moving 5e8 B in and 5e8 B out per dispatch."""
+    the plain copy kernel moving 5e8 B in and 5e8 B out per dispatch.  The counted child runs
+    use the launch shape the timed run was tuned to."""
+    env = dict(os.environ, COURANT_ISEG_H=str(shape[0]), COURANT_ISEG_E=str(shape[1]))
     tmp = tempfile.mkdtemp(prefix="courant_pmc_", dir="/tmp")
     try:
         out = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             run = _pmc_pass(counter, ["--kernel-pass", "--workload", workload, "--steps", "8", "--warmup", "2"],
-                            tmp, "step")
+                            tmp, "step", env)
             cal = _pmc_pass(counter, ["--copy-only", "0.5"], tmp, "copy")
             if not run or not cal or not cal.get("cem::k_copy"):
                 return None
@@ -304,7 +306,7 @@ def main():
     }
     sim.close()
     if not args.no_pmc:
-        tr = pmc_traffic(args.workload)
+        tr = pmc_traffic(args.workload, shape)
         if tr:
             kname = {"H": "cem::k_update_h", "E": "cem::k_update_e"}[dom]
             out["roofline"]["traffic"] = round(tr["kernels"].get(kname, 0.0))
