@@ -59,7 +59,28 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-Wl,-Bsymbolic", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_napi()
     return LIB
+
+
+NAPI_DIR = os.path.join(HERE, "napi")
+NAPI_ADDON = os.path.join(HERE, "courant_napi.node")
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
+
+
+def build_napi():
+    """The Node addon (courant_napi.node, next to libcourant.so): wrappers generated from
+    include/courant.h, compiled against the system's node_api.h; skipped where Node's
+    headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    inc = os.path.join(BUILD, "courant_napi_gen.inc")
+    _run([sys.executable, os.path.join(NAPI_DIR, "gen_addon.py"), os.path.join(INCLUDE, "courant.h"), inc])
+    tmp = NAPI_ADDON + ".tmp"
+    _run([CXX, "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-I" + NODE_INCLUDE, "-I" + INCLUDE, "-I" + BUILD,
+          os.path.join(NAPI_DIR, "courant_napi.cpp"), "-L" + HERE, "-lcourant", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, NAPI_ADDON)
+    return NAPI_ADDON
 
 
 if __name__ == "__main__":

@@ -1,0 +1,93 @@
+"""The Node addon (courant_amd/courant_napi.node): the courant.h ABI presented as the
+wasm-shaped `exports` object web/src/solver/courant.ts:19-85 reads, driven by
+tests/napi/harness.js the way that wrapper drives the wasm module.
+
+  * CPU: every function of include/courant.h is exported, with malloc / free and a
+    memory.buffer arena; strings, plan arrays, error returns and the allocator behave
+    as through the C ABI (the same values as the ctypes path, bit for bit)
+  * GPU: a lumped-port line stepped in chunks of 500 through the addon, with a field
+    written through a held view before stepping and read through it after, gives
+    bit-identical spectra, energy and field values to the same calls made through ctypes
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from courant_amd import lib as L
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+ADDON = os.path.join(ROOT, "courant_amd", "courant_napi.node")
+HARNESS = os.path.join(HERE, "napi", "harness.js")
+NODE = shutil.which("node")
+
+needs_addon = pytest.mark.skipif(not NODE or not os.path.exists(ADDON), reason="node or courant_napi.node missing")
+
+
+def run_harness(mode):
+    r = subprocess.run([NODE, HARNESS, mode, ADDON], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def build_line(cem):
+    """The harness's line, through the ctypes binding (same calls, same order)."""
+    x = [i * 1e-4 for i in range(61)]
+    y = [j * 1e-4 for j in range(41)]
+    z = [k * 5e-5 for k in range(31)]
+    s = cem.create_sim(x, y, z)
+    for f in range(6):
+        s.set_boundary(f, 1, 6)
+    s.material_box(-1, 1, -1, 1, z[6], z[12], 4.3, 0)
+    s.pec_sheet(2, z[6], -1, 1, -1, 1)
+    s.pec_sheet(2, z[12], x[10], x[50], y[18], y[22])
+    p1 = s.add_lumped_port(2, 10, 20, 6, 12, -1, 50)
+    p2 = s.add_lumped_port(2, 50, 20, 6, 12, -1, 50)
+    s.set_gaussian(p1, 1, 60e-12, 15e-12)
+    s.set_frequencies([1e9 * (i + 1) for i in range(40)])
+    s.finalize(0.99)
+    return s, p1, p2
+
+
+def interleave(z):
+    return np.stack([z.real, z.imag], -1).ravel()
+
+
+@needs_addon
+def test_addon_exports_the_whole_abi(cem):
+    from test_abi import declared
+    out = run_harness("exports")
+    assert out["names"] == declared(os.path.join(ROOT, "include", "courant.h"))
+    assert len(out["names"]) == 91 and out["malloc"] and out["arena_bytes"] >= 16 << 20
+    assert out["version"] == cem.version()
+    s, _, _ = build_line(cem)
+    assert out["dt"] == s.dt and out["nfreq"] == s.nf
+    cb = np.ctypeslib.as_array(cem.lib.cem_plan_array(s.h, 1, 2), shape=(s.N,))
+    assert out["cb_sum"] == sum(cb.tolist())
+    ide = np.ctypeslib.as_array(cem.lib.cem_plan_array(s.h, 2, 1), shape=(41,))
+    assert out["ide_y"] == ide.tolist()
+    assert out["bad_port"] == -1 and out["error"] == "already finalized"
+    assert out["alloc"] == {"distinct": True, "reused": True}
+
+
+@needs_addon
+@pytest.mark.gpu
+def test_addon_line_matches_the_ctypes_path(cem, gpu_available):
+    print("\n[gpu case] node addon: lumped line, 2000 steps", flush=True)
+    out = run_harness("line")
+    s, p1, p2 = build_line(cem)
+    f = s.field(2).reshape(-1)
+    f[s.index(30, 10, 9)] = 1e-3
+    for _ in range(4):
+        s.step(500)
+    assert out["steps"] == s.steps == 2000
+    assert out["energy"] == s.energy()
+    for key, got in (("v1", s.port_v(p1)), ("i1", s.port_i(p1)), ("v2", s.port_v(p2)), ("i2", s.port_i(p2))):
+        assert np.array_equal(np.array(out[key]), interleave(got)), key
+    assert out["ez_probe"] == s.field(2).reshape(-1)[s.index(30, 20, 9)]
+    assert np.max(np.abs(out["v2"])) > 0
