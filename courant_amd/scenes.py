@@ -118,7 +118,7 @@ GCPW_DEFAULTS = dict(
     ports="lumped", fmax=40e9, df=0.1e9, dc=True, cfl=0.99,
     lossy=False, tcu=0.0, mask=0.0, mask_er=3.5, ds=(4.1, 0.02, 1e9), npole=6,
     sigma_cu=5.8e7, rough=2, rough_p1=0.5e-6, rough_p2=2.0 / (4 * math.pi * 0.25e-12),
-    feed=4.0e-3, mon=2.5e-3, neff0=1.65, probes=True,
+    feed=4.0e-3, mon=2.5e-3, neff0=1.65, probes=True, excite=1,
 )
 
 # cfg1: lumped-port GCPW on 120x80x60 with CPML on all six faces (ground sheet above a bottom air layer)
@@ -220,7 +220,7 @@ def gcpw(cem, **over):
     if not mode:
         p1 = s.add_lumped_port(L.Z, ip1, j0, kg, kh, -1, 50.0)
         p2 = s.add_lumped_port(L.Z, ip2, j0, kg, kh, -1, 50.0)
-        s.set_gaussian(p1, 1.0, 4.5 * tau, tau)
+        s.set_gaussian(p1 if p["excite"] == 1 else p2, 1.0, 4.5 * tau, tau)
     else:
         jl, jr = nearest(Y, -(a + b) / 2), nearest(Y, (a + b) / 2)
         mon = int(round(p["mon"] / p["dx"]))
@@ -229,7 +229,7 @@ def gcpw(cem, **over):
         s1, s2 = npml + 2, nx - npml - 2
         p1 = s.add_mode_port(dir=1, i_src=s1, i_mon=s1 + mon, x_ref=X[ip1], **common)
         p2 = s.add_mode_port(dir=-1, i_src=s2, i_mon=s2 - mon - 1, x_ref=X[ip2], **common)
-        s.set_mode_excitation(p1, 4.5 * tau, tau)
+        s.set_mode_excitation(p1 if p["excite"] == 1 else p2, 4.5 * tau, tau)
     info["ports"] = (p1, p2)
     nf = int(round(fmax / df))
     if p["dc"]:

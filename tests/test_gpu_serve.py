@@ -5,6 +5,9 @@
     and |dS| <= 1e-4 everywhere
   * every board's S matrix is bit-identical alone and in a concurrent mix (the kernels
     have no cross-simulation state and no order-dependent reductions)
+  * a full mode-port 2-port (both excitations, renormalised to 50 ohm on the GPU) against
+    the f64 reference core: S21, S12 within 1e-4 dB / 1e-3 deg, |dS11|, |dS22| <= 1e-4, and
+    its Touchstone file against the core's under the "gpu" rule of tools/compare_s2p.py
 """
 import os
 
@@ -64,3 +67,29 @@ def test_results_identical_alone_and_in_the_mix(cem, gpu_available):
     for x, y in zip(alone[::-1] + alone, served):
         assert np.array_equal(x, y)
     print("served %d boards in %.2f s, p50 %.3f s, max %.3f s" % (len(lat), wall, np.median(lat), lat.max()))
+
+
+def test_full_two_port_touchstone_matches_reference(cem, ref, gpu_available):
+    print_case("full 2-port, mode ports, lossy gcpw, 3000 steps per excitation")
+    from courant_amd import scenes as S
+    from courant_amd import touchstone, twoport
+    from test_gpu_parity import MODE_MINI
+    a = twoport.run(ref, lambda e: S.gcpw(ref, **dict(MODE_MINI, excite=e)), 3000)
+    b = twoport.run(cem, lambda e: S.gcpw(cem, **dict(MODE_MINI, excite=e)), 3000)
+    # transmission (S21, S12) in dB and degrees, reflection (S11, S22) as |dS|: the
+    # tolerances of the rest of the suite
+    ta, tb = a["S"][:, [1, 0], [0, 1]], b["S"][:, [1, 0], [0, 1]]
+    ddb = np.abs(20 * np.log10(np.abs(tb)) - 20 * np.log10(np.abs(ta)))
+    ddeg = np.abs(np.angle(tb / ta, deg=True))
+    drefl = np.abs(b["S"][:, [0, 1], [0, 1]] - a["S"][:, [0, 1], [0, 1]])
+    print("max |S21|, |S12| err %.2e dB, %.2e deg; |dS11|, |dS22| %.2e" % (ddb.max(), ddeg.max(), drefl.max()))
+    assert ddb.max() <= 1e-4 and ddeg.max() <= 1e-3 and drefl.max() <= 1e-4
+    # the line is mirror-symmetric: reciprocal and symmetric up to rounding
+    assert np.max(np.abs(b["S"][:, 0, 1] - b["S"][:, 1, 0])) <= 1e-5 * np.max(np.abs(b["S"][:, 1, 0]))
+    assert np.max(np.abs(b["S"][:, 0, 0] - b["S"][:, 1, 1])) <= 1e-5 * np.max(np.abs(b["S"][:, 0, 0]))
+    fa = touchstone.read_s2p(twoport.to_touchstone(a, "mode-port gcpw", ref.version()))
+    fb = touchstone.read_s2p(twoport.to_touchstone(b, "mode-port gcpw", cem.version()))
+    res = touchstone.compare(fb, fa, "gpu")
+    for c in res["checks"]:
+        print("  %s: %.3g %s" % (c["name"], c["value"], c["unit"]))
+    assert res["pass"]
