@@ -547,3 +547,16 @@ def test_four_layer_board_matches_reference(cem, ref, gpu_available):
     for p in (0, 1):
         assert rel(b.port_v(p), a.port_v(p)) < SPEC_TOL
         assert rel(b.port_i(p), a.port_i(p)) < SPEC_TOL
+
+
+def test_gpu_source_spectrum_is_the_gaussian_transform(cem, gpu_available):
+    """The FP64 running DFT on the GPU against the closed-form transform of the Gaussian drive
+    (see test_oracle.py), 1e-8 of the spectrum's peak."""
+    from test_oracle import SRC_MINI, gaussian_transform, source_steps
+    print_case("source spectrum closed form")
+    s, info = S.gcpw(cem, **SRC_MINI)
+    s.step(source_steps(s))
+    want = gaussian_transform(s, info["freqs"])
+    err = np.max(np.abs(s.source_spectrum() - want)) / np.max(np.abs(want))
+    print("max error %.2e of the peak" % err)
+    assert err <= 1e-8

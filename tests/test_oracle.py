@@ -284,3 +284,46 @@ def test_fp16_storage_misses_the_sparameter_tolerance(ref):
     print("max |S21| error: FP32 storage %.2e dB, FP16 storage %.2e dB" % (err["f32"], err["f16s"]))
     assert err["f32"] <= 1e-4
     assert err["f16s"] > 1e-1
+
+
+# ---------------------------------------------------------------------------
+# DFT: closed form of the Gaussian drive's transform
+# ---------------------------------------------------------------------------
+SRC_MINI = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=3e-3, yair=0.4e-3, zair=0.6e-3, zbelow=0.2e-3,
+                pml_below=True, df=0.5e9, probes=False)
+
+
+def gaussian_transform(sim, freqs):
+    """amp tau sqrt(pi) exp(-(w tau / 2)^2) exp(-j w t0): the continuous transform of the
+    drive amp exp(-((t - t0) / tau)^2) (sim.c:834-838)."""
+    amp, t0, tau = sim.excitation_lumped
+    w = 2 * np.pi * np.asarray(freqs)
+    return amp * tau * np.sqrt(np.pi) * np.exp(-(w * tau / 2) ** 2) * np.exp(-1j * w * t0)
+
+
+def source_steps(sim):
+    amp, t0, tau = sim.excitation_lumped
+    return int(np.ceil((t0 + 7 * tau) / sim.dt))
+
+
+def test_source_spectrum_is_the_gaussian_transform(ref):
+    """The running DFT of the source (phasors resynchronised every 1024 steps and rotated in
+    between, sim.c:840-870) against the closed form, once the pulse has ended: the sum over
+    t_n = (n + 1/2) dt misses only e^-(t0/tau)^2 ~ 2e-9 of the pulse before t = 0 and aliases
+    at exp(-(pi tau / dt)^2).  Reference core and oracle, 1e-8 of the spectrum's peak."""
+    s, info = S.gcpw(ref, **SRC_MINI)
+    o = OracleSim(s)
+    n = source_steps(s)
+    s.step(n)
+    o.step(n)
+    want = gaussian_transform(s, info["freqs"])
+    peak = np.max(np.abs(want))
+    for got in (s.source_spectrum(), o.source_spectrum()):
+        assert np.max(np.abs(got - want)) <= 1e-8 * peak
+    # a planted mistake, the drive sampled at t_H but rotated by the E phasor (t_E = t_H +
+    # dt/2), is a phase error of w dt / 2 and fails the check by orders of magnitude
+    tH = (np.arange(n) + 0.5) * s.dt
+    amp, t0, tau = s.excitation_lumped
+    w = 2 * np.pi * info["freqs"]
+    wrong = (amp * np.exp(-((tH - t0) / tau) ** 2) * s.dt) @ np.exp(-1j * np.outer(tH + 0.5 * s.dt, w))
+    assert np.max(np.abs(wrong - want)) > 1e-4 * peak

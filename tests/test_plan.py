@@ -167,3 +167,25 @@ def test_cfg_scene_sizes():
         p.update(cfg)
         X, Y, Z, _ = S.gcpw_mesh(p)
         assert (len(X) - 1, len(Y) - 1, len(Z) - 1) == cells
+
+
+def test_debye_material_without_poles_is_the_plain_dielectric(cem, ref):
+    """A Debye material whose pole strengths are all zero is the plain dielectric of its
+    eps_inf: no dispersive edges and the same coefficients, bit for bit (engine and core)."""
+    x, y, z = np.linspace(0, 2e-3, 21), np.linspace(0, 1.5e-3, 16), np.linspace(0, 1e-3, 11)
+    for lib in (cem, ref):
+        plans = []
+        for debye in (False, True):
+            s = lib.create_sim(x, y, z)
+            if debye:
+                s.set_debye_poles(np.array([1e9, 5e9]))
+                s.material_box_debye(0.4e-3, 1.6e-3, -1, 1, 0.2e-3, 0.6e-3, 4.3, 0.01, np.zeros(2))
+            else:
+                s.material_box(0.4e-3, 1.6e-3, -1, 1, 0.2e-3, 0.6e-3, 4.3, 0.01)
+            s.finalize(0.99)
+            plans.append(s.plan())
+        a, b = plans
+        assert b["ndisp"] == 0
+        for key in ("ca", "cb"):
+            for c in range(3):
+                assert np.array_equal(a[key][c], b[key][c])
