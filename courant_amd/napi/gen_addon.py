@@ -76,44 +76,44 @@ def wrapper(ret, name, params):
     L = [f"napi_value js_{name}(napi_env e, napi_callback_info ci) {{", "    Call a(e, ci);"]
     if sim_first:
         L.append("    cem_sim *s0 = a.S(0);")
-        L.append("    sync_in(s0);")
+        L.append("    a.in->sync_in(s0);")
     if any(t in ("cem_mode2d *", "const cem_mode2d *") for t in types) and name in RETURNS:
         L.append("    const ModeH *mh = a.MH(0);")
         L.append("    if (!mh) return ret_num(e, 0);")
     call = f"{name}({args})"
     r = ret.replace(" ", "")
     if name == "cem_destroy":
-        L += ["    drop_mirrors(s0);", f"    {call};", "    a.release_sim(0);", "    return ret_void(e);"]
+        L += ["    a.in->drop_mirrors(s0);", f"    {call};", "    a.release_sim(0);", "    return ret_void(e);"]
     elif name == "cem_mode2d_destroy":
-        L += ["    drop_mirrors(a.M(0));", f"    {call};", "    a.release_mode(0);", "    return ret_void(e);"]
+        L += ["    a.in->drop_mirrors(a.M(0));", f"    {call};", "    a.release_mode(0);", "    return ret_void(e);"]
     elif name in STRINGS:
-        L.append(f"    return ret_num(e, mirror_string({STRINGS[name]}, {call}));")
+        L.append(f"    return ret_num(e, a.in->mirror_string({STRINGS[name]}, {call}));")
     elif name == "cem_create":
-        L.append(f"    return ret_num(e, new_handle_sim({call}));")
+        L.append(f"    return ret_num(e, a.in->new_sim({call}));")
     elif name == "cem_mode2d_create":
-        L.append(f"    return ret_num(e, new_handle_mode({call}, a.I(0), a.I(1)));")
+        L.append(f"    return ret_num(e, a.in->new_mode({call}, a.I(0), a.I(1)));")
     elif name == "cem_cross_section_x":
         L.append(f"    cem_mode2d *m = {call};")
-        L.append("    sync_out(s0);")
-        L.append("    return ret_num(e, new_handle_mode(m, a.I(3) - a.I(2), a.I(5) - a.I(4)));")
+        L.append("    a.in->sync_out(s0);")
+        L.append("    return ret_num(e, a.in->new_mode(m, a.I(3) - a.I(2), a.I(5) - a.I(4)));")
     elif name in RETURNS:
         n, kind, idx, wr = RETURNS[name]
         owner = "s0" if sim_first else "mh->m"
         L.append(f"    const double *p = {call};")
         if sim_first:
-            L.append("    sync_out(s0);")
+            L.append("    a.in->sync_out(s0);")
         L.append(f"    if (!p) return ret_num(e, 0);")
-        L.append(f"    return ret_num(e, mirror({owner}, {kind}, {idx}, p, ({n}) * sizeof(double), "
+        L.append(f"    return ret_num(e, a.in->mirror({owner}, {kind}, {idx}, p, ({n}) * sizeof(double), "
                  f"{'true' if wr else 'false'}));")
     elif r == "void":
         L.append(f"    {call};")
         if sim_first:
-            L.append("    sync_out(s0);")
+            L.append("    a.in->sync_out(s0);")
         L.append("    return ret_void(e);")
     elif r in ("int", "long", "double", "size_t"):
         L.append(f"    const double r = (double)({call});")
         if sim_first:
-            L.append("    sync_out(s0);")
+            L.append("    a.in->sync_out(s0);")
         L.append("    return ret_num(e, r);")
     else:
         raise ValueError(f"{name}: return type {ret}")

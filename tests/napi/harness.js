@@ -5,11 +5,14 @@
 //
 //   node harness.js exports <addon>   export list, strings and a finalized plan (no GPU)
 //   node harness.js line <addon>      a small lumped-port line stepped on the GPU
+//   node harness.js workers <addon>   the export check in two worker threads while the
+//                                     main thread holds a plan and data in its own arena
 //
 // Prints one JSON object.
 'use strict';
 
-const [mode, addonPath] = process.argv.slice(2);
+const { Worker, isMainThread, parentPort, workerData } = require('worker_threads');
+const [mode, addonPath] = isMainThread ? process.argv.slice(2) : [workerData.mode, workerData.addon];
 const ex = require(addonPath);
 
 function allocF64(values) {
@@ -54,8 +57,10 @@ function buildLine() {
   return { s, p1, p2 };
 }
 
-const out = {};
-if (mode === 'exports') {
+function exportsReport() {
+  const out = {};
+  out.first_block = ex.malloc(8);   // a fresh arena hands out its first block
+  ex.free(out.first_block);
   out.names = Object.keys(ex).filter((k) => k.startsWith('cem_')).sort();
   out.malloc = typeof ex.malloc === 'function' && typeof ex.free === 'function';
   out.arena_bytes = ex.memory.buffer.byteLength;
@@ -76,6 +81,44 @@ if (mode === 'exports') {
   out.alloc = { distinct: a !== b && a > 0 && b > 0, reused: c === a };
   ex.free(b); ex.free(c);
   ex.cem_destroy(s);
+  return out;
+}
+
+function emit(out) { process.stdout.write(JSON.stringify(out) + '\n'); }
+
+let out = {};
+if (!isMainThread) {
+  parentPort.postMessage(exportsReport());
+} else if (mode === 'exports') {
+  out = exportsReport();
+} else if (mode === 'workers') {
+  // the main thread's own simulation and data, held across the workers' lifetimes
+  const pattern = Array.from({ length: 4096 }, (_, i) => Math.sin(i) * 1e3);
+  const held = allocF64(pattern);
+  const { s } = buildLine();
+  const dt = ex.cem_dt(s);
+  const run = () => new Promise((resolve, reject) => {
+    const w = new Worker(__filename, { workerData: { mode: 'exports', addon: addonPath } });
+    w.once('message', resolve);
+    w.once('error', reject);
+  });
+  Promise.all([run(), run()]).then((reports) => {
+    const mine = exportsReport();
+    delete mine.first_block;   // the main arena holds `held`, so its first block is taken
+    const firsts = reports.map((r) => r.first_block);
+    reports.forEach((r) => delete r.first_block);
+    const same = reports.every((r) => JSON.stringify(r) === JSON.stringify(mine));
+    const now = readF64(held, pattern.length);
+    emit({
+      workers: reports.length,
+      workers_equal_main: same,
+      worker_first_blocks: firsts,
+      pattern_ok: now.every((v, i) => v === pattern[i]),
+      plan_ok: ex.cem_dt(s) === dt && ex.cem_nfreq(s) === 40,
+    });
+    ex.cem_destroy(s);
+    ex.free(held);
+  }, (err) => { process.stderr.write(String(err.stack || err) + '\n'); process.exit(1); });
 } else if (mode === 'line') {
   const { s, p1, p2 } = buildLine();
   // write-then-step through a held view of a library-owned field (courant.ts setField)
@@ -96,4 +139,4 @@ if (mode === 'exports') {
 } else {
   throw new Error(`unknown mode ${mode}`);
 }
-process.stdout.write(JSON.stringify(out) + '\n');
+if (isMainThread && mode !== 'workers') emit(out);

@@ -5,6 +5,10 @@ tests/napi/harness.js the way that wrapper drives the wasm module.
   * CPU: every function of include/courant.h is exported, with malloc / free and a
     memory.buffer arena; strings, plan arrays, error returns and the allocator behave
     as through the C ABI (the same values as the ctypes path, bit for bit)
+  * CPU: each Node environment gets its own instance: two worker threads that load the
+    addon while the main thread holds a plan and data in its arena each start from a
+    fresh arena, report what the main thread reports, and leave the main thread's data
+    and plan untouched
   * GPU: a lumped-port line stepped in chunks of 500 through the addon, with a field
     written through a held view before stepping and read through it after, gives
     bit-identical spectra, energy and field values to the same calls made through ctypes
@@ -73,6 +77,14 @@ def test_addon_exports_the_whole_abi(cem):
     assert out["ide_y"] == ide.tolist()
     assert out["bad_port"] == -1 and out["error"] == "already finalized"
     assert out["alloc"] == {"distinct": True, "reused": True}
+
+
+@needs_addon
+def test_addon_state_is_per_worker_thread():
+    out = run_harness("workers")
+    assert out["workers"] == 2 and out["workers_equal_main"]
+    assert out["worker_first_blocks"] == [16, 16]
+    assert out["pattern_ok"] and out["plan_ok"]
 
 
 @needs_addon
