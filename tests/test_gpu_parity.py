@@ -445,6 +445,7 @@ def test_graph_replay_matches_direct_launches(cem, gpu_available, monkeypatch, s
     a = build()
     monkeypatch.setenv("COURANT_GRAPH", "0")
     b = build()
+    b.energy()      # the switch is read when the sim first runs on the GPU
     monkeypatch.delenv("COURANT_GRAPH")
     nport = cem.lib.cem_plan_nport(a.h)
     for n in (200, 450, 37, 64, 700):
