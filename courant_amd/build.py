@@ -17,7 +17,7 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcourant.so")
 
-HOST_SOURCES = ["setup.cpp", "materials.cpp", "mode2d.cpp", "modeport.cpp"]
+HOST_SOURCES = ["setup.cpp", "materials.cpp", "mode2d.cpp", "modeport.cpp", "board.cpp"]
 DEVICE_SOURCES = ["engine.hip", "mode_gpu.hip"]
 
 CXX = os.environ.get("CXX", "g++")

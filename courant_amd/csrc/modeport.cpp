@@ -234,8 +234,8 @@ static int solve_all_ports(cem_sim *s) {
     for (auto &p : s->mport)
         if (!p.solved && p.xs) {
             reset_port(s, p);
-            jobs.push_back({&p, p.xs, 1});
-            jobs.push_back({&p, mode2d_clone(p.xs), -1});
+            jobs.push_back({&p, p.xs, 1, 0, {}});
+            jobs.push_back({&p, mode2d_clone(p.xs), -1, 0, {}});
         }
     std::vector<std::thread> th;
     for (auto &jb : jobs)

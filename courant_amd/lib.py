@@ -212,12 +212,12 @@ class Courant:
 class Sim:
     """One simulation handle; method names follow courant.ts's Sim."""
 
-    def __init__(self, cem, x, y, z):
+    def __init__(self, cem, x, y, z, handle=None):
         self.cem = cem
         self.lib = cem.lib
         self.x, self.y, self.z = _f64(x), _f64(y), _f64(z)
         self.n = (len(self.x) - 1, len(self.y) - 1, len(self.z) - 1)
-        h = self.lib.cem_create(self.n[0], self.n[1], self.n[2], _dp(self.x), _dp(self.y), _dp(self.z))
+        h = handle or self.lib.cem_create(self.n[0], self.n[1], self.n[2], _dp(self.x), _dp(self.y), _dp(self.z))
         if not h:
             raise CourantError(cem.last_error())
         self.h = h

@@ -30,13 +30,16 @@ def test_header_declares_91_entry_points():
 
 def test_library_exports_every_declared_symbol():
     lib = exported(L.DEFAULT_LIB)
-    missing = [n for n in declared("courant.h") + declared("courant_amd.h") if not hasattr(lib, n)]
+    missing = [n for n in declared("courant.h") + declared("courant_amd.h") + declared("courant_board.h")
+               if not hasattr(lib, n)]
     assert not missing, missing
 
 
 def test_binding_covers_header():
+    from courant_amd import board
     assert set(declared("courant.h")) == set(L.SIGNATURES)
     assert set(declared("courant_amd.h")) == set(L.AMD_SIGNATURES)
+    assert set(declared("courant_board.h")) == set(board.SIGNATURES)
 
 
 def test_constants_and_version(cem):
