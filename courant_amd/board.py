@@ -477,3 +477,87 @@ def build_board_sim(cem, board, ports, **opts):
     sim.excitation_mode = (4.5 * tau, tau)
     return {"sim": sim, "freqs": sim.freqs, "ports": list(ids), "cells": plan["cells"], "t_stop": t_stop.value,
             "mesh": m}
+
+
+# ---------------------------------------------------------------------------
+# board runs (web/tools/run-board.ts:53-85)
+# ---------------------------------------------------------------------------
+def run_excitation(cem, board, ports, excite, decay=1e-5, tmax=5e-9, steps=None, **opts):
+    """One run of a board with port `excite` driven: build, then step in chunks of 500
+    until the field energy is below `decay` of its peak after the drive or the run passes
+    `tmax` (run-board.ts:57-72), or exactly `steps` steps when given.  Returns the waves of
+    both ports ({"a", "b", "z0"} each), the frequencies and the step count."""
+    bs = build_board_sim(cem, board, ports, excite=excite, **opts)
+    sim = bs["sim"]
+    try:
+        if steps:
+            sim.step(int(steps))
+        else:
+            peak = 0.0
+            while True:
+                sim.step(500)
+                W = sim.energy()
+                peak = max(peak, W)
+                t = sim.steps * sim.dt
+                if (t > bs["t_stop"] and W < decay * peak) or t > tmax:
+                    break
+        waves = [sim.mode_port_waves(p) for p in bs["ports"][:2]]
+        return {"waves": waves, "freqs": bs["freqs"], "steps": sim.steps, "cells": bs["cells"]}
+    finally:
+        sim.close()
+
+
+def board_s_matrix(cem, runs):
+    """S (nf x 2 x 2) renormalised to 50 ohm from one run (port 1 driven: S22 = S11 and S12
+    = S21 assumed, a symmetric reciprocal line) or two (port 2 driven as well), each port
+    referred to its line impedance of the first run (run-board.ts:74-83); FP64 on the GPU."""
+    from . import twoport
+    w1, w2 = runs[0]["waves"]
+    nf = len(runs[0]["freqs"])
+    S = np.empty((nf, 2, 2), complex)
+    S[:, 0, 0] = w1["b"] / w1["a"]
+    S[:, 1, 0] = w2["b"] / w1["a"]
+    if len(runs) > 1:
+        v1, v2 = runs[1]["waves"]
+        S[:, 0, 1] = v1["b"] / v2["a"]
+        S[:, 1, 1] = v2["b"] / v2["a"]
+    else:
+        S[:, 0, 1] = S[:, 1, 0]
+        S[:, 1, 1] = S[:, 0, 0]
+    return twoport.renormalise(cem, S, w1["z0"], w2["z0"], 50.0)
+
+
+def run_board(cem, board, net=None, planes=(0.0, 0.0), hmin=20e-6, fmax=40e9, df=0.5e9, lossy=False, sheets=False,
+              full2=False, stackup=None, decay=1e-5, tmax=5e-9, window=None):
+    """The command-line board run of the reference (run-board.ts:36-85) as a function: the
+    two-port net between the left and right port planes (m into the board), both planes
+    extruded into feeds, mode ports, one run (or two with full2), S renormalised to 50 ohm.
+    Defaults are the tool's: hmin 20 um, fmax 40 GHz, df 0.5 GHz, copper of its stackup
+    thickness (sheets=True: zero-thickness), energy decay 1e-5, at most 5 ns per run.
+    Returns {"f", "S", "z0", "net", "planes", "cells", "steps", "wall", "full"}."""
+    import time
+    if stackup is not None:
+        board = board.apply_stackup(stackup)
+    if net is None:
+        nets = port_nets(board, planes[0], planes[1])
+        if not nets:
+            raise ValueError("no net crosses both port planes")
+        net = nets[0]
+    ports = [{"net": net, "edge": "xlo", "offset": planes[0]}, {"net": net, "edge": "xhi", "offset": planes[1]}]
+    opts = dict(fmax=fmax, df=df, hmin=hmin, lossy=lossy, thick_copper=not sheets, y_window=window or 0.0)
+    t0 = time.perf_counter()
+    runs = [run_excitation(cem, board, ports, e, decay=decay, tmax=tmax, **opts) for e in range(2 if full2 else 1)]
+    S = board_s_matrix(cem, runs)
+    return {"f": runs[0]["freqs"], "S": S, "z0": runs[0]["waves"][0]["z0"], "net": net, "planes": tuple(planes),
+            "cells": runs[0]["cells"], "steps": sum(r["steps"] for r in runs), "wall": time.perf_counter() - t0,
+            "full": bool(full2)}
+
+
+def board_touchstone(result, name, version):
+    """Touchstone text of a run_board result, titled as the reference's tool titles it."""
+    from . import touchstone
+    def js_str(v):          # Number -> String as JavaScript prints it (shortest round trip)
+        return str(int(v)) if float(v).is_integer() else repr(float(v))
+
+    title = "%s, net %s, planes %s mm" % (name, result["net"], "/".join(js_str(p * 1e3) for p in result["planes"]))
+    return touchstone.write_s2p(result["f"], result["S"], title, version, full=result["full"], renormalised=True)

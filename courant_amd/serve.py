@@ -76,6 +76,25 @@ def cells(b):
     return b["nx"] * b["ny"] * b["nz"]
 
 
+def board_request(board, net=None, planes=(0.0, 0.0), hmin=20e-6, fmax=40e9, df=0.5e9, lossy=False, sheets=False,
+                  full2=True, decay=1e-5, tmax=5e-9, window=None, steps=None):
+    """A served request for an imported board (courant_amd.board.Board): the two-port net
+    between the port planes, as courant_amd.board.run_board runs it.  Its mesh is planned at
+    once, for the admission estimate."""
+    from . import board as BD
+    if net is None:
+        nets = BD.port_nets(board, planes[0], planes[1])
+        if not nets:
+            raise ValueError("no net crosses both port planes")
+        net = nets[0]
+    ports = [{"net": net, "edge": "xlo", "offset": planes[0]}, {"net": net, "edge": "xhi", "offset": planes[1]}]
+    opts = dict(fmax=fmax, df=df, hmin=hmin, lossy=lossy, thick_copper=not sheets, y_window=window or 0.0)
+    plan = BD.plan_board(board, ports, **opts)
+    nx, ny, nz = plan["cells"]
+    return {"kind": "board", "board": board, "ports": ports, "opts": opts, "nport": 2 if full2 else 1,
+            "decay": decay, "tmax": tmax, "steps": steps, "nx": nx, "ny": ny, "nz": nz, "lossy": lossy}
+
+
 def estimate_bytes(b):
     """Device footprint of one excitation, the admission test of the request loop.  Grows
     from the reference's cell estimate (web/src/mesh/mesher.ts:143-156) by what it leaves
@@ -87,9 +106,15 @@ def estimate_bytes(b):
     nx, ny, nz = b["nx"], b["ny"], b["nz"]
     n = (nx + 1) * (ny + 1) * (nz + 1)
     psi = 2 * NPML * 8 * 2 * ((nx + 1) * (ny + 1) + (ny + 1) * (nz + 1) + (nx + 1) * (nz + 1))
+    lossy = b.get("lossy", True)
+    if b.get("kind") == "board":
+        # an imported board: Debye states on up to every edge (8 poles), SIBC on up to
+        # three edges per point of every z plane that holds copper, bounded by the grid
+        ndisp = 3 * n if lossy else 0
+        nsibc = min(3 * n, 3 * (nx + 1) * (ny + 1) * 8) if lossy else 0
+        return 27 * n + psi + ndisp * (4 * 8 + 16) + nsibc * (12 * 4 + 36) + (16 << 20)
     ndisp = 3 * (nx + 1) * (ny + 1) * (b["h_cells"] + 1)
     nsibc = 3 * (nx + 1) * (ny + 1)
-    lossy = b.get("lossy", True)
     return (27 * n + psi + (ndisp * (4 * b["npole"] + 16) if lossy else 0)
             + (nsibc * (12 * 4 + 36) if lossy else 0) + (16 << 20))
 
@@ -152,7 +177,13 @@ def steps_for(b, dt):
 
 
 def solve(cem, b, excite):
-    """One right-hand side: build, step, return (V, I) port spectra (2 x nf each) and steps."""
+    """One right-hand side: build, step, return (V, I) port spectra (2 x nf each) and steps;
+    for an imported board, the run of courant_amd.board.run_excitation and its steps."""
+    if b.get("kind") == "board":
+        from . import board as BD
+        r = BD.run_excitation(cem, b["board"], b["ports"], excite, decay=b["decay"], tmax=b["tmax"],
+                              steps=b.get("steps"), **b["opts"])
+        return r, r["steps"]
     s, ports = build(cem, b, excite)
     try:
         n = steps_for(b, s.dt)
@@ -184,6 +215,15 @@ def s_matrix(vis, z0=50.0):
     return B @ np.linalg.inv(A)
 
 
+def assemble(cem, b, results):
+    """S of board b from its excitations' results: B A^-1 from the lumped ports' spectra,
+    or the mode-port wave ratios renormalised to 50 ohm for an imported board."""
+    if b.get("kind") == "board":
+        from . import board as BD
+        return BD.board_s_matrix(cem, results)
+    return s_matrix(results)
+
+
 def jobs_of(boards):
     return [(k, b, e) for k, b in enumerate(boards) for e in excitations(b)]
 
@@ -204,7 +244,7 @@ def run_batch(cem, boards, workers):
     for (k, b, _), (vi, n) in zip(jobs, res):
         per[k].append(vi)
         upd += cells(b) * n
-    return [s_matrix(p) for p in per], wall, upd
+    return [assemble(cem, b, p) for b, p in zip(boards, per)], wall, upd
 
 
 class Server:
@@ -263,7 +303,7 @@ class Server:
             if not done:
                 res = [f.result() for f in futs]
                 vis = [r[0] for r in res]
-                done["r"] = (s_matrix(vis), max(r[1] for r in res) - t_arr)
+                done["r"] = (assemble(self.cem, b, vis), max(r[1] for r in res) - t_arr)
             return done["r"]
 
         return result

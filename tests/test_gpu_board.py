@@ -16,6 +16,10 @@ the reference's own tests of that path:
     (docs/design/0008-mesher-and-import.md:64-67) at the precision it quotes them
   * a lossy board (Djordjevic-Sarkar dielectric, smooth copper loss) builds and runs: its
     |S21| falls with frequency and stays passive
+  * the reference's board run (web/tools/run-board.ts:53-85) as courant_amd.board.run_board:
+    both ports excited, S renormalised to 50 ohm, reciprocal and symmetric on the symmetric
+    microstrip, its Touchstone file read back to the written digits; the same board served
+    through courant_amd.serve (two concurrent copies) gives bit-identical S
 """
 import math
 import os
@@ -24,6 +28,7 @@ import numpy as np
 import pytest
 
 from courant_amd import board as B
+from courant_amd import serve, touchstone
 
 pytestmark = pytest.mark.gpu
 
@@ -197,3 +202,39 @@ def test_lossy_board_runs(cem, gpu_available):
         assert s21[38] < s21[3] - 0.05                                # dielectric and copper loss grow with f
     finally:
         sim.close()
+
+
+RUN = dict(hmin=40e-6, df=1e9, sheets=True, full2=True, tmax=3e-9)
+
+
+def test_board_run_full_two_port_and_touchstone(cem, gpu_available):
+    print_case("board run: microstrip fixture, both ports excited, Touchstone")
+    r = B.run_board(cem, fixture(cem, "msl"), **RUN)
+    S = r["S"]
+    recip = np.max(np.abs(S[:, 0, 1] - S[:, 1, 0]))
+    sym = np.max(np.abs(S[:, 0, 0] - S[:, 1, 1]))
+    print("mesh %s, %d steps in %.2f s; |S12 - S21| %.2e, |S11 - S22| %.2e, |S11| max %.1f dB" %
+          (r["cells"], r["steps"], r["wall"], recip, sym, 20 * np.log10(np.abs(S[1:, 0, 0]).max())))
+    assert r["net"] == "RF" and r["full"] and len(r["f"]) == 40
+    assert recip < 1e-3 * np.max(np.abs(S[:, 1, 0])) and sym < 1e-3
+    # lossless line: power balance to the accuracy of the fixture's own transmission
+    # (||S21| - 1| <= 0.012 dB, radiation from the open line included)
+    power = np.abs(S[:, 0, 0]) ** 2 + np.abs(S[:, 1, 0]) ** 2
+    print("max |S| %.5f, power balance %.2e" % (np.abs(S).max(), np.max(np.abs(power - 1))))
+    assert np.abs(S).max() < 1.002 and np.max(np.abs(power - 1)) < 0.01
+    text = B.board_touchstone(r, "msl.kicad_pcb", cem.version())
+    assert text.splitlines()[0].endswith("msl.kicad_pcb, net RF, planes 0/0 mm")
+    assert "both ports excited" in text.splitlines()[1]
+    f, St, R = touchstone.read_s2p(text)
+    assert R == 50 and np.array_equal(f, r["f"])
+    assert np.max(np.abs(St - S)) <= 1e-9 * max(1.0, np.max(np.abs(S)))
+
+
+def test_served_board_matches_the_board_run(cem, gpu_available):
+    print_case("served board request (two concurrent copies) vs the board run")
+    b = fixture(cem, "msl")
+    req = serve.board_request(b, hmin=RUN["hmin"], df=RUN["df"], sheets=True, full2=True, tmax=RUN["tmax"])
+    S, wall, upd = serve.run_batch(cem, [req, req], 2)
+    r = B.run_board(cem, b, **RUN)
+    print("served 2 boards (4 runs) in %.2f s, %.2f Gcell-updates/s" % (wall, upd / wall / 1e9))
+    assert np.array_equal(S[0], r["S"]) and np.array_equal(S[1], r["S"])

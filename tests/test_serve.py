@@ -134,3 +134,55 @@ def test_served_board_fixture_reproduces_on_the_reference(ref):
     # no gain at any frequency
     np.testing.assert_allclose(g["S"][:, 0, 1], g["S"][:, 1, 0], rtol=1e-9)
     assert np.all(np.abs(g["S"]) < 1.0 + 1e-6)
+
+
+def board_fixture(name):
+    from courant_amd import board as B
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "boards", name + ".kicad_pcb")
+    return B.import_kicad(open(path).read())
+
+
+def test_board_request_plans_its_mesh():
+    from courant_amd import board as B
+    b = board_fixture("msl")
+    r = serve.board_request(b, hmin=40e-6, df=1e9, sheets=True)
+    p = B.plan_board(b, r["ports"], **r["opts"])
+    assert (r["nx"], r["ny"], r["nz"]) == p["cells"] and serve.cells(r) == np.prod(p["cells"])
+    assert r["ports"][0]["net"] == "RF" and serve.excitations(r) == (0, 1)
+    lossy = serve.board_request(b, hmin=40e-6, df=1e9, sheets=True, lossy=True)
+    n = (r["nx"] + 1) * (r["ny"] + 1) * (r["nz"] + 1)
+    assert 27 * n < serve.estimate_bytes(r) < serve.estimate_bytes(lossy)
+    assert serve.excitations(serve.board_request(b, hmin=40e-6, df=1e9, full2=False)) == (0,)
+    with pytest.raises(ValueError, match="no net crosses"):     # the launch pads stop the line at the edges
+        serve.board_request(board_fixture("launch"))
+
+
+def test_board_s_matrix_assembly(ref):
+    """Wave ratios of one or two runs, renormalised to 50 ohm through the reference core's
+    cem_s2_renormalize: the identity when both lines are 50 ohm, the symmetric fill of a
+    single run (S22 = S11, S12 = S21) as the reference's board tool assumes, and a matched
+    25 ohm line seen from 50 ohm."""
+    from courant_amd import board as B
+    nf = 5
+    rng = np.random.default_rng(7)
+    a1, a2 = rng.normal(size=nf) + 1j * rng.normal(size=nf), rng.normal(size=nf) + 1j * rng.normal(size=nf)
+    S = 0.3 * (rng.normal(size=(nf, 2, 2)) + 1j * rng.normal(size=(nf, 2, 2)))
+    f = np.arange(1, nf + 1) * 1e9
+    z50 = np.full(nf, 50.0)
+    run1 = {"freqs": f, "waves": [{"a": a1, "b": S[:, 0, 0] * a1, "z0": z50}, {"a": 0 * a1, "b": S[:, 1, 0] * a1, "z0": z50}]}
+    run2 = {"freqs": f, "waves": [{"a": 0 * a2, "b": S[:, 0, 1] * a2, "z0": z50}, {"a": a2, "b": S[:, 1, 1] * a2, "z0": z50}]}
+    assert np.allclose(B.board_s_matrix(ref, [run1, run2]), S, rtol=0, atol=1e-13)
+    one = B.board_s_matrix(ref, [run1])
+    for got, want in ((one[:, 0, 0], S[:, 0, 0]), (one[:, 1, 1], S[:, 0, 0]), (one[:, 0, 1], S[:, 1, 0]),
+                      (one[:, 1, 0], S[:, 1, 0])):
+        assert np.allclose(got, want, rtol=0, atol=1e-13)
+    # a matched, lossless 25 ohm line between 50 ohm references: Gamma = (25 - 50) / (25 + 50)
+    # at the input, and S11 = Gamma (1 - T^2) / (1 - Gamma^2 T^2), S21 = T (1 - Gamma^2) / (1 - Gamma^2 T^2)
+    z25 = np.full(nf, 25.0)
+    T = np.exp(-0.3j * np.arange(1, nf + 1))     # T = 1 would make I - S singular
+    line = {"freqs": f, "waves": [{"a": a1, "b": 0 * a1, "z0": z25}, {"a": 0 * a1, "b": T * a1, "z0": z25}]}
+    r = B.board_s_matrix(ref, [line])
+    g = -1 / 3
+    d = 1 - g * g * T * T
+    assert np.allclose(r[:, 0, 0], g * (1 - T * T) / d, rtol=0, atol=1e-13)
+    assert np.allclose(r[:, 1, 0], T * (1 - g * g) / d, rtol=0, atol=1e-13)
