@@ -17,6 +17,8 @@ with the checks the reference's own tests of that path make and closed forms:
     mask added through the stackup (thick.test.ts:21)
   * plan and build: port and copper planes on grid lines, CPML cells, the strip as PEC
     edges through the board and the feed, the reference's error messages
+  * a C11 caller (tests/c/board_build.c) builds a board through the header and gets the
+    Python path's mesh and ports
 """
 import math
 import os
@@ -289,3 +291,21 @@ def test_board_errors_follow_the_reference():
         B.import_kicad("(something_else)")
     with pytest.raises(L.CourantError, match="outline"):
         B.import_kicad("(kicad_pcb (net 0 \"\"))")
+
+
+def test_c_caller_builds_a_board(tmp_path):
+    """tests/c/board_build.c, compiled as C11 against include/courant_board.h and the
+    library: the mesh and ports it gets are the Python path's."""
+    import subprocess
+    root = os.path.dirname(HERE)
+    exe = str(tmp_path / "board_build")
+    lib_dir = os.path.join(root, "courant_amd")
+    subprocess.run(["gcc", "-std=c11", "-O1", "-Wall", "-Werror", os.path.join(HERE, "c", "board_build.c"),
+                    "-I" + os.path.join(root, "include"), "-L" + lib_dir, "-lcourant", "-Wl,-rpath," + lib_dir, "-lm",
+                    "-o", exe], check=True)
+    out = subprocess.run([exe, os.path.join(BOARDS, "msl.kicad_pcb")], stdout=subprocess.PIPE, text=True,
+                         check=True, timeout=120).stdout.split()
+    p = B.plan_board(fixture("msl"), RF_PORTS, fmax=40e9, df=1e9, hmin=40e-6)
+    assert tuple(map(int, out[1:4])) == p["cells"] == tuple(map(int, out[5:8]))
+    assert out[9:11] == ["0", "1"] and int(out[14]) == 40
+    assert float(out[12]) == pytest.approx(10.5 * math.sqrt(6.9) / (math.pi * 1.25 * 40e9), rel=1e-15)
