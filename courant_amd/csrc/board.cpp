@@ -1472,6 +1472,12 @@ extern "C" int cem_mesh_axis(const cem_axis_features *f, double hmin, double hma
                              int cap, double *report, char *warnings, int wcap) {
     return guarded([&] {
         if (!f || !(hmin > 0) || !(hmax > 0) || !(f->hi > f->lo)) throw Fail("mesh axis: bad features or sizes");
+        if ((f->nsheet > 0 && (!f->sheet_x || !f->sheet_metal_hi)) || (f->nplanes > 0 && !f->planes) ||
+            (f->nspans > 0 && !f->spans) || (f->ncaps > 0 && !f->caps) || (f->nrefine > 0 && !f->refine) ||
+            (f->nextra > 0 && !f->extra))
+            throw Fail("mesh axis: a feature count without its array");
+        for (int i = 0; i < f->nspans; i++)
+            if (!(f->spans[3 * i + 2] >= 1)) throw Fail("mesh axis: a span needs at least one cell");
         Features F;
         F.lo = f->lo;
         F.hi = f->hi;

@@ -113,6 +113,19 @@ def test_line_merging_and_span_warning():
     assert len(r["warnings"]) == 1 and r["warnings"][0].endswith("wanted 8")
 
 
+def test_mesher_rejects_bad_input():
+    import ctypes as C
+    cem = B._engine()
+    f = B.AxisFeatures()
+    f.lo, f.hi, f.nsheet, f.nrefine = 0.0, 1e-3, 2, -1          # a count without its array
+    assert cem.lib.cem_mesh_axis(C.byref(f), 20e-6, 1e-4, 1.3, None, 0, None, None, 0) == -1
+    assert "without its array" in cem.last_error()
+    with pytest.raises(L.CourantError, match="at least one cell"):
+        B.mesh_axis(0, 1e-3, 20e-6, 1e-4, spans=[(0, 1e-4, 0)])
+    with pytest.raises(L.CourantError, match="bad features"):
+        B.mesh_axis(1e-3, 0, 20e-6, 1e-4)
+
+
 def test_with_pml_and_estimate_closed_form():
     assert list(B.with_pml([0.0, 1.0, 3.0], 2, 0.5)) == [-1.0, -0.5, 0.0, 1.0, 3.0, 3.5, 4.0]
     assert list(B.with_pml([0.0, 1.0], 2, 0.5, lo=False)) == [0.0, 1.0, 1.5, 2.0]
