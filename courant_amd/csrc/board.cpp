@@ -385,7 +385,9 @@ AxisMesh mesh_axis(const Features &f, double h0, double hmax, double ratio) {
     std::vector<double> cum;
     for (size_t q = 0; q + 1 < lines.size(); q++) {
         const double a = lines[q], b = lines[q + 1];
-        const long M = std::max(64L, (long)std::ceil((b - a) / (h0 / 8)));
+        const double Md = std::ceil((b - a) / (h0 / 8));
+        if (!(Md < 1e8)) throw Fail("mesh axis: hmin too small for the domain");
+        const long M = std::max(64L, (long)Md);
         cum.assign(1, 0.0);
         for (long m = 0; m < M; m++) {
             const double x0 = a + ((b - a) * m) / M, x1 = a + ((b - a) * (m + 1)) / M;
@@ -445,6 +447,7 @@ struct SExpr {
 struct SReader {
     const char *s;
     size_t i = 0, n;
+    int depth = 0;
     explicit SReader(const char *text) : s(text), n(std::strlen(text)) {}
     void skip() {
         while (i < n && std::isspace((unsigned char)s[i])) i++;
@@ -453,6 +456,7 @@ struct SReader {
         skip();
         SExpr e;
         if (i < n && s[i] == '(') {
+            if (++depth > 256) throw Fail("S-expression nested too deeply");
             i++;
             e.list = true;
             for (;;) {
@@ -460,6 +464,7 @@ struct SReader {
                 if (i >= n) throw Fail("unbalanced parentheses");
                 if (s[i] == ')') {
                     i++;
+                    depth--;
                     return e;
                 }
                 e.items.push_back(read());
@@ -527,6 +532,7 @@ std::string str(const SExpr *e, size_t k = 1) {
 }
 
 double to_m(double v) { return v * 1e-3; }
+int to_int(double v) { return std::isfinite(v) && std::fabs(v) < 2e9 ? (int)v : 0; }   // net numbers
 Pt P(double x, double y) { return Pt{to_m(x), -to_m(y)}; }   // KiCad mm, y down -> m, y up
 
 Pt rotate(const Pt &p, double deg) {
@@ -622,7 +628,7 @@ Board import_kicad(const char *text) {
     const SExpr root = rd.read();
     if (head(root) != "kicad_pcb") throw Fail("not a kicad_pcb file");
     Board b;
-    for (const SExpr *n : kids(&root, "net")) b.set_net((int)item_num(n, 1), str(n, 2));
+    for (const SExpr *n : kids(&root, "net")) b.set_net(to_int(item_num(n, 1)), str(n, 2));
     // stackup (setup (stackup (layer ...))), listed top-down
     std::vector<Layer> top_down;
     const SExpr *setup = kid(&root, "setup");
@@ -665,12 +671,12 @@ Board import_kicad(const char *text) {
     auto point = [](const SExpr *e) { return P(num(e, 1), num(e, 2)); };
     for (const SExpr *s : kids(&root, "segment"))
         b.tracks.push_back(Track{point(kid(s, "start")), point(kid(s, "end")), to_m(num(kid(s, "width"))),
-                                 str(kid(s, "layer")), (int)truthy_or(num(kid(s, "net")), 0)});
+                                 str(kid(s, "layer")), to_int(num(kid(s, "net")))});
     for (const SExpr *a : kids(&root, "arc")) {   // arc tracks (KiCad 7+) as short segments
         const auto pts = arc_through(point(kid(a, "start")), point(kid(a, "mid")), point(kid(a, "end")));
         for (size_t k = 0; k + 1 < pts.size(); k++)
             b.tracks.push_back(Track{pts[k], pts[k + 1], to_m(num(kid(a, "width"))), str(kid(a, "layer")),
-                                     (int)truthy_or(num(kid(a, "net")), 0)});
+                                     to_int(num(kid(a, "net")))});
     }
     for (const SExpr *v : kids(&root, "via")) {
         const SExpr *ly = kid(v, "layers");
@@ -680,11 +686,11 @@ Board import_kicad(const char *text) {
         via.drill = to_m(num(kid(v, "drill")));
         via.la = ly && ly->items.size() > 1 ? ly->items[1].atom : "F.Cu";
         via.lb = ly && ly->items.size() > 2 ? ly->items[2].atom : "B.Cu";
-        via.net = (int)truthy_or(num(kid(v, "net")), 0);
+        via.net = to_int(num(kid(v, "net")));
         b.vias.push_back(via);
     }
     for (const SExpr *zn : kids(&root, "zone")) {
-        const int net = (int)truthy_or(num(kid(zn, "net")), 0);
+        const int net = to_int(num(kid(zn, "net")));
         for (const SExpr *fp : kids(zn, "filled_polygon")) {
             Poly q;
             for (const SExpr *xy : kids(kid(fp, "pts"), "xy")) q.pts.push_back(P(item_num(xy, 1), item_num(xy, 2)));
@@ -702,7 +708,7 @@ Board import_kicad(const char *text) {
             const std::string shape = pad->items.size() > 3 && !pad->items[3].list ? pad->items[3].atom : "rect";
             const double prot = truthy_or(num(pat, 3), 0);
             const double w = num(sz, 1), h = num(sz, 2);
-            const int net = (int)truthy_or(num(kid(pad, "net")), 0);
+            const int net = to_int(num(kid(pad, "net")));
             // pad centre: the footprint rotation applies to the pad offset (KiCad, y down)
             const Pt off = rotate(Pt{num(pat, 1), -num(pat, 2)}, frot);
             const double cx = fx + off.x, cy = fy - off.y;

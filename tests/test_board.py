@@ -304,6 +304,12 @@ def test_board_errors_follow_the_reference():
         B.import_kicad("(something_else)")
     with pytest.raises(L.CourantError, match="outline"):
         B.import_kicad("(kicad_pcb (net 0 \"\"))")
+    with pytest.raises(L.CourantError, match="nested too deeply"):
+        B.import_kicad("(" * 300 + ")" * 300)
+    with pytest.raises(L.CourantError, match="unbalanced"):
+        B.import_kicad("(kicad_pcb (net 0 \"\")")
+    with pytest.raises(L.CourantError, match="hmin too small"):
+        B.mesh_axis(0, 1.0, 1e-9, 1e-3)
 
 
 def test_c_caller_builds_a_board(tmp_path):
