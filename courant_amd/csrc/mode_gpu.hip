@@ -443,7 +443,7 @@ __device__ __forceinline__ void fill_back(const Geo g, const zc *__restrict__ ba
 // band columns the walk needs into a three-deep LDS ring, two chunks ahead,
 // with one workgroup barrier per chunk.
 template <int SF, int SB>
-__device__ void band_solve(const Geo g, const zc *__restrict__ band, const int *__restrict__ ipiv, zc *y,
+__device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ band, const int *__restrict__ ipiv, zc *y,
                            int chunk_f, int chunk_b, zc *ring, int *rpiv, long long *tm) {
     const int n = g.n, kl = g.bw, kv = 2 * g.bw;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -483,7 +483,23 @@ __device__ void band_solve(const Geo g, const zc *__restrict__ band, const int *
                         const int i = 64 * (jb + S) + lane;
                         pend = jb + S < nblk && i < n ? y[i] : zero;
                     }
-                    const int p = cp[j - cj0], last = min(j + kl, n - 1);
+                    // the column's pivot and its multipliers for every slot, read from
+                    // the ring together (clamped in-column offsets) before any use;
+                    // outside rows j+1 .. last a multiplier is zero, which leaves its
+                    // entry unchanged, so every slot updates without a branch
+                    const zc *l = cur + (size_t)(j - cj0) * K;
+                    const int p = __builtin_amdgcn_readfirstlane(cp[j - cj0]), last = min(j + kl, n - 1);
+                    zc lv[MAXS];
+#pragma unroll
+                    for (int k = 0; k < MAXS; k++) {
+                        const int i = 64 * (jb + k) + lane;
+                        lv[k] = l[min(max(i - j - 1, 0), K - 1)];
+                    }
+#pragma unroll
+                    for (int k = 0; k < MAXS; k++) {
+                        const int i = 64 * (jb + k) + lane;
+                        if (!(k < S && i > j && i <= last)) lv[k] = zero;
+                    }
                     zc vj = bcast(v[0], j & 63);
                     if (p != j) {
                         const unsigned mp = 1u << ((p >> 6) - jb);
@@ -492,12 +508,8 @@ __device__ void band_solve(const Geo g, const zc *__restrict__ band, const int *
                         if (lane == (p & 63)) slot_put(v, mp, vj);
                         vj = vp;
                     }
-                    const zc *l = cur + (size_t)(j - cj0) * K;
 #pragma unroll
-                    for (int k = 0; k < MAXS; k++) {
-                        const int i = 64 * (jb + k) + lane;
-                        if (k < S && i > j && i <= last) v[k] = zfms(v[k], l[i - j - 1], vj);
-                    }
+                    for (int k = 0; k < MAXS; k++) v[k] = zfms(v[k], lv[k], vj);
                 }
             } else if (ch + 2 < nchunk) {
                 fill_fwd(g, band, ipiv, ring, rpiv, CH, ch + 2, threadIdx.x - 64, kInvThreads - 64);
@@ -549,14 +561,25 @@ __device__ void band_solve(const Geo g, const zc *__restrict__ band, const int *
                         pend = jb - S >= 0 ? y[64 * (jb - S) + lane] : zero;
                     }
                     const zc *u = cur + (size_t)(cj1 - 1 - j) * K;   // u[rr] = U(j - kv + rr, j), u[kv] = 1/U(j, j)
-                    const zc xj = zmul(bcast(v[0], j & 63), u[kv]);
-                    if (lane == (j & 63)) v[0] = xj;
+                    // the reciprocal pivot and every slot's multiplier read together
+                    // (clamped in-column offsets); zero outside rows first .. j-1
                     const int first = max(0, j - kv);
+                    const zc ukv = u[kv];
+                    zc uv[MAXS];
 #pragma unroll
                     for (int k = 0; k < MAXS; k++) {
                         const int i = 64 * (jb - k) + lane;
-                        if (k < S && i >= first && i < j) v[k] = zfms(v[k], u[i - (j - kv)], xj);
+                        uv[k] = u[min(max(i - (j - kv), 0), K - 1)];
                     }
+#pragma unroll
+                    for (int k = 0; k < MAXS; k++) {
+                        const int i = 64 * (jb - k) + lane;
+                        if (!(k < S && i >= first && i < j)) uv[k] = zero;
+                    }
+                    const zc xj = zmul(bcast(v[0], j & 63), ukv);
+                    if (lane == (j & 63)) v[0] = xj;
+#pragma unroll
+                    for (int k = 0; k < MAXS; k++) v[k] = zfms(v[k], uv[k], xj);
                 }
             } else if (ch + 2 < nchunk) {
                 fill_back(g, band, ring, CH, ch + 2, threadIdx.x - 64, kInvThreads - 64);
