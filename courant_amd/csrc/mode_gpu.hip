@@ -6,8 +6,9 @@
 //                 batch of colours per launch; the results are scattered into
 //                 the complex band (kl = ku = bw, ldab = 3 bw + 1, column-major)
 //   k_diag      : PEC unknowns pushed to -1e6 (|sigma| + 1), then - sigma
-//   k_lu_panel  : one panel of nb columns factorised in LDS by one workgroup:
-//                 partial pivoting over the kl rows below the diagonal, row
+//   k_lu_panel  : one panel of nb columns factorised by one workgroup, one
+//                 thread per panel row holding the row in registers: partial
+//                 pivoting over the kl rows below the diagonal, row
 //                 interchanges within the panel, scaling, rank-1 updates
 //   k_lu_trail  : the panel's interchanges and rank-1 updates applied to the
 //                 trailing columns (one wavefront per column, column in
@@ -80,6 +81,31 @@ __device__ __forceinline__ double bcast(double v, int src) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 __device__ __forceinline__ zc bcast(zc v, int src) { return mk(bcast(v.r, src), bcast(v.i, src)); }
+// the lane's value exchanged by a DPP move (CTRL: quad xor 1 = 0xB1, quad xor 2 =
+// 0x4E, half-row mirror = 0x141, row mirror = 0x140) or by a swizzle that exchanges
+// the two 16-lane rows of each 32-lane half
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double swap_rows16(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffll), 0x401F);
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// largest value over the wavefront, in every lane (all 64 lanes active; no NaN)
+__device__ __forceinline__ double wave_max(double v) {
+    v = fmax(v, dpp_move<0xB1>(v));
+    v = fmax(v, dpp_move<0x4E>(v));
+    v = fmax(v, dpp_move<0x141>(v));
+    v = fmax(v, dpp_move<0x140>(v));
+    v = fmax(v, swap_rows16(v));
+    return fmax(bcast(v, 0), bcast(v, 32));
+}
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
@@ -236,74 +262,108 @@ __global__ void k_diag(Geo g, Coef c, zc sigma, double big, zc *band) {
 }
 
 // ---- band LU -------------------------------------------------------------
-// Panel of columns [j0, j1), rows [j0, j0 + R), R = min(j1 - 1 + kl, n - 1) - j0 + 1,
-// column-major in LDS.  Entries below row c + kl of column c are zero and stay so.
-__global__ void __launch_bounds__(kLuThreads) k_lu_panel(Geo g, zc *band, int *ipiv, int *info, int j0, int j1) {
-    extern __shared__ zc P[];
-    __shared__ int piv_row;
-    __shared__ double piv_abs;
+// Panel of columns [j0, j1), nbp = j1 - j0 <= NB, rows [j0, j0 + R),
+// R = min(j1 - 1 + kl, n - 1) - j0 + 1 <= blockDim.x: thread r holds row j0 + r of
+// the panel's not yet factorised columns in registers, a[0] being column j (entries
+// below row c + kl of column c are zero and stay so).  Per column j: the pivot
+// (largest |.| over rows j..j + kl, the first on ties), the interchange of rows j
+// and p over the columns >= j through LDS, the scaling of the multipliers and the
+// rank-1 update of the later panel columns; column j is then final and stored, and
+// the window shifts by one column (a rolled loop: the kernel runs once per panel,
+// and fully unrolled code would be fetched from L2 each time).  Every entry sees
+// the operations of the unblocked column loop in its order.  Two barriers per
+// column: the LDS buffers alternate with the column's parity, so a buffer is
+// rewritten only after every thread has passed the next column's first barrier.
+template <int NB>
+__global__ void __launch_bounds__(512) k_lu_panel(Geo g, zc *band, int *ipiv, int *info, int j0, int j1) {
+    __shared__ double wbest[2][8];
+    __shared__ int wrow[2][8];
+    __shared__ zc prow[2][NB], jrow[2][NB];
     const int kl = g.bw, nbp = j1 - j0;
     const int R = min(j1 - 1 + kl, g.n - 1) - j0 + 1;
-    copy_batched<16>(P, R * nbp, threadIdx.x, kLuThreads, [=](int t) {
-        const int cc = t / R, rr = t % R, col = j0 + cc, row = j0 + rr;
-        return row <= col + kl ? band[bidx(g, row, col)] : mk(0.0, 0.0);
-    });
-    __syncthreads();
-    for (int j = j0; j < j1; j++) {
-        const int last = min(j + kl, g.n - 1);
-        const int cj = j - j0;
-        zc *colj = P + (size_t)cj * R;
-        if (threadIdx.x < 64) {   // pivot: largest |.|, first one on ties (rows ascending)
-            double best = -1.0;
-            int bi = j;
-            for (int i = j + (int)threadIdx.x; i <= last; i += 64) {
-                const double v = zabs(colj[i - j0]);
-                if (v > best) {
-                    best = v;
-                    bi = i;
-                }
-            }
-            for (int off = 32; off >= 1; off >>= 1) {
-                const double ob = __shfl_xor(best, off);
-                const int oi = __shfl_xor(bi, off);
-                if (ob > best || (ob == best && oi < bi)) {
-                    best = ob;
-                    bi = oi;
-                }
-            }
-            if (threadIdx.x == 0) {
-                piv_row = bi;
-                piv_abs = best;
+    const int r = threadIdx.x, row = j0 + r, lane = r & 63, wv = r >> 6, nw = blockDim.x >> 6;
+    zc a[NB];
+#pragma unroll
+    for (int c = 0; c < NB; c++)
+        a[c] = r < R && c < nbp && row <= j0 + c + kl ? band[bidx(g, row, j0 + c)] : mk(0.0, 0.0);
+    // loads complete here (vmcnt 0): otherwise the loop, whose column stores share the
+    // counter, waits for every store before each use of a row register
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    for (int cj = 0; cj < nbp; cj++) {
+        const int j = j0 + cj, last = min(j + kl, g.n - 1), s = cj & 1;
+        double best = -1.0;
+        int bi = j;
+        if (row >= j && row <= last) {
+            const double v = zabs(a[0]);
+            if (v > best) {
+                best = v;
+                bi = row;
             }
         }
+        // the wavefront's largest value; its first row (rows ascend with the lanes);
+        // row j where no lane holds a candidate value (-1 everywhere)
+        {
+            const double m = wave_max(best);
+            const unsigned long long hit = __ballot(best == m);
+            bi = m == -1.0 ? j : j0 + (wv << 6) + __builtin_ctzll(hit);
+            best = m;
+        }
+        if (lane == 0) {
+            wbest[s][wv] = best;
+            wrow[s][wv] = bi;
+        }
         __syncthreads();
-        const int p = piv_row;
-        if (threadIdx.x == 0) {
+        {
+            double wb[8];
+            int wr[8];
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                wb[w] = wbest[s][w];
+                wr[w] = wrow[s][w];
+            }
+            best = wb[0];
+            bi = wr[0];
+#pragma unroll
+            for (int w = 1; w < 8; w++)
+                if (w < nw && (wb[w] > best || (wb[w] == best && wr[w] < bi))) {
+                    best = wb[w];
+                    bi = wr[w];
+                }
+        }
+        const int p = bi;
+        if (r == 0) {
             ipiv[j] = p;
-            if (piv_abs == 0.0 && *info == 0) *info = j + 1;
+            if (best == 0.0 && *info == 0) *info = j + 1;
         }
-        if (p != j)
-            for (int cc = cj + (int)threadIdx.x; cc < nbp; cc += kLuThreads) {
-                zc *col = P + (size_t)cc * R;
-                const zc a = col[j - j0];
-                col[j - j0] = col[p - j0];
-                col[p - j0] = a;
+        // whole windows: the registers past the panel's last column hold zeros and are
+        // never stored, so they are exchanged and updated without a test
+        if (row == p) {
+#pragma unroll
+            for (int c = 0; c < NB; c++) prow[s][c] = a[c];
+        } else if (row == j) {
+#pragma unroll
+            for (int c = 0; c < NB; c++) jrow[s][c] = a[c];
+        }
+        __syncthreads();
+        if (p != j) {
+            if (row == j) {
+#pragma unroll
+                for (int c = 0; c < NB; c++) a[c] = prow[s][c];
+            } else if (row == p) {
+#pragma unroll
+                for (int c = 0; c < NB; c++) a[c] = jrow[s][c];
             }
-        __syncthreads();
-        const zc pv = colj[cj];
-        for (int i = j + 1 + (int)threadIdx.x; i <= last; i += kLuThreads) colj[i - j0] = zdiv(colj[i - j0], pv);
-        __syncthreads();
-        const int nr = last - j, ncol = nbp - cj - 1;
-        for (int t = threadIdx.x; t < nr * ncol; t += kLuThreads) {
-            const int cc = cj + 1 + t / nr, i = j + 1 + t % nr;
-            zc *col = P + (size_t)cc * R;
-            col[i - j0] = zsub(col[i - j0], zmul(colj[i - j0], col[cj]));
         }
-        __syncthreads();
-    }
-    for (int t = threadIdx.x; t < R * nbp; t += kLuThreads) {
-        const int cc = t / R, rr = t % R, col = j0 + cc, row = j0 + rr;
-        if (row <= col + kl) band[bidx(g, row, col)] = P[t];
+        if (row > j && row <= last) {
+            const zc l = zdiv(a[0], prow[s][0]);
+            a[0] = l;
+#pragma unroll
+            for (int c = 1; c < NB; c++) a[c] = zsub(a[c], zmul(l, prow[s][c]));
+        }
+        if (r < R && row <= j + kl) band[bidx(g, row, j)] = a[0];
+#pragma unroll
+        for (int c = 0; c + 1 < NB; c++) a[c] = a[c + 1];
+        a[NB - 1] = mk(0.0, 0.0);
     }
 }
 
@@ -773,9 +833,10 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     g.ldab = 3 * g.bw + 1;
     g.wc = 2 * g.bw + 1;
     const int n = g.n, kl = g.bw, kv = 2 * g.bw;
-    // panel width: the panel (kl + nb rows x nb columns) fits 60 KiB of LDS (wider
-    // panels measured slower: the trailing walk per column grows with nb)
-    int nb = 32;
+    // panel width: at most 16 columns (the panel kernel keeps a row's entries in
+    // registers; the trailing walk per column grows with nb) and the panel
+    // (kl + nb rows x nb columns) within 60 KiB of LDS for the trailing update
+    int nb = 16;
     while (nb > 1 && (size_t)(kl + nb) * nb * sizeof(zc) > 60 * 1024) nb /= 2;
     // triangular solves: three slots of chunk columns in the LDS ring; register
     // window of S = ceil(K / 64) + 1 blocks of 64 entries
@@ -862,11 +923,14 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
         const int R = std::min(j1 - 1 + kl, n - 1) - j0 + 1;
         const size_t lds = (size_t)R * (j1 - j0) * sizeof(zc);
         if (lds > lu_lds_set) {   // dynamic LDS beyond 64 KiB needs the per-kernel attribute
-            MG_CHECK(hipFuncSetAttribute((const void *)k_lu_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             MG_CHECK(hipFuncSetAttribute((const void *)k_lu_trail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             lu_lds_set = lds;
         }
-        hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(kLuThreads), lds, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+        const dim3 pb((unsigned)((R + 63) / 64 * 64));   // one thread per panel row (R <= 512)
+        if (nb > 8)
+            hipLaunchKernelGGL(k_lu_panel<16>, dim3(1), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+        else
+            hipLaunchKernelGGL(k_lu_panel<8>, dim3(1), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
         const int ncol = std::min(j1 - 1 + kv, n - 1) - j1 + 1;
         if (ncol > 0) {
             const int wpb = kLuThreads / 64;
