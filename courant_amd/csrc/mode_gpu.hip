@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -779,6 +780,12 @@ struct ModeGpu {
     DMem<int> ipiv, info;
     DMem<double> out;
     std::vector<zc> hcoef;
+    // the band LU's launches (two per panel), captured for one geometry and set of
+    // buffers and replayed by every later solve of that geometry
+    hipGraphExec_t lu = nullptr;
+    const void *lu_key[3] = {};
+    int lu_n = -1, lu_bw = -1;
+    std::chrono::steady_clock::time_point t_last{};   // end of the previous solve (debug timing)
 };
 
 bool mode_gpu_enabled() {
@@ -793,6 +800,7 @@ bool mode_gpu_enabled() {
 void mode_gpu_release(ModeGpu *m) {
     if (!m) return;
     if (m->st) (void)hipStreamSynchronize(m->st);
+    if (m->lu) (void)hipGraphExecDestroy(m->lu);
     for (auto *b : {&m->band, &m->coef, &m->x, &m->xo, &m->y, &m->hw, &m->hu, &m->hv, &m->ew}) b->release();
     m->pec.release();
     m->ipiv.release();
@@ -821,6 +829,7 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
         }
     }
     ModeGpu &M = **ctx;
+    const auto t_enter = std::chrono::steady_clock::now();
     Geo g;
     g.ny = p.ny;
     g.nz = p.nz;
@@ -916,27 +925,57 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     hipLaunchKernelGGL(k_diag, dim3((n + T - 1) / T), dim3(T), 0, st, g, c, zc{p.sigma.real(), p.sigma.imag()}, p.big,
                        M.band.p);
     if (debug) MG_CHECK(hipEventRecord(ev[1], st));
-    // band LU, panel by panel
-    size_t lu_lds_set = 0;
-    for (int j0 = 0; j0 < n; j0 += nb) {
-        const int j1 = std::min(n, j0 + nb);
-        const int R = std::min(j1 - 1 + kl, n - 1) - j0 + 1;
-        const size_t lds = (size_t)R * (j1 - j0) * sizeof(zc);
-        if (lds > lu_lds_set) {   // dynamic LDS beyond 64 KiB needs the per-kernel attribute
-            MG_CHECK(hipFuncSetAttribute((const void *)k_lu_trail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            lu_lds_set = lds;
+    // band LU, panel by panel: two launches per panel (1897 panels on a 30 k-unknown
+    // port).  Replayed from a graph captured on the first solve of a geometry, so that
+    // solves running at once on other host threads (the port sweeps) do not queue
+    // behind thousands of launch submissions; COURANT_GRAPH=0 launches directly.
+    {
+        const size_t lds_max = (size_t)(std::min(nb - 1 + kl, n - 1) + 1) * std::min(nb, n) * sizeof(zc);
+        MG_CHECK(hipFuncSetAttribute((const void *)k_lu_trail, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds_max));
+    }
+    auto launch_lu = [&]() {
+        for (int j0 = 0; j0 < n; j0 += nb) {
+            const int j1 = std::min(n, j0 + nb);
+            const int R = std::min(j1 - 1 + kl, n - 1) - j0 + 1;
+            const size_t lds = (size_t)R * (j1 - j0) * sizeof(zc);
+            const dim3 pb((unsigned)((R + 63) / 64 * 64));   // one thread per panel row (R <= 512)
+            if (nb > 8)
+                hipLaunchKernelGGL(k_lu_panel<16>, dim3(1), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+            else
+                hipLaunchKernelGGL(k_lu_panel<8>, dim3(1), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+            const int ncol = std::min(j1 - 1 + kv, n - 1) - j1 + 1;
+            if (ncol > 0) {
+                const int wpb = kLuThreads / 64;
+                hipLaunchKernelGGL(k_lu_trail, dim3((ncol + wpb - 1) / wpb), dim3(kLuThreads), lds, st, g, M.band.p,
+                                   M.ipiv.p, j0, j1);
+            }
         }
-        const dim3 pb((unsigned)((R + 63) / 64 * 64));   // one thread per panel row (R <= 512)
-        if (nb > 8)
-            hipLaunchKernelGGL(k_lu_panel<16>, dim3(1), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
-        else
-            hipLaunchKernelGGL(k_lu_panel<8>, dim3(1), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
-        const int ncol = std::min(j1 - 1 + kv, n - 1) - j1 + 1;
-        if (ncol > 0) {
-            const int wpb = kLuThreads / 64;
-            hipLaunchKernelGGL(k_lu_trail, dim3((ncol + wpb - 1) / wpb), dim3(kLuThreads), lds, st, g, M.band.p,
-                               M.ipiv.p, j0, j1);
+    };
+    const char *gv = std::getenv("COURANT_GRAPH");
+    if (gv && std::atoi(gv) == 0) {
+        launch_lu();
+    } else {
+        const void *key[3] = {M.band.p, M.ipiv.p, M.info.p};
+        if (!M.lu || M.lu_n != n || M.lu_bw != kl || std::memcmp(key, M.lu_key, sizeof key) != 0) {
+            if (M.lu) (void)hipGraphExecDestroy(M.lu);
+            M.lu = nullptr;
+            hipGraph_t graph = nullptr;
+            MG_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            launch_lu();
+            MG_CHECK(hipStreamEndCapture(st, &graph));
+            const hipError_t e = hipGraphInstantiate(&M.lu, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                M.lu = nullptr;
+                set_error("GPU mode solver: cannot instantiate the LU graph");
+                return -1;
+            }
+            std::memcpy(M.lu_key, key, sizeof key);
+            M.lu_n = n;
+            M.lu_bw = kl;
         }
+        MG_CHECK(hipGraphLaunch(M.lu, st));
     }
     if (debug) MG_CHECK(hipEventRecord(ev[2], st));
     InvArgs ia;
@@ -977,8 +1016,14 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     if (debug) {
         float t[3];
         for (int k = 0; k < 3; k++) MG_CHECK(hipEventElapsedTime(&t[k], ev[k], ev[k + 1]));
-        std::fprintf(stderr, "mode_gpu: n %d bw %d nb %d chunks %d/%d | assembly %.2f ms, LU %.2f ms, %d iterations %.2f ms\n",
-                     n, kl, nb, chunk_f, chunk_b, t[0], t[1], (int)out[3] + 1, t[2]);
+        const auto t_now = std::chrono::steady_clock::now();
+        const double wall = std::chrono::duration<double, std::milli>(t_now - t_enter).count();
+        const double between =
+            M.t_last.time_since_epoch().count() ? std::chrono::duration<double, std::milli>(t_enter - M.t_last).count() : 0.0;
+        M.t_last = t_now;
+        std::fprintf(stderr, "mode_gpu: n %d bw %d nb %d chunks %d/%d | assembly %.2f ms, LU %.2f ms, %d iterations %.2f ms"
+                     " | call %.1f ms, host time since the previous call %.1f ms\n",
+                     n, kl, nb, chunk_f, chunk_b, t[0], t[1], (int)out[3] + 1, t[2], wall, between);
         std::fprintf(stderr, "mode_gpu: cycles fwd walk %.3g wait %.3g | back walk %.3g wait %.3g | fill fwd %.3g back %.3g\n",
                      out[4], out[5], out[6], out[7], out[8], out[9]);
         for (auto &e : ev) (void)hipEventDestroy(e);
