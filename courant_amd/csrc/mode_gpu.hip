@@ -760,6 +760,30 @@ struct DMem {
     }
 };
 
+// pinned host staging of one solver context: copies from and to pageable memory go
+// through the runtime's shared staging path, where one host thread's copy can wait
+// for another thread's long kernel; pinned copies are plain DMA on the stream
+template <typename T>
+struct HMem {
+    T *p = nullptr;
+    size_t n = 0;
+    bool get(size_t count) {
+        if (count <= n) return true;
+        release();
+        if (hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        n = count;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 // dynamic LDS beyond 64 KiB needs the per-kernel attribute (gfx950: up to 160 KiB)
 template <int SF, int SB>
 int run_inverse(size_t lds, hipStream_t st, const InvArgs &ia) {
@@ -779,7 +803,9 @@ struct ModeGpu {
     DMem<unsigned char> pec;
     DMem<int> ipiv, info;
     DMem<double> out;
-    std::vector<zc> hcoef;
+    HMem<zc> hcoef, hx;           // coefficients; start vector in, eigenvector out
+    HMem<unsigned char> hpec;
+    HMem<double> hout;            // info, then the kernel's ten result words
     // the band LU's launches (two per panel), captured for one geometry and set of
     // buffers and replayed by every later solve of that geometry
     hipGraphExec_t lu = nullptr;
@@ -806,6 +832,10 @@ void mode_gpu_release(ModeGpu *m) {
     m->ipiv.release();
     m->info.release();
     m->out.release();
+    m->hcoef.release();
+    m->hx.release();
+    m->hpec.release();
+    m->hout.release();
     if (m->st) (void)hipStreamDestroy(m->st);
     delete m;
 }
@@ -821,7 +851,18 @@ void mode_gpu_release(ModeGpu *m) {
 int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> *xout, std::complex<double> *lam) {
     if (!*ctx) {
         *ctx = new ModeGpu();
-        if (hipStreamCreateWithFlags(&(*ctx)->st, hipStreamNonBlocking) != hipSuccess) {
+        // a stream with a CU mask (here every CU) gets a hardware queue of its own; plain
+        // streams share a small pool of queues (4 by default), and two port sweeps whose
+        // streams share one wait for each other's kernels.  A plain stream if refused.
+        int dev = 0, ncu = 0;
+        bool own_queue = false;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) {
+            std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0xffffffffu);
+            if (ncu % 32) mask.back() = (1u << (ncu % 32)) - 1u;
+            own_queue = hipExtStreamCreateWithCUMask(&(*ctx)->st, (uint32_t)mask.size(), mask.data()) == hipSuccess;
+        }
+        if (!own_queue && hipStreamCreateWithFlags(&(*ctx)->st, hipStreamNonBlocking) != hipSuccess) {
             delete *ctx;
             *ctx = nullptr;
             set_error("GPU mode solver: cannot create stream");
@@ -868,16 +909,21 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
         set_error("GPU mode solver: device out of memory");
         return -1;
     }
+    // the pinned buffers are rewritten below: no copy of an earlier (failed) call in flight
+    MG_CHECK(hipStreamSynchronize(st));
+    if (!M.hcoef.get(ncoef) || !M.hx.get(n) || !M.hpec.get(g.nu + g.nv) || !M.hout.get(11)) {
+        set_error("GPU mode solver: cannot allocate pinned host memory");
+        return -1;
+    }
     const bool debug = std::getenv("COURANT_MODE_DEBUG") != nullptr;
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[5] = {};
     if (debug)
         for (auto &e : ev) MG_CHECK(hipEventCreate(&e));
     if (debug) MG_CHECK(hipEventRecord(ev[0], st));
     // coefficients (host-built, FP64 complex) and PEC masks
-    M.hcoef.resize(ncoef);
     size_t o = 0;
     auto put = [&](const std::complex<double> *src, int len) {
-        for (int i = 0; i < len; i++) M.hcoef[o + i] = zc{src[i].real(), src[i].imag()};
+        for (int i = 0; i < len; i++) M.hcoef.p[o + i] = zc{src[i].real(), src[i].imag()};
         o += len;
     };
     put(p.iyc, p.ny);
@@ -887,14 +933,12 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     put(p.eu, g.nu);
     put(p.ev, g.nv);
     put(p.iw, g.nn);
-    MG_CHECK(hipMemcpyAsync(M.coef.p, M.hcoef.data(), ncoef * sizeof(zc), hipMemcpyHostToDevice, st));
-    std::vector<unsigned char> pm(g.nu + g.nv);
-    std::memcpy(pm.data(), p.pec_u, g.nu);
-    std::memcpy(pm.data() + g.nu, p.pec_v, g.nv);
-    MG_CHECK(hipMemcpyAsync(M.pec.p, pm.data(), pm.size(), hipMemcpyHostToDevice, st));
-    std::vector<zc> x0(n);
-    for (int i = 0; i < n; i++) x0[i] = zc{p.x0[i].real(), p.x0[i].imag()};
-    MG_CHECK(hipMemcpyAsync(M.x.p, x0.data(), n * sizeof(zc), hipMemcpyHostToDevice, st));
+    MG_CHECK(hipMemcpyAsync(M.coef.p, M.hcoef.p, ncoef * sizeof(zc), hipMemcpyHostToDevice, st));
+    std::memcpy(M.hpec.p, p.pec_u, g.nu);
+    std::memcpy(M.hpec.p + g.nu, p.pec_v, g.nv);
+    MG_CHECK(hipMemcpyAsync(M.pec.p, M.hpec.p, g.nu + g.nv, hipMemcpyHostToDevice, st));
+    for (int i = 0; i < n; i++) M.hx.p[i] = zc{p.x0[i].real(), p.x0[i].imag()};
+    MG_CHECK(hipMemcpyAsync(M.x.p, M.hx.p, n * sizeof(zc), hipMemcpyHostToDevice, st));
     MG_CHECK(hipMemsetAsync(M.xo.p, 0, n * sizeof(zc), st));
     MG_CHECK(hipMemsetAsync(M.band.p, 0, (size_t)g.ldab * n * sizeof(zc), st));
     MG_CHECK(hipMemsetAsync(M.info.p, 0, sizeof(int), st));
@@ -1006,30 +1050,33 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     }
     MG_CHECK(hipGetLastError());
     if (debug) MG_CHECK(hipEventRecord(ev[3], st));
-    int info = 0;
-    double out[10];
-    std::vector<zc> xr(n);
-    MG_CHECK(hipMemcpyAsync(&info, M.info.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    MG_CHECK(hipMemcpyAsync(out, M.out.p, sizeof out, hipMemcpyDeviceToHost, st));
-    MG_CHECK(hipMemcpyAsync(xr.data(), M.x.p, n * sizeof(zc), hipMemcpyDeviceToHost, st));
+    int *const hinfo = reinterpret_cast<int *>(M.hout.p);
+    const double *const out = M.hout.p + 1;
+    MG_CHECK(hipMemcpyAsync(hinfo, M.info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    MG_CHECK(hipMemcpyAsync(M.hout.p + 1, M.out.p, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MG_CHECK(hipMemcpyAsync(M.hx.p, M.x.p, n * sizeof(zc), hipMemcpyDeviceToHost, st));
+    if (debug) MG_CHECK(hipEventRecord(ev[4], st));
+    const auto t_sync = std::chrono::steady_clock::now();
     MG_CHECK(hipStreamSynchronize(st));
+    const int info = *hinfo;
     if (debug) {
-        float t[3];
-        for (int k = 0; k < 3; k++) MG_CHECK(hipEventElapsedTime(&t[k], ev[k], ev[k + 1]));
+        float t[4];
+        for (int k = 0; k < 4; k++) MG_CHECK(hipEventElapsedTime(&t[k], ev[k], ev[k + 1]));
         const auto t_now = std::chrono::steady_clock::now();
+        const double t_wait = std::chrono::duration<double, std::milli>(t_now - t_sync).count();
         const double wall = std::chrono::duration<double, std::milli>(t_now - t_enter).count();
         const double between =
             M.t_last.time_since_epoch().count() ? std::chrono::duration<double, std::milli>(t_enter - M.t_last).count() : 0.0;
         M.t_last = t_now;
         std::fprintf(stderr, "mode_gpu: n %d bw %d nb %d chunks %d/%d | assembly %.2f ms, LU %.2f ms, %d iterations %.2f ms"
-                     " | call %.1f ms, host time since the previous call %.1f ms\n",
-                     n, kl, nb, chunk_f, chunk_b, t[0], t[1], (int)out[3] + 1, t[2], wall, between);
+                     " | result copies %.2f ms | call %.1f ms, final wait %.1f ms, host time since the previous call %.1f ms\n",
+                     n, kl, nb, chunk_f, chunk_b, t[0], t[1], (int)out[3] + 1, t[2], t[3], wall, t_wait, between);
         std::fprintf(stderr, "mode_gpu: cycles fwd walk %.3g wait %.3g | back walk %.3g wait %.3g | fill fwd %.3g back %.3g\n",
                      out[4], out[5], out[6], out[7], out[8], out[9]);
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
     if (info) return -1;   // singular pivot: the host path reports the same as a failed LU
-    for (int i = 0; i < n; i++) xout[i] = std::complex<double>(xr[i].r, xr[i].i);
+    for (int i = 0; i < n; i++) xout[i] = std::complex<double>(M.hx.p[i].r, M.hx.p[i].i);
     *lam = std::complex<double>(out[0], out[1]);
     return out[2] == 0.0 ? 0 : 1;
 }
