@@ -198,6 +198,22 @@ def _served(cem, serve, nreq, rate, workers, budget_gb):
     return served
 
 
+def mode_sweep(cem, df):
+    """Wall time of the eigenmode sweep of both ports of the lossy GCPW (mode ports,
+    300 x 152 x 99 cells) at output frequencies spaced df: the sweep runs on the first request
+    for a port's waves, after a few steps."""
+    from courant_amd import scenes as S
+    sim, info = S.gcpw(cem, **dict(S.CFG2, df=df))
+    try:
+        sim.step(64)
+        t = time.perf_counter()
+        sim.mode_port_waves(info["ports"][0])
+        wall = time.perf_counter() - t
+        return {"frequencies": len(info["freqs"]), "ports": 2, "wall_s": round(wall, 2)}
+    finally:
+        sim.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +234,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=5.0, help="Poisson arrival rate, boards/s")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solves (host threads, one stream each)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="device footprint admitted at once")
+    ap.add_argument("--mode-sweep", type=float, default=0.0, metavar="DF",
+                    help="also time the lossy GCPW's port-mode sweep at output frequencies spaced DF Hz")
     args = ap.parse_args()
     if args.copy_only:
         from courant_amd import lib
@@ -319,6 +337,8 @@ def main():
             out["batched"] = b
         if v:
             out["served"] = v
+    if args.mode_sweep > 0:
+        out["mode_sweep"] = mode_sweep(cem, args.mode_sweep)
     if not args.no_cpu and args.workload == "cfg2":
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -531,6 +531,34 @@ def test_mode2d_gpu_matches_host_lossy_port(cem, gpu_available, monkeypatch):
         cem.lib.cem_mode2d_destroy(m)
 
 
+def test_mode2d_lu_graph_matches_direct_launches(cem, gpu_available, monkeypatch):
+    """The band LU's launches are replayed from a graph captured on a context's first solve;
+    COURANT_GRAPH=0 launches them directly.  Solves alternate between the two on one context
+    (capture, then replays with direct-launch solves and host reads between them), at several
+    frequencies with seeded starts: every eigenvalue, loss, power and eigenvector is bitwise
+    the same."""
+    print_case("mode2d LU: graph replays vs direct launches")
+    s, info = S.gcpw(cem, **MODE_MINI)
+    nx, ny, nz = info["cells"]
+    m = cem.lib.cem_cross_section_x(s.h, 3, 0, ny, 0, nz)
+    assert m
+    try:
+        _solve_mode.sizes[m] = (ny * (nz + 1), (ny + 1) * nz)
+        seed = None
+        for k, f in enumerate((20e9, 21e9, 22e9, 30e9)):
+            start = None if seed is None else seed.ctypes.data_as(L._DP)
+            monkeypatch.setenv("COURANT_GRAPH", "0")
+            d = _solve_mode(cem, m, f, 1.8, start)
+            monkeypatch.delenv("COURANT_GRAPH")
+            g = _solve_mode(cem, m, f, 1.8, start)
+            assert d[0] == 0 and g[0] == 0
+            assert d[1:4] == g[1:4], (k, d[1:4], g[1:4])
+            assert np.array_equal(d[4], g[4]), k
+            seed = g[4].copy()
+    finally:
+        cem.lib.cem_mode2d_destroy(m)
+
+
 def test_four_layer_board_matches_reference(cem, ref, gpu_available):
     """The four-layer board generator at a reduced size (meanders on every copper layer, vias,
     6-pole Debye dielectrics, Huray SIBC, lumped ports): fields after 800 steps within 1e-4 of the
