@@ -505,7 +505,7 @@ __device__ __forceinline__ void fill_back(const Geo g, const zc *__restrict__ ba
 // with one workgroup barrier per chunk.
 template <int SF, int SB>
 __device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ band, const int *__restrict__ ipiv, zc *y,
-                           int chunk_f, int chunk_b, zc *ring, int *rpiv, long long *tm) {
+                           int chunk_f, int chunk_b, zc *ring, int *rpiv, const zc *zslot, long long *tm) {
     const int n = g.n, kl = g.bw, kv = 2 * g.bw;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const zc zero = mk(0.0, 0.0);
@@ -546,20 +546,16 @@ __device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ b
                     }
                     // the column's pivot and its multipliers for every slot, read from
                     // the ring together (clamped in-column offsets) before any use;
-                    // outside rows j+1 .. last a multiplier is zero, which leaves its
-                    // entry unchanged, so every slot updates without a branch
+                    // outside rows j+1 .. j+kl a multiplier is read from the zero entry (rows
+                    // past n are zeros in the ring), which leaves its entry unchanged, so
+                    // every slot updates without a branch or a mask
                     const zc *l = cur + (size_t)(j - cj0) * K;
-                    const int p = __builtin_amdgcn_readfirstlane(cp[j - cj0]), last = min(j + kl, n - 1);
+                    const int p = __builtin_amdgcn_readfirstlane(cp[j - cj0]);
                     zc lv[MAXS];
 #pragma unroll
                     for (int k = 0; k < MAXS; k++) {
-                        const int i = 64 * (jb + k) + lane;
-                        lv[k] = l[min(max(i - j - 1, 0), K - 1)];
-                    }
-#pragma unroll
-                    for (int k = 0; k < MAXS; k++) {
-                        const int i = 64 * (jb + k) + lane;
-                        if (!(k < S && i > j && i <= last)) lv[k] = zero;
+                        const int o = 64 * (jb + k) + lane - j - 1;
+                        lv[k] = *((unsigned)o < (unsigned)K ? l + o : zslot);
                     }
                     zc vj = bcast(v[0], j & 63);
                     if (p != j) {
@@ -622,20 +618,14 @@ __device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ b
                         pend = jb - S >= 0 ? y[64 * (jb - S) + lane] : zero;
                     }
                     const zc *u = cur + (size_t)(cj1 - 1 - j) * K;   // u[rr] = U(j - kv + rr, j), u[kv] = 1/U(j, j)
-                    // the reciprocal pivot and every slot's multiplier read together
-                    // (clamped in-column offsets); zero outside rows first .. j-1
-                    const int first = max(0, j - kv);
+                    // the reciprocal pivot and every slot's multiplier read together; outside
+                    // rows j-kv .. j-1 from the zero entry (rows below 0 are zeros in the ring)
                     const zc ukv = u[kv];
                     zc uv[MAXS];
 #pragma unroll
                     for (int k = 0; k < MAXS; k++) {
-                        const int i = 64 * (jb - k) + lane;
-                        uv[k] = u[min(max(i - (j - kv), 0), K - 1)];
-                    }
-#pragma unroll
-                    for (int k = 0; k < MAXS; k++) {
-                        const int i = 64 * (jb - k) + lane;
-                        if (!(k < S && i >= first && i < j)) uv[k] = zero;
+                        const int o = 64 * (jb - k) + lane - (j - kv);
+                        uv[k] = *((unsigned)o < (unsigned)kv ? u + o : zslot);
                     }
                     const zc xj = zmul(bcast(v[0], j & 63), ukv);
                     if (lane == (j & 63)) v[0] = xj;
@@ -662,11 +652,13 @@ __device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ b
 
 template <int SF, int SB>
 __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
-    extern __shared__ zc ring[];   // three chunks of band columns, then their pivots
-    int *rpiv = reinterpret_cast<int *>(ring + a.ring);
+    extern __shared__ zc ring[];   // three chunks of band columns, a zero entry, the pivots
+    zc *zslot = ring + a.ring;
+    int *rpiv = reinterpret_cast<int *>(ring + a.ring + 1);
     __shared__ double red[kInvThreads / 64];
     __shared__ long long tmr[8];
     if (threadIdx.x < 8) tmr[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *zslot = mk(0.0, 0.0);
     __syncthreads();
     const int n = a.g.n;
     zc *__restrict__ X = a.x;
@@ -708,7 +700,7 @@ __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
             Y[q] = X[q];
         }
         __syncthreads();
-        band_solve<SF, SB>(a.g, a.band, a.ipiv, a.y, a.chunk_f, a.chunk_b, ring, rpiv, a.timing ? tmr : nullptr);
+        band_solve<SF, SB>(a.g, a.band, a.ipiv, a.y, a.chunk_f, a.chunk_b, ring, rpiv, zslot, a.timing ? tmr : nullptr);
         __syncthreads();
         double mr = 0.0, mi = 0.0;
         #pragma unroll 8
@@ -894,7 +886,7 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     const int chunk_b = std::max(1, (int)(kRingBytes / (3 * sizeof(zc) * (kv + 1))));
     const int ring = std::max(3 * chunk_f * kl, 3 * chunk_b * (kv + 1));
     const int sf = (kl + 63) / 64 + 1, sb = (kv + 63) / 64 + 1;
-    const size_t inv_lds = (size_t)ring * sizeof(zc) + (size_t)3 * chunk_f * sizeof(int);
+    const size_t inv_lds = (size_t)(ring + 1) * sizeof(zc) + (size_t)3 * chunk_f * sizeof(int);
     if (kl + nb > 64 * kTrailRows || sf > 8 || inv_lds > kRingBytes + 4096) {
         set_error("GPU mode solver: port window too wide (nz)");
         return -1;
