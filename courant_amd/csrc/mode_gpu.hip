@@ -533,7 +533,10 @@ __device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ b
             if (wv == 0) {
                 const zc *cur = ring + (size_t)(ch % 3) * CH * K;
                 const int *cp = rpiv + (ch % 3) * CH;
-                for (int j = cj0; j < cj1; j++) {
+                // the block change sits outside the column loop: inside it every slot stays
+                // in its registers (a rotation on one path of the loop had the compiler
+                // copying all slots at every other column)
+                for (int j = cj0; j < cj1;) {
                     const int jb = j >> 6;
                     if ((j & 63) == 0 && jb > 0) {
                         // block jb-1 is final: out to y; slots move down, block jb-1+S enters
@@ -544,29 +547,32 @@ __device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ b
                         const int i = 64 * (jb + S) + lane;
                         pend = jb + S < nblk && i < n ? y[i] : zero;
                     }
-                    // the column's pivot and its multipliers for every slot, read from
-                    // the ring together (clamped in-column offsets) before any use;
-                    // outside rows j+1 .. j+kl a multiplier is read from the zero entry (rows
-                    // past n are zeros in the ring), which leaves its entry unchanged, so
-                    // every slot updates without a branch or a mask
-                    const zc *l = cur + (size_t)(j - cj0) * K;
-                    const int p = __builtin_amdgcn_readfirstlane(cp[j - cj0]);
-                    zc lv[MAXS];
+                    const int jend = min(cj1, 64 * jb + 64);
+                    for (; j < jend; j++) {
+                        // the column's pivot and its multipliers for every slot, read from
+                        // the ring together before any use; outside rows j+1 .. j+kl a
+                        // multiplier is read from the zero entry (rows past n are zeros in the
+                        // ring), which leaves its entry unchanged, so every slot updates
+                        // without a branch or a mask
+                        const zc *l = cur + (size_t)(j - cj0) * K;
+                        const int p = __builtin_amdgcn_readfirstlane(cp[j - cj0]);
+                        zc lv[MAXS];
 #pragma unroll
-                    for (int k = 0; k < MAXS; k++) {
-                        const int o = 64 * (jb + k) + lane - j - 1;
-                        lv[k] = *((unsigned)o < (unsigned)K ? l + o : zslot);
-                    }
-                    zc vj = bcast(v[0], j & 63);
-                    if (p != j) {
-                        const unsigned mp = 1u << ((p >> 6) - jb);
-                        const zc vp = bcast(slot_get(v, mp), p & 63);
-                        if (lane == (j & 63)) v[0] = vp;
-                        if (lane == (p & 63)) slot_put(v, mp, vj);
-                        vj = vp;
-                    }
+                        for (int k = 0; k < MAXS; k++) {
+                            const int o = 64 * (jb + k) + lane - j - 1;
+                            lv[k] = *((unsigned)o < (unsigned)K ? l + o : zslot);
+                        }
+                        zc vj = bcast(v[0], j & 63);
+                        if (p != j) {
+                            const unsigned mp = 1u << ((p >> 6) - jb);
+                            const zc vp = bcast(slot_get(v, mp), p & 63);
+                            if (lane == (j & 63)) v[0] = vp;
+                            if (lane == (p & 63)) slot_put(v, mp, vj);
+                            vj = vp;
+                        }
 #pragma unroll
-                    for (int k = 0; k < MAXS; k++) v[k] = zfms(v[k], lv[k], vj);
+                        for (int k = 0; k < MAXS; k++) v[k] = zfms(v[k], lv[k], vj);
+                    }
                 }
             } else if (ch + 2 < nchunk) {
                 fill_fwd(g, band, ipiv, ring, rpiv, CH, ch + 2, threadIdx.x - 64, kInvThreads - 64);
@@ -607,7 +613,7 @@ __device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ b
             const long long c0 = tm ? clock64() : 0;
             if (wv == 0) {
                 const zc *cur = ring + (size_t)(ch % 3) * CH * K;
-                for (int j = cj1 - 1; j >= cj0; j--) {
+                for (int j = cj1 - 1; j >= cj0;) {   // block change outside the column loop
                     const int jb = j >> 6;
                     if ((j & 63) == 63 && jb < top) {
                         // block jb+1 is final: out to y; slots move up, block jb+1-S enters
@@ -617,20 +623,23 @@ __device__ __forceinline__ void band_solve(const Geo g, const zc *__restrict__ b
                         slot_put(v, 1u << (S - 1), pend);
                         pend = jb - S >= 0 ? y[64 * (jb - S) + lane] : zero;
                     }
-                    const zc *u = cur + (size_t)(cj1 - 1 - j) * K;   // u[rr] = U(j - kv + rr, j), u[kv] = 1/U(j, j)
-                    // the reciprocal pivot and every slot's multiplier read together; outside
-                    // rows j-kv .. j-1 from the zero entry (rows below 0 are zeros in the ring)
-                    const zc ukv = u[kv];
-                    zc uv[MAXS];
+                    const int jend = max(cj0, 64 * jb) - 1;
+                    for (; j > jend; j--) {
+                        const zc *u = cur + (size_t)(cj1 - 1 - j) * K;   // u[rr] = U(j - kv + rr, j), u[kv] = 1/U(j, j)
+                        // the reciprocal pivot and every slot's multiplier read together; outside
+                        // rows j-kv .. j-1 from the zero entry (rows below 0 are zeros in the ring)
+                        const zc ukv = u[kv];
+                        zc uv[MAXS];
 #pragma unroll
-                    for (int k = 0; k < MAXS; k++) {
-                        const int o = 64 * (jb - k) + lane - (j - kv);
-                        uv[k] = *((unsigned)o < (unsigned)kv ? u + o : zslot);
+                        for (int k = 0; k < MAXS; k++) {
+                            const int o = 64 * (jb - k) + lane - (j - kv);
+                            uv[k] = *((unsigned)o < (unsigned)kv ? u + o : zslot);
+                        }
+                        const zc xj = zmul(bcast(v[0], j & 63), ukv);
+                        if (lane == (j & 63)) v[0] = xj;
+#pragma unroll
+                        for (int k = 0; k < MAXS; k++) v[k] = zfms(v[k], uv[k], xj);
                     }
-                    const zc xj = zmul(bcast(v[0], j & 63), ukv);
-                    if (lane == (j & 63)) v[0] = xj;
-#pragma unroll
-                    for (int k = 0; k < MAXS; k++) v[k] = zfms(v[k], uv[k], xj);
                 }
             } else if (ch + 2 < nchunk) {
                 fill_back(g, band, ring, CH, ch + 2, threadIdx.x - 64, kInvThreads - 64);
