@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "devmem.hpp"
@@ -785,13 +786,37 @@ struct HMem {
     }
 };
 
-// dynamic LDS beyond 64 KiB needs the per-kernel attribute (gfx950: up to 160 KiB)
 template <int SF, int SB>
 int run_inverse(size_t lds, hipStream_t st, const InvArgs &ia) {
-    if (hipFuncSetAttribute((const void *)k_inverse<SF, SB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+    hipLaunchKernelGGL((k_inverse<SF, SB>), dim3(1), dim3(kInvThreads), lds, st, ia);
+    return 0;
+}
+
+// Dynamic LDS beyond 64 KiB needs the per-kernel attribute (gfx950: up to 160 KiB).
+// It is set once per device, to the largest size any solve launches with, under a
+// lock: the port sweeps solve on several host threads at once, and setting a kernel's
+// attribute on one thread while another launches that kernel is not safe.
+constexpr size_t kTrailLdsMax = 60 * 1024;          // the panel width keeps (kl + nb) nb entries within it
+constexpr size_t kInvLdsMax = kRingBytes + 4096;    // the solver rejects windows needing more
+int set_lds_attributes() {
+    static std::mutex mu;
+    static std::vector<char> done;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)done.size() <= dev) done.resize(dev + 1, 0);
+    if (done[dev]) return 0;
+    const void *inv[] = {(const void *)k_inverse<2, 2>, (const void *)k_inverse<2, 3>, (const void *)k_inverse<3, 4>,
+                         (const void *)k_inverse<3, 5>, (const void *)k_inverse<4, 6>, (const void *)k_inverse<4, 7>,
+                         (const void *)k_inverse<5, 8>, (const void *)k_inverse<5, 9>, (const void *)k_inverse<6, 10>,
+                         (const void *)k_inverse<6, 11>, (const void *)k_inverse<7, 12>, (const void *)k_inverse<7, 13>,
+                         (const void *)k_inverse<8, 14>, (const void *)k_inverse<8, 15>};
+    for (const void *f : inv)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInvLdsMax) != hipSuccess) return -1;
+    if (hipFuncSetAttribute((const void *)k_lu_trail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTrailLdsMax) !=
         hipSuccess)
         return -1;
-    hipLaunchKernelGGL((k_inverse<SF, SB>), dim3(1), dim3(kInvThreads), lds, st, ia);
+    done[dev] = 1;
     return 0;
 }
 
@@ -888,7 +913,7 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     // registers; the trailing walk per column grows with nb) and the panel
     // (kl + nb rows x nb columns) within 60 KiB of LDS for the trailing update
     int nb = 16;
-    while (nb > 1 && (size_t)(kl + nb) * nb * sizeof(zc) > 60 * 1024) nb /= 2;
+    while (nb > 1 && (size_t)(kl + nb) * nb * sizeof(zc) > kTrailLdsMax) nb /= 2;
     // triangular solves: three slots of chunk columns in the LDS ring; register
     // window of S = ceil(K / 64) + 1 blocks of 64 entries
     const int chunk_f = std::max(1, (int)(kRingBytes / (3 * sizeof(zc) * kl + 3 * sizeof(int))));
@@ -896,8 +921,12 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     const int ring = std::max(3 * chunk_f * kl, 3 * chunk_b * (kv + 1));
     const int sf = (kl + 63) / 64 + 1, sb = (kv + 63) / 64 + 1;
     const size_t inv_lds = (size_t)(ring + 1) * sizeof(zc) + (size_t)3 * chunk_f * sizeof(int);
-    if (kl + nb > 64 * kTrailRows || sf > 8 || inv_lds > kRingBytes + 4096) {
+    if (kl + nb > 64 * kTrailRows || sf > 8 || inv_lds > kInvLdsMax) {
         set_error("GPU mode solver: port window too wide (nz)");
+        return -1;
+    }
+    if (set_lds_attributes()) {
+        set_error("GPU mode solver: cannot set the kernels' LDS limits");
         return -1;
     }
     const int ncoef = p.ny + (p.ny + 1) + p.nz + (p.nz + 1) + g.nu + g.nv + g.nn;
@@ -974,11 +1003,6 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     // port).  Replayed from a graph captured on the first solve of a geometry, so that
     // solves running at once on other host threads (the port sweeps) do not queue
     // behind thousands of launch submissions; COURANT_GRAPH=0 launches directly.
-    {
-        const size_t lds_max = (size_t)(std::min(nb - 1 + kl, n - 1) + 1) * std::min(nb, n) * sizeof(zc);
-        MG_CHECK(hipFuncSetAttribute((const void *)k_lu_trail, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds_max));
-    }
     auto launch_lu = [&]() {
         for (int j0 = 0; j0 < n; j0 += nb) {
             const int j1 = std::min(n, j0 + nb);

@@ -162,8 +162,14 @@ static int sweep_port(cem_sim *s, ModePort &p, cem_mode2d *xs, int q0, int step,
         double ng = cem_mode2d_beta(xs) / (2 * M_PI * mode_freq_hint(xs) / CEM_C0);
         std::memcpy(start.data(), cem_mode2d_eu(xs), 2 * ny * (nz + 1) * sizeof(double));
         std::memcpy(start.data() + 2 * ny * (nz + 1), cem_mode2d_ev(xs), 2 * (ny + 1) * nz * sizeof(double));
-        if (cem_mode2d_solve(xs, fw, ng, start.data(), 300, 1e-13)) {
+        const int rc = cem_mode2d_solve(xs, fw, ng, start.data(), 300, 1e-13);
+        if (rc > 0) {
             set_error("mode port: eigenmode did not converge at an output frequency");
+            return -1;
+        }
+        if (rc < 0) {   // the solver's own error (device or memory), kept for the caller
+            const std::string e = std::string("mode port: ") + cem_last_error();
+            set_error(e.c_str());
             return -1;
         }
         if (q == q0 && !write_q0) continue;
