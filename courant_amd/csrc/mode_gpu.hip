@@ -1027,6 +1027,10 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     } else {
         const void *key[3] = {M.band.p, M.ipiv.p, M.info.p};
         if (!M.lu || M.lu_n != n || M.lu_bw != kl || std::memcmp(key, M.lu_key, sizeof key) != 0) {
+            // one capture at a time in the process (once per context; the sweeps' clones
+            // reach their first solve together)
+            static std::mutex capture_mu;
+            std::lock_guard<std::mutex> lock(capture_mu);
             if (M.lu) (void)hipGraphExecDestroy(M.lu);
             M.lu = nullptr;
             hipGraph_t graph = nullptr;
