@@ -88,17 +88,34 @@ int cem::modeports_finalize(cem_sim *s) {
         const cem_mode_port_def &d = p.def;
         p.xs = cem_cross_section_x(s, d.i_mon, d.j0, d.j1, d.k0, d.k1);
         if (!p.xs) return -1;
-        bool ok = false;
-        for (int t = 0; t < 5 && !ok; t++) {
-            if (!cem_mode2d_solve(p.xs, warp(s, d.f0), p.def.neff0 * tries[t], nullptr, 300, 1e-13)) {
-                ok = true;
+    }
+    // the eigen-solve at warp(f0), retried at scaled effective indices; the ports are
+    // independent, so with the GPU solver they solve at once, one host thread and one
+    // device context each, as the frequency sweeps do
+    auto solve_f0 = [s](ModePort &p) {
+        for (int t = 0; t < 5; t++)
+            if (!cem_mode2d_solve(p.xs, warp(s, p.def.f0), p.def.neff0 * tries[t], nullptr, 300, 1e-13)) {
                 p.def.neff0 *= tries[t];
+                return true;
             }
-        }
+        return false;
+    };
+    std::vector<char> solved(s->mport.size(), 0);
+    if (mode2d_on_gpu() && s->mport.size() > 1) {
+        std::vector<std::thread> th;
+        for (size_t k = 0; k < s->mport.size(); k++)
+            th.emplace_back([&solve_f0, &solved, s, k]() { solved[k] = solve_f0(s->mport[k]); });
+        for (auto &t : th) t.join();
+    } else {
+        for (size_t k = 0; k < s->mport.size(); k++) solved[k] = solve_f0(s->mport[k]);
+    }
+    for (char ok : solved)
         if (!ok) {
             set_error("mode port: eigenmode did not converge");
             return -1;
         }
+    for (auto &p : s->mport) {
+        const cem_mode_port_def &d = p.def;
         const int ny = d.j1 - d.j0, nz = d.k1 - d.k0;
         const double *eu = cem_mode2d_eu(p.xs), *ev = cem_mode2d_ev(p.xs);
         double vmax = 0;
