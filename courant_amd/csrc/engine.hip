@@ -2318,6 +2318,79 @@ extern "C" int cem_amd_s2_renormalize(double *S, int nf, const double *z1, const
     return rc;
 }
 
+// Allocation-free renormalisation.  Its device staging (S, then z1, then z2) is
+// this file-scope array rather than memory from the pool: kNoAllocMaxNf
+// frequencies at most, 10 doubles each (5 MiB, held by every GPU the library
+// runs on until the process ends).  Every call on one GPU shares that GPU's
+// array, so calls that overlap on one GPU race; each call overwrites all of
+// the staging it reads, so nothing has to start at zero.
+constexpr int kNoAllocMaxNf = 65536;
+static __device__ __align__(256) double scratch[10 * kNoAllocMaxNf];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_dev;   // device address of `scratch`, per device
+
+// Device address of this device's copy of `scratch`, looked up once per device
+// (with that device current) by hipGetSymbolAddress; the array's own name is a
+// host-side stand-in and never a device address.
+double *scratch_on(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            char m[160];
+            std::snprintf(m, sizeof m, "s2 renormalize (no-alloc): hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+            set_error(m);
+            return nullptr;
+        }
+        g_scratch_dev[dev] = (double *)p;
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
+
+extern "C" int cem_amd_s2_renormalize_noalloc(double *S, int nf, const double *z1, const double *z2, double zref) {
+    if (!S || !z1 || !z2 || nf < 0) {
+        set_error("s2 renormalize (no-alloc): bad arguments");
+        return -1;
+    }
+    if (nf > kNoAllocMaxNf) {
+        char m[160];
+        std::snprintf(m, sizeof m, "s2 renormalize (no-alloc): %d frequencies exceed the fixed staging of %d", nf,
+                      kNoAllocMaxNf);
+        set_error(m);
+        return -1;
+    }
+    if (nf == 0) return 0;
+    int ndev = 0, dev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1 || hipGetDevice(&dev) != hipSuccess) {
+        set_error("no GPU device available for the courant engine");
+        return -1;
+    }
+    double *base = scratch_on(dev);
+    if (!base) return -1;
+    const size_t n = (size_t)nf;
+    double *dS = base, *d1 = base + 8 * n, *d2 = base + 9 * n;
+    hipStream_t st = nullptr;   // the device's null stream: no stream is created
+    if (hipMemcpyAsync(dS, S, 8 * n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d1, z1, n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d2, z2, n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_error("s2 renormalize (no-alloc): upload failed");
+        return -1;
+    }
+    hipLaunchKernelGGL(k_s2_renormalize, dim3((nf + 127) / 128), dim3(128), 0, st, dS, nf, d1, d2, zref);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(S, dS, 8 * n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("s2 renormalize (no-alloc): device failure");
+        return -1;
+    }
+    return 0;
+}
+
 // Stream-copy bandwidth of this GPU: copies `bytes` (rounded down to 16 B) between
 // two device buffers `iters` times and returns the best GB/s (read + write bytes).
 extern "C" int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e) {

@@ -131,6 +131,7 @@ AMD_SIGNATURES = {
     "cem_amd_step_device": (_I, [_P, _L]),
     "cem_amd_copy_bandwidth": (_D, [_S, _I]),
     "cem_amd_s2_renormalize": (_I, [_DP, _I, _DP, _DP, _D]),
+    "cem_amd_s2_renormalize_noalloc": (_I, [_DP, _I, _DP, _DP, _D]),
     "cem_amd_launch_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 

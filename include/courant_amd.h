@@ -37,6 +37,16 @@ CEM_API int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *
  * over frequencies.  Returns 0, or -1 with cem_last_error() set. */
 CEM_API int cem_amd_s2_renormalize(double *S, int nf, const double *z1, const double *z2, double zref);
 
+/* The same renormalisation without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Its device staging
+ * is a fixed array in the library (5 MiB per GPU, held until the process
+ * ends), so nf is limited to 65536; larger nf returns -1 with an error.  The
+ * array is shared by every call on one GPU: calls that overlap on one GPU
+ * race.  Works on the device's null stream.  Returns 0, or -1 with
+ * cem_last_error() set. */
+CEM_API int cem_amd_s2_renormalize_noalloc(double *S, int nf, const double *z1, const double *z2, double zref);
+
 /* Launch shape of the stencil kernels (x planes walked per thread by the H and
  * E updates).  Returns 1 once the autotuner (or COURANT_ISEG_H / _E) has fixed
  * it, 0 before, -1 if the simulation has not run on the GPU. */
