@@ -78,29 +78,37 @@ constexpr int kGatherBlock = 1024;    // steps per captured chunk (divides kSeri
 // ---------------------------------------------------------------------------
 // kernel argument blocks
 // ---------------------------------------------------------------------------
+// Device grid.  Every field array is (nx+1) x (ny+1) rows of kp floats, k
+// fastest, where the row pitch kp is nz+1 rounded up to a multiple of 4: each
+// row starts on a 16-byte boundary, so a lane of the stencil kernels moves the
+// four consecutive k points of one "group" with a single 128-bit access and a
+// wavefront moves 1 KiB of an array per instruction.  The kp - nz - 1 padding
+// points of a row hold zero.  Plan indices (i si + j sj + k of the reference)
+// are translated to i si_dev + j kp + k when they are uploaded.
 struct Grid {
-    int nx, ny, nz, S;      // S = (ny+1)(nz+1) points per x plane
-    long long si, sj, N;
-    int nchunk;             // ceil(S / 64) wavefront chunks per plane
+    int nx, ny, nz;
+    int kp, ng;             // row pitch in floats, float4 groups per row (kp / 4)
+    int S4;                 // groups per x plane: (ny + 1) ng
+    long long si, sj, N;    // device strides (si = (ny+1) kp, sj = kp), points per component
+    long long si4, N4;      // si / 4, N / 4
 };
 
-struct Pml {                // CPML per derivative axis a: slot per index (-1 outside a slab)
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// CPML of one kernel (H or E): per derivative axis a the slab slot of each index
+// along a (-1 outside the slabs) and the recursion coefficients b, c.  The psi
+// arrays are laid out for the 128-bit lanes:
+//   x derivative: [slot][j][kp]                float4 index  slot S4 + p
+//   y derivative: [slot][i][kp]                float4 index  (slot (nx+1) + i) ng + gk
+//   z derivative: per (i, j) column, the k groups that hold slab points, one
+//                 float4 each in k order       float4 index  (i (ny+1) + j) zg + zo[gk]
+// (state export converts to and from the reference's slab-major order).
+struct Pml {
     const int *slot[3];
     const float *b[3], *c[3];
-    long long per[3];       // points per slab plane
-    int zpl;                // slab planes of the z-derivative arrays (both faces)
+    const int *zo;          // per k group: its place in a column's z layout, -1 outside
+    int zg;                 // k groups per column in the z layout
 };
-
-// psi of the x and y derivatives keeps the reference's slab-major order
-// (slot * per + in-plane index), which is k-fastest and so coalesced for the
-// column threads.  The z-derivative arrays are stored slot-fastest on the
-// device, ((i (ny+1) + j) zpl + slot): the lanes of a wavefront that lie in a
-// z slab then touch consecutive words instead of one slab plane each
-// (measured: z-face CPML cost 92 us of H and 105 us of E on 600x400x160 in
-// slab-major order).  State export transposes to the reference order.
-__device__ __forceinline__ long long psi_z(const Grid &g, const Pml &m, int i, int j, int sz) {
-    return ((long long)i * (g.ny + 1) + j) * m.zpl + sz;
-}
 
 struct HArgs {
     Grid g;
@@ -113,7 +121,7 @@ struct HArgs {
     int has_pml;
 };
 
-// Per-edge physics parameters, resident on the device so the E kernel's
+// Per-edge physics parameters, resident on the device so the post kernel's
 // argument block stays small.
 struct PostParams {
     int ndisp, npole, nsb, nterm;
@@ -123,17 +131,30 @@ struct PostParams {
     float om[kSibcTerms];   // 1 - e_k, from -expm1(-a_k dt)
 };
 
+constexpr int kFusedPoles = 8;     // most Debye poles the E kernel updates itself
+constexpr int kDebyeRows = 256;    // most (beta_p / dt) material rows it holds in LDS
+
 struct EArgs {
     Grid g;
     float *E;               // 3N
     const float *H;         // 3N
     const float *ca, *cb;   // 3N each (uncompressed coefficients)
-    const unsigned *cidx;   // N: packed 8-bit coefficient indices of Ex, Ey, Ez (compressed)
-    const float2 *ctab;     // 3 x 256 (ca, cb) pairs (compressed)
+    const unsigned *cidx;   // N: 8-bit classes of Ex, Ey, Ez (compressed coefficients)
+    const float2 *ctab;     // 3 x 256 (ca, cb) per class
+    const int *crow;        // 3 x 256 Debye material row of a class; -1: not updated here
     const float *es[3];     // ide per axis
     Pml pml;
     float *psi[3][2];       // psiE[c][w]
     int has_pml;
+    // Debye ADE of the dispersive edges this kernel updates.  J is held over a box
+    // of the grid, [i - bi0][j - bj0][comp][k - 4 bg0][pole], the poles of a point
+    // padded to an even count: a lane reads its points' currents with 8- or 16-byte
+    // loads, and one (i, j) row of the box is one contiguous block
+    float *J;
+    const float *btab;      // per material row: beta_p / dt for each pole
+    int nrow;
+    int bi0, bi1, bj0, bj1, bg0, bg1;
+    float kc[kFusedPoles], km1[kFusedPoles];
 };
 
 // Device-side step and series-row counters: every per-step kernel argument is
@@ -167,27 +188,33 @@ __device__ __forceinline__ void waveforms(const Waveform &w, long long n, float 
 // Post-phase edge lists.  One thread per edge runs every step that applies to
 // its edge, in the reference order; an edge can be in several lists (a
 // dielectric edge tangential to a PEC outer face is both dispersive and a
-// surface edge), so the lists are linked: threads [0, ndisp) take the
+// surface edge), so the lists are linked: threads [0, ndisp) take the listed
 // dispersive edges (with their SIBC slot and source slot, if any), then nsbo
 // surface edges that are not dispersive, then nxs remaining source edges.
-// Indices are global E indices comp*N + q.
+// Indices are device E indices comp*N + q.  When the E kernel updates the
+// Debye currents itself, only the dispersive edges it cannot take (surface
+// edges, mode-port source edges, edges on the x = max face) are listed here;
+// their J entries then live in the E kernel's box and d_jb gives each one's
+// index there.
 struct PostArgs {
     float *E;               // 3N
     const float *H;         // 3N
-    long long N, si, sj;
+    long long N, si, sj;    // device strides
     const PostParams *pp;
     int ndisp, nsb, nsbo, nxs;
     const unsigned *d_idx;
-    const unsigned *d_mat;  // per dispersive edge: row of dtab (bits 0-29), has an SIBC slot
-                            // (bit 31), has a source slot (bit 30)
+    const unsigned *d_mat;  // per listed dispersive edge: row of dtab (bits 0-29), has an SIBC
+                            // slot (bit 31), has a source slot (bit 30)
     const float *dtab;      // per material: cb, then beta/dt per pole
     int dstride;            // 1 + npole
-    // the few dispersive edges that are also SIBC or source edges: ascending edge
-    // numbers with their slots (found by binary search from the flag bits)
+    // the few listed dispersive edges that are also SIBC or source edges: ascending
+    // edge numbers with their slots (found by binary search from the flag bits)
     const int *dsb_t, *dsb_u, *dsrc_t, *dsrc_x;
     int ndsb, ndsrc;
-    float *J;               // npole * ndisp (pole-major, reference order)
-    float *Eold;            // E of each dispersive edge before this step's E update
+    float *J;               // pole p of listed edge t at (d_jb ? d_jb[t] : t) + p * jstride
+    const long long *d_jb;
+    long long jstride;
+    float *Eold;            // E of each listed dispersive edge before this step's E update
     const unsigned *sb_idx; // per SIBC slot (reference order)
     const unsigned char *face;
     float *Y;               // nterm * nsb (term-major)
@@ -242,15 +269,59 @@ __device__ __forceinline__ int xcd_tile(int wg, int G) {
     return x * q + min(x, r) + (wg >> 3);
 }
 
-// One thread per (j,k) column of an i-segment; the E values of plane i+1 are
-// carried in registers into the next iteration.
+// Lane width W: each thread of the stencil kernels owns W consecutive k points
+// of one (i, j) row (a "group"; W divides kp), moved with one W-wide load or
+// store per array.
+template <int W> struct LaneVec;
+template <> struct LaneVec<1> { typedef float T; };
+template <> struct LaneVec<2> { typedef float T __attribute__((ext_vector_type(2))); };
+template <> struct LaneVec<4> { typedef float T __attribute__((ext_vector_type(4))); };
+
+template <int W>
+__device__ __forceinline__ void ldw(const float *__restrict__ p, float (&x)[W]) {
+    typedef typename LaneVec<W>::T V;
+    const V v = *(const V *)p;
+    if constexpr (W == 1) {
+        x[0] = v;
+    } else {
+#pragma unroll
+        for (int e = 0; e < W; e++) x[e] = v[e];
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void stw(float *__restrict__ p, const float (&x)[W]) {
+    typedef typename LaneVec<W>::T V;
+    V v;
+    if constexpr (W == 1) {
+        v = x[0];
+    } else {
+#pragma unroll
+        for (int e = 0; e < W; e++) v[e] = x[e];
+    }
+    *(V *)p = v;
+}
+
+template <int W>
+__device__ __forceinline__ void zerow(float (&x)[W]) {
+#pragma unroll
+    for (int e = 0; e < W; e++) x[e] = 0.f;
+}
+
+// H update (sim.c:579-606) fused with the CPML psi_H recursions (sim.c:680-754).
+// One thread per group of an i-segment; the k + 1 neighbour of the group's last
+// point comes from one 32-bit load, the E of plane i+1 is carried in registers
+// into the next iteration.  Each point is updated with the reference's
+// expression in the reference's order.
+template <int W>
 __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, int iseg, int npblk) {
     const Grid g = a.g;
+    const int ngw = g.kp / W;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int pblk = tile % npblk, seg = tile / npblk;
     const int p = pblk * kBlock + threadIdx.x;
-    if (p >= g.S) return;
-    const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
+    if (p >= (g.ny + 1) * ngw) return;
+    const int j = p / ngw, gk = p - j * ngw, k0 = W * gk;
     const int ib = i0 + seg * iseg;
     const int ie = min(i1, ib + iseg);
     const float *__restrict__ Ex = a.E;
@@ -259,95 +330,174 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
     float *__restrict__ Hx = a.H;
     float *__restrict__ Hy = a.H + g.N;
     float *__restrict__ Hz = a.H + 2 * g.N;
-    const bool jin = j < g.ny, kin = k < g.nz;
-    const float hy = jin ? a.hs[1][j] : 0.f, hz = kin ? a.hs[2][k] : 0.f;
+    const bool jin = j < g.ny;
+    const float hy = jin ? a.hs[1][j] : 0.f;
+    bool kin[W], inz[W];
+    float hz[W], bz[W], cz[W];
     const int sy = a.has_pml && jin ? a.pml.slot[1][j] : -1;
-    const int sz = a.has_pml && kin ? a.pml.slot[2][k] : -1;
     const float by = sy >= 0 ? a.pml.b[1][j] : 0.f, cy = sy >= 0 ? a.pml.c[1][j] : 0.f;
-    const float bz = sz >= 0 ? a.pml.b[2][k] : 0.f, cz = sz >= 0 ? a.pml.c[2][k] : 0.f;
-    long long q = (long long)ib * g.si + p;
-    float ey = Ey[q], ez = Ez[q];
+    const int zo = a.has_pml ? a.pml.zo[k0 / 4] : -1;
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+        const int k = k0 + e;
+        kin[e] = k < g.nz;
+        hz[e] = kin[e] ? a.hs[2][k] : 0.f;
+        inz[e] = zo >= 0 && kin[e] && a.pml.slot[2][k] >= 0;
+        bz[e] = inz[e] ? a.pml.b[2][k] : 0.f;
+        cz[e] = inz[e] ? a.pml.c[2][k] : 0.f;
+    }
+    const long long kn = k0 + W < g.kp ? W : 0;   // offset of the next group's first point
+    long long q = (long long)ib * g.si + (long long)j * g.sj + k0;
+    float ey[W], ez[W];
+    ldw<W>(Ey + q, ey);
+    ldw<W>(Ez + q, ez);
     for (int i = ib; i < ie; i++, q += g.si) {
         const bool iin = i < g.nx;
-        const float ex = Ex[q];
-        const float ey_k = kin ? Ey[q + 1] : 0.f, ex_k = kin ? Ex[q + 1] : 0.f;
-        const float ez_j = jin ? Ez[q + g.sj] : 0.f, ex_j = jin ? Ex[q + g.sj] : 0.f;
-        const float ey_i = iin ? Ey[q + g.si] : 0.f, ez_i = iin ? Ez[q + g.si] : 0.f;
-        const float dz_ey = ey_k - ey, dy_ez = ez_j - ez;
-        const float dz_ex = ex_k - ex, dy_ex = ex_j - ex;
-        const float dx_ez = ez_i - ez, dx_ey = ey_i - ey;
+        float ex[W], ezj[W], exj[W], eyi[W], ezi[W], hxv[W], hyv[W], hzv[W];
+        ldw<W>(Ex + q, ex);
+        const float eyk = Ey[q + kn], exk = Ex[q + kn];
+        if (jin) {
+            ldw<W>(Ez + q + g.sj, ezj);
+            ldw<W>(Ex + q + g.sj, exj);
+        } else {
+            zerow<W>(ezj);
+            zerow<W>(exj);
+        }
+        if (iin) {
+            ldw<W>(Ey + q + g.si, eyi);
+            ldw<W>(Ez + q + g.si, ezi);
+        } else {
+            zerow<W>(eyi);
+            zerow<W>(ezi);
+        }
         const float hx = iin ? a.hs[0][i] : 0.f;
         const int sx = a.has_pml && iin ? a.pml.slot[0][i] : -1;
-        // psi of the y and z slabs, loaded for every lane through clamped offsets
-        // (offset 0 lies inside every psi array) so the loads are not behind
-        // lane-divergent branches; stores stay conditional
-        const long long oy = sy >= 0 ? sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k : 0;
-        const long long oz = sz >= 0 ? psi_z(g, a.pml, i, j, sz) : 0;
-        const float v0y = a.has_pml ? a.psi[0][0][oy] : 0.f, v0z = a.has_pml ? a.psi[0][1][oz] : 0.f;
-        const float v1z = a.has_pml ? a.psi[1][0][oz] : 0.f, v2y = a.has_pml ? a.psi[2][1][oy] : 0.f;
-        if (jin && kin) {   // Hx: all i
-            float h = Hx[q] - (dy_ez * hy - dz_ey * hz);
-            if (sy >= 0) {
-                const float v = by * v0y + cy * dy_ez;
-                a.psi[0][0][oy] = v;
-                h -= a.dth * v;
-            }
-            if (sz >= 0) {
-                const float v = bz * v0z + cz * dz_ey;
-                a.psi[0][1][oz] = v;
-                h += a.dth * v;
-            }
-            Hx[q] = h;
+        ldw<W>(Hx + q, hxv);
+        ldw<W>(Hy + q, hyv);
+        ldw<W>(Hz + q, hzv);
+        // psi of the slabs that hold this group (x: the plane, y: the row, z: the lane)
+        float py0[W], py2[W], px1[W], px2[W], pz0[W], pz1[W];
+        float bx = 0.f, cx = 0.f;
+        long long oy = 0, ox = 0, oz = 0;
+        if (sy >= 0) {
+            oy = ((long long)sy * (g.nx + 1) + i) * g.kp + k0;
+            ldw<W>(a.psi[0][0] + oy, py0);
+            ldw<W>(a.psi[2][1] + oy, py2);
         }
-        if (iin && kin) {   // Hy: j <= ny
-            float h = Hy[q] - (dz_ex * hz - dx_ez * hx);
-            if (sx >= 0) {
-                float *ps = a.psi[1][1] + sx * a.pml.per[0] + p;
-                const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_ez;
-                *ps = v;
-                h += a.dth * v;
-            }
-            if (sz >= 0) {
-                const float v = bz * v1z + cz * dz_ex;
-                a.psi[1][0][oz] = v;
-                h -= a.dth * v;
-            }
-            Hy[q] = h;
+        if (sx >= 0) {
+            ox = (long long)sx * g.si + (long long)j * g.sj + k0;
+            ldw<W>(a.psi[1][1] + ox, px1);
+            ldw<W>(a.psi[2][0] + ox, px2);
+            bx = a.pml.b[0][i];
+            cx = a.pml.c[0][i];
         }
-        if (iin && jin) {   // Hz: k <= nz
-            float h = Hz[q] - (dx_ey * hx - dy_ex * hy);
-            if (sx >= 0) {
-                float *ps = a.psi[2][0] + sx * a.pml.per[0] + p;
-                const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_ey;
-                *ps = v;
-                h -= a.dth * v;
-            }
-            if (sy >= 0) {
-                const float v = by * v2y + cy * dy_ex;
-                a.psi[2][1][oy] = v;
-                h += a.dth * v;
-            }
-            Hz[q] = h;
+        if (zo >= 0) {
+            oz = (((long long)i * (g.ny + 1) + j) * a.pml.zg + zo) * 4 + (k0 & 3);
+            ldw<W>(a.psi[0][1] + oz, pz0);
+            ldw<W>(a.psi[1][0] + oz, pz1);
         }
-        ey = ey_i;
-        ez = ez_i;
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+            const float ey_k = e + 1 < W ? ey[(e + 1) % W] : eyk, ex_k = e + 1 < W ? ex[(e + 1) % W] : exk;
+            const float dz_ey = ey_k - ey[e], dy_ez = ezj[e] - ez[e];
+            const float dz_ex = ex_k - ex[e], dy_ex = exj[e] - ex[e];
+            const float dx_ez = ezi[e] - ez[e], dx_ey = eyi[e] - ey[e];
+            if (jin && kin[e]) {   // Hx: all i
+                float h = hxv[e] - (dy_ez * hy - dz_ey * hz[e]);
+                if (sy >= 0) {
+                    const float v = by * py0[e] + cy * dy_ez;
+                    py0[e] = v;
+                    h -= a.dth * v;
+                }
+                if (inz[e]) {
+                    const float v = bz[e] * pz0[e] + cz[e] * dz_ey;
+                    pz0[e] = v;
+                    h += a.dth * v;
+                }
+                hxv[e] = h;
+            }
+            if (iin && kin[e]) {   // Hy: j <= ny
+                float h = hyv[e] - (dz_ex * hz[e] - dx_ez * hx);
+                if (sx >= 0) {
+                    const float v = bx * px1[e] + cx * dx_ez;
+                    px1[e] = v;
+                    h += a.dth * v;
+                }
+                if (inz[e]) {
+                    const float v = bz[e] * pz1[e] + cz[e] * dz_ex;
+                    pz1[e] = v;
+                    h -= a.dth * v;
+                }
+                hyv[e] = h;
+            }
+            if (iin && jin) {      // Hz: k <= nz
+                float h = hzv[e] - (dx_ey * hx - dy_ex * hy);
+                if (sx >= 0) {
+                    const float v = bx * px2[e] + cx * dx_ey;
+                    px2[e] = v;
+                    h -= a.dth * v;
+                }
+                if (sy >= 0) {
+                    const float v = by * py2[e] + cy * dy_ex;
+                    py2[e] = v;
+                    h += a.dth * v;
+                }
+                hzv[e] = h;
+            }
+        }
+        if (jin) stw<W>(Hx + q, hxv);
+        if (iin) stw<W>(Hy + q, hyv);
+        if (iin && jin) stw<W>(Hz + q, hzv);
+        // psi entries of points not updated are stored back unchanged
+        if (sy >= 0) {
+            stw<W>(a.psi[0][0] + oy, py0);
+            stw<W>(a.psi[2][1] + oy, py2);
+        }
+        if (sx >= 0) {
+            stw<W>(a.psi[1][1] + ox, px1);
+            stw<W>(a.psi[2][0] + ox, px2);
+        }
+        if (zo >= 0) {
+            stw<W>(a.psi[0][1] + oz, pz0);
+            stw<W>(a.psi[1][0] + oz, pz1);
+        }
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+            ey[e] = eyi[e];
+            ez[e] = ezi[e];
+        }
     }
 }
 
-// One thread per (j,k) column of an i-segment; the H values of plane i-1 are
-// carried in registers from the previous iteration.
-template <bool kCmp>
+// E update (sim.c:608-637) fused with CPML psi_E (sim.c:756-832) and, for NP > 0,
+// with the Debye ADE of the dispersive edges whose class carries a material row:
+// the correction E -= cb sum (1+k_p)/2 J_p (dispersive.c:93-100) after the bulk
+// update and the current update J_p += (k_p - 1) J_p + beta_p/dt (E_new - E_old)
+// (dispersive.c:102-112), with E_old the value this kernel has just read, in the
+// reference's order and with the reference's expressions.  Same lane shape as
+// the H update; the H of plane i-1 is carried in registers.  The (ca, cb) class
+// table, the classes' Debye rows and the material rows sit in LDS.
+template <int NP, bool kCmp, int W>
 __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, int iseg, int npblk) {
+    __shared__ float2 s_tab[kCmp ? 768 : 1];
+    __shared__ int s_row[(kCmp && NP > 0) ? 768 : 1];
+    __shared__ float s_bt[(kCmp && NP > 0) ? kDebyeRows * NP : 1];
+    if (kCmp) {
+        for (int t = threadIdx.x; t < 768; t += kBlock) {
+            s_tab[t] = a.ctab[t];
+            if (NP > 0) s_row[t] = a.crow[t];
+        }
+        if (NP > 0)
+            for (int t = threadIdx.x; t < a.nrow * NP; t += kBlock) s_bt[t] = a.btab[t];
+        __syncthreads();
+    }
     const Grid g = a.g;
-    // (ca, cb) table read through the caches: a handful of pairs per component,
-    // so lookups hit L1; staging it in LDS cost a workgroup barrier before the
-    // first field load (measured 6 % slower on the 4.6 M-point board)
-    const float2 *__restrict__ tab = a.ctab;
+    const int ngw = g.kp / W;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int pblk = tile % npblk, seg = tile / npblk;
     const int p = pblk * kBlock + threadIdx.x;
-    if (p >= g.S) return;
-    const int j = p / (g.nz + 1), k = p - j * (g.nz + 1);
+    if (p >= (g.ny + 1) * ngw) return;
+    const int j = p / ngw, gk = p - j * ngw, k0 = W * gk;
     const int ib = i0 + seg * iseg;
     const int ie = min(i1, ib + iseg);
     float *__restrict__ Ex = a.E;
@@ -356,108 +506,236 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     const float *__restrict__ Hx = a.H;
     const float *__restrict__ Hy = a.H + g.N;
     const float *__restrict__ Hz = a.H + 2 * g.N;
-    const float *__restrict__ ca = a.ca;
-    const float *__restrict__ cb = a.cb;
-    const unsigned *__restrict__ cidx = a.cidx;
-    const bool jb = j >= 1 && j < g.ny, kb = k >= 1 && k < g.nz;   // interior node ranges
-    const float ey_s = a.es[1][j], ez_s = a.es[2][k];
+    const bool jb = j >= 1 && j < g.ny;
+    const float ey_s = a.es[1][j];
+    bool kb[W], kz[W], inz[W];
+    float ez_s[W], bz[W], cz[W];
     const int sy = a.has_pml ? a.pml.slot[1][j] : -1;
-    const int sz = a.has_pml ? a.pml.slot[2][k] : -1;
     const float by = sy >= 0 ? a.pml.b[1][j] : 0.f, cy = sy >= 0 ? a.pml.c[1][j] : 0.f;
-    const float bz = sz >= 0 ? a.pml.b[2][k] : 0.f, cz = sz >= 0 ? a.pml.c[2][k] : 0.f;
-    long long q = (long long)ib * g.si + p;
-    float hy_m = ib >= 1 ? Hy[q - g.si] : 0.f, hz_m = ib >= 1 ? Hz[q - g.si] : 0.f;
+    const int zo = a.has_pml ? a.pml.zo[k0 / 4] : -1;
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+        const int k = k0 + e;
+        kb[e] = k >= 1 && k < g.nz;
+        kz[e] = k < g.nz;
+        ez_s[e] = k <= g.nz ? a.es[2][k] : 0.f;
+        inz[e] = zo >= 0 && kz[e] && a.pml.slot[2][k] >= 0;
+        bz[e] = inz[e] ? a.pml.b[2][k] : 0.f;
+        cz[e] = inz[e] ? a.pml.c[2][k] : 0.f;
+    }
+    const bool box_jk = NP > 0 && j >= a.bj0 && j < a.bj1 && k0 >= 4 * a.bg0 && k0 < 4 * a.bg1;
+    const long long km = k0 ? -1 : 0;   // offset of the previous group's last point
+    long long q = (long long)ib * g.si + (long long)j * g.sj + k0;
+    float hym[W], hzm[W];
+    if (ib >= 1) {
+        ldw<W>(Hy + q - g.si, hym);
+        ldw<W>(Hz + q - g.si, hzm);
+    } else {
+        zerow<W>(hym);
+        zerow<W>(hzm);
+    }
     for (int i = ib; i < ie; i++, q += g.si) {
         const bool ib_ = i >= 1 && i < g.nx;
-        const float hx = Hx[q], hy = Hy[q], hz = Hz[q];
-        const float hx_k = k >= 1 ? Hx[q - 1] : 0.f, hy_k = k >= 1 ? Hy[q - 1] : 0.f;
-        const float hx_j = j >= 1 ? Hx[q - g.sj] : 0.f, hz_j = j >= 1 ? Hz[q - g.sj] : 0.f;
+        // Debye currents of the group first, every component and pole at once, so
+        // that their loads are in flight together with the field loads
+        const bool in_box = kCmp && NP > 0 && box_jk && i >= a.bi0 && i < a.bi1;
+        constexpr int NPE = (NP + 1) & ~1;            // poles per point, padded to even
+        constexpr int NJ = NP > 0 ? W * NPE : 2;      // floats of one component per lane
+        constexpr int CH = (W >= 2 && NJ % 4 == 0) ? 4 : 2;
+        float jv[3][NJ];
+        float *Jl = nullptr;
+        long long jc = 0;                             // component stride in the box row
+        if (in_box) {
+            const long long bkp = 4 * (a.bg1 - a.bg0);
+            jc = bkp * NPE;
+            Jl = a.J + ((long long)(i - a.bi0) * (a.bj1 - a.bj0) + (j - a.bj0)) * 3 * jc + (k0 - 4 * a.bg0) * NPE;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int u = 0; u < NJ; u += CH) {
+                    float t[CH];
+                    ldw<CH>(Jl + c * jc + u, t);
+#pragma unroll
+                    for (int v = 0; v < CH; v++) jv[c][u + v] = t[v];
+                }
+        }
+        float hx[W], hy[W], hz[W], hxj[W], hzj[W];
+        ldw<W>(Hx + q, hx);
+        ldw<W>(Hy + q, hy);
+        ldw<W>(Hz + q, hz);
+        const float hxk = Hx[q + km], hyk = Hy[q + km];
+        if (j >= 1) {
+            ldw<W>(Hx + q - g.sj, hxj);
+            ldw<W>(Hz + q - g.sj, hzj);
+        } else {
+            zerow<W>(hxj);
+            zerow<W>(hzj);
+        }
         const float ex_s = a.es[0][i];
         const int sx = a.has_pml ? a.pml.slot[0][i] : -1;
-        float2 cx, cy2, cz2;   // (ca, cb) of Ex, Ey, Ez at q
+        unsigned cls[W];
+        float ca[3][W], cb[3][W];
         if (kCmp) {
-            const unsigned ci = cidx[q];
-            cx = tab[ci & 255u];
-            cy2 = tab[256 + ((ci >> 8) & 255u)];
-            cz2 = tab[512 + ((ci >> 16) & 255u)];
+            if constexpr (W == 4) {
+                const uint4 c4 = *(const uint4 *)(a.cidx + q);
+                cls[0] = c4.x;
+                cls[1 % W] = c4.y;
+                cls[2 % W] = c4.z;
+                cls[3 % W] = c4.w;
+            } else if constexpr (W == 2) {
+                const uint2 c2 = *(const uint2 *)(a.cidx + q);
+                cls[0] = c2.x;
+                cls[1 % W] = c2.y;
+            } else {
+                cls[0] = a.cidx[q];
+            }
+#pragma unroll
+            for (int e = 0; e < W; e++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float2 t = s_tab[c * 256 + ((cls[e] >> (8 * c)) & 255u)];
+                    ca[c][e] = t.x;
+                    cb[c][e] = t.y;
+                }
         } else {
-            cx = make_float2(i < g.nx ? ca[q] : 0.f, i < g.nx ? cb[q] : 0.f);
-            cy2 = make_float2(j < g.ny ? ca[g.N + q] : 0.f, j < g.ny ? cb[g.N + q] : 0.f);
-            cz2 = make_float2(k < g.nz ? ca[2 * g.N + q] : 0.f, k < g.nz ? cb[2 * g.N + q] : 0.f);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                ldw<W>(a.ca + c * g.N + q, ca[c]);
+                ldw<W>(a.cb + c * g.N + q, cb[c]);
+            }
         }
-        // own E of the three components, loaded for every lane ahead of the
-        // component branches (q indexes every component array; values of the
-        // components a point does not update go unused): 439 -> 397 us on the
-        // 38.8 M-point board, where the k < nz branch splits every row's last wave
-        const float ex0 = Ex[q], ey0 = Ey[q], ez0 = Ez[q];
-        // psi of the y and z slabs, loaded for every lane through clamped offsets
-        // (offset 0 lies inside every psi array) so the loads are not behind
-        // lane-divergent branches; stores stay conditional
-        const long long oy = sy >= 0 ? sy * a.pml.per[1] + (long long)i * (g.nz + 1) + k : 0;
-        const long long oz = sz >= 0 ? psi_z(g, a.pml, i, j, sz) : 0;
-        const float v0y = a.has_pml ? a.psi[0][0][oy] : 0.f, v0z = a.has_pml ? a.psi[0][1][oz] : 0.f;
-        const float v1z = a.has_pml ? a.psi[1][0][oz] : 0.f, v2y = a.has_pml ? a.psi[2][1][oy] : 0.f;
-        if (i < g.nx) {     // Ex
-            const float e0 = ex0, cbv = cx.y;
-            float e = e0;
-            if (jb && kb) {
-                const float dy_hz = hz - hz_j, dz_hy = hy - hy_k;
-                e = cx.x * e0 + cbv * (dy_hz * ey_s - dz_hy * ez_s);
+        float ex0[W], ey0[W], ez0[W];
+        ldw<W>(Ex + q, ex0);
+        ldw<W>(Ey + q, ey0);
+        ldw<W>(Ez + q, ez0);
+        float exn[W], eyn[W], ezn[W];
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+            exn[e] = ex0[e];
+            eyn[e] = ey0[e];
+            ezn[e] = ez0[e];
+        }
+        float py0[W], py2[W], px1[W], px2[W], pz0[W], pz1[W];
+        float bx = 0.f, cx = 0.f;
+        long long oy = 0, ox = 0, oz = 0;
+        if (sy >= 0) {
+            oy = ((long long)sy * (g.nx + 1) + i) * g.kp + k0;
+            ldw<W>(a.psi[0][0] + oy, py0);
+            ldw<W>(a.psi[2][1] + oy, py2);
+        }
+        if (sx >= 0) {
+            ox = (long long)sx * g.si + (long long)j * g.sj + k0;
+            ldw<W>(a.psi[1][1] + ox, px1);
+            ldw<W>(a.psi[2][0] + ox, px2);
+            bx = a.pml.b[0][i];
+            cx = a.pml.c[0][i];
+        }
+        if (zo >= 0) {
+            oz = (((long long)i * (g.ny + 1) + j) * a.pml.zg + zo) * 4 + (k0 & 3);
+            ldw<W>(a.psi[0][1] + oz, pz0);
+            ldw<W>(a.psi[1][0] + oz, pz1);
+        }
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+            const float hx_k = e ? hx[(e + W - 1) % W] : hxk, hy_k = e ? hy[(e + W - 1) % W] : hyk;
+            if (jb && kb[e]) {     // Ex
+                const float cbv = cb[0][e];
+                const float dy_hz = hz[e] - hzj[e], dz_hy = hy[e] - hy_k;
+                float v0 = ca[0][e] * ex0[e] + cbv * (dy_hz * ey_s - dz_hy * ez_s[e]);
                 if (sy >= 0) {
-                    const float v = by * v0y + cy * dy_hz;
-                    a.psi[0][0][oy] = v;
-                    e += cbv * v;
+                    const float v = by * py0[e] + cy * dy_hz;
+                    py0[e] = v;
+                    v0 += cbv * v;
                 }
-                if (sz >= 0) {
-                    const float v = bz * v0z + cz * dz_hy;
-                    a.psi[0][1][oz] = v;
-                    e -= cbv * v;
+                if (inz[e]) {
+                    const float v = bz[e] * pz0[e] + cz[e] * dz_hy;
+                    pz0[e] = v;
+                    v0 -= cbv * v;
                 }
+                exn[e] = v0;
             }
-            Ex[q] = e;
-        }
-        if (j < g.ny) {     // Ey
-            const float e0 = ey0, cbv = cy2.y;
-            float e = e0;
-            if (ib_ && kb) {
-                const float dz_hx = hx - hx_k, dx_hz = hz - hz_m;
-                e = cy2.x * e0 + cbv * (dz_hx * ez_s - dx_hz * ex_s);
+            if (j < g.ny && ib_ && kb[e]) {   // Ey
+                const float cbv = cb[1][e];
+                const float dz_hx = hx[e] - hx_k, dx_hz = hz[e] - hzm[e];
+                float v0 = ca[1][e] * ey0[e] + cbv * (dz_hx * ez_s[e] - dx_hz * ex_s);
                 if (sx >= 0) {
-                    float *ps = a.psi[1][1] + sx * a.pml.per[0] + p;
-                    const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hz;
-                    *ps = v;
-                    e -= cbv * v;
+                    const float v = bx * px1[e] + cx * dx_hz;
+                    px1[e] = v;
+                    v0 -= cbv * v;
                 }
-                if (sz >= 0) {
-                    const float v = bz * v1z + cz * dz_hx;
-                    a.psi[1][0][oz] = v;
-                    e += cbv * v;
+                if (inz[e]) {
+                    const float v = bz[e] * pz1[e] + cz[e] * dz_hx;
+                    pz1[e] = v;
+                    v0 += cbv * v;
                 }
+                eyn[e] = v0;
             }
-            Ey[q] = e;
-        }
-        if (k < g.nz) {     // Ez
-            const float e0 = ez0, cbv = cz2.y;
-            float e = e0;
-            if (ib_ && jb) {
-                const float dx_hy = hy - hy_m, dy_hx = hx - hx_j;
-                e = cz2.x * e0 + cbv * (dx_hy * ex_s - dy_hx * ey_s);
+            if (kz[e] && ib_ && jb) {   // Ez
+                const float cbv = cb[2][e];
+                const float dx_hy = hy[e] - hym[e], dy_hx = hx[e] - hxj[e];
+                float v0 = ca[2][e] * ez0[e] + cbv * (dx_hy * ex_s - dy_hx * ey_s);
                 if (sx >= 0) {
-                    float *ps = a.psi[2][0] + sx * a.pml.per[0] + p;
-                    const float v = a.pml.b[0][i] * *ps + a.pml.c[0][i] * dx_hy;
-                    *ps = v;
-                    e += cbv * v;
+                    const float v = bx * px2[e] + cx * dx_hy;
+                    px2[e] = v;
+                    v0 += cbv * v;
                 }
                 if (sy >= 0) {
-                    const float v = by * v2y + cy * dy_hx;
-                    a.psi[2][1][oy] = v;
-                    e -= cbv * v;
+                    const float v = by * py2[e] + cy * dy_hx;
+                    py2[e] = v;
+                    v0 -= cbv * v;
+                }
+                ezn[e] = v0;
+            }
+        }
+        if (in_box) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float *en = c == 0 ? exn : c == 1 ? eyn : ezn;
+                const float *e0 = c == 0 ? ex0 : c == 1 ? ey0 : ez0;
+#pragma unroll
+                for (int e = 0; e < W; e++) {
+                    const int row = s_row[c * 256 + ((cls[e] >> (8 * c)) & 255u)];
+                    if (row < 0) continue;
+                    float *jp = jv[c] + e * NPE;
+                    float acc = 0.f;
+#pragma unroll
+                    for (int pp = 0; pp < NP; pp++) acc += a.kc[pp] * jp[pp];
+                    const float ev = en[e] - cb[c][e] * acc;
+                    en[e] = ev;
+                    const float de = ev - e0[e];
+                    const float *bt = s_bt + row * NP;
+#pragma unroll
+                    for (int pp = 0; pp < NP; pp++) jp[pp] = jp[pp] + a.km1[pp] * jp[pp] + bt[pp] * de;
+                }
+#pragma unroll
+                for (int u = 0; u < NJ; u += CH) {
+                    float t[CH];
+#pragma unroll
+                    for (int v = 0; v < CH; v++) t[v] = jv[c][u + v];
+                    stw<CH>(Jl + c * jc + u, t);
                 }
             }
-            Ez[q] = e;
         }
-        hy_m = hy;
-        hz_m = hz;
+        stw<W>(Ex + q, exn);
+        if (j < g.ny) stw<W>(Ey + q, eyn);
+        stw<W>(Ez + q, ezn);
+        if (sy >= 0) {
+            stw<W>(a.psi[0][0] + oy, py0);
+            stw<W>(a.psi[2][1] + oy, py2);
+        }
+        if (sx >= 0) {
+            stw<W>(a.psi[1][1] + ox, px1);
+            stw<W>(a.psi[2][0] + ox, px2);
+        }
+        if (zo >= 0) {
+            stw<W>(a.psi[0][1] + oz, pz0);
+            stw<W>(a.psi[1][0] + oz, pz1);
+        }
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+            hym[e] = hy[e];
+            hzm[e] = hz[e];
+        }
     }
 }
 
@@ -537,16 +815,17 @@ __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
     const long long n = a.ctl->step;
     float vs, gm;
     waveforms(a.wave, n, &vs, &gm);
-    if (t < a.ndisp) {                       // dispersive edge
+    if (t < a.ndisp) {                       // listed dispersive edge
         const unsigned q = a.d_idx[t];
         const unsigned m = a.d_mat[t];
         const float *mat = a.dtab + (m & kDispMat) * a.dstride;
+        const long long jb = a.d_jb ? a.d_jb[t] : (long long)t;
         float e = a.E[q];
         // every pole's J loaded once, together (unrolled, guarded), and kept in
-        // registers for the update: 396 -> 342 us on the 19 M-edge board
+        // registers for the update
         float jv[kMaxPoles];
 #pragma unroll
-        for (int p = 0; p < kMaxPoles; p++) jv[p] = p < pp.npole ? a.J[(long long)p * a.ndisp + t] : 0.f;
+        for (int p = 0; p < kMaxPoles; p++) jv[p] = p < pp.npole ? a.J[jb + p * a.jstride] : 0.f;
         float acc = 0.f;
 #pragma unroll
         for (int p = 0; p < kMaxPoles; p++)
@@ -560,7 +839,7 @@ __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
         const float de = e - a.Eold[t];
 #pragma unroll
         for (int p = 0; p < kMaxPoles; p++)
-            if (p < pp.npole) a.J[(long long)p * a.ndisp + t] = jv[p] + pp.km1[p] * jv[p] + mat[1 + p] * de;
+            if (p < pp.npole) a.J[jb + p * a.jstride] = jv[p] + pp.km1[p] * jv[p] + mat[1 + p] * de;
         a.E[q] = e;
         a.Eold[t] = e;                       // E of this edge before the next E update
     } else if (t - a.ndisp < a.nsbo) {       // surface edge without dispersion
@@ -597,7 +876,7 @@ __global__ void k_set_ctl(Ctl *c, long long step, int row) {
     c->row = row;
 }
 
-// E of every dispersive edge, taken as E_old of the next E update
+// E of every listed dispersive edge, taken as E_old of the next E update
 __global__ void __launch_bounds__(kBlock) k_prime_eold(const float *E, const unsigned *d_idx, float *Eold, int n) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t < n) Eold[t] = E[d_idx[t]];
@@ -711,7 +990,8 @@ __global__ void __launch_bounds__(kBlock) k_energy(EnergyArgs a, double *partial
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < g.N; q += stride) {
         const int i = (int)(q / g.si);
         const int rem = (int)(q - (long long)i * g.si);
-        const int j = rem / (g.nz + 1), k = rem - j * (g.nz + 1);
+        const int j = rem / g.kp, k = rem - j * g.kp;
+        if (k > g.nz) continue;   // row padding
         const int ijk[3] = {i, j, k};
         const int n[3] = {g.nx, g.ny, g.nz};
         for (int c = 0; c < 3; c++) {
@@ -952,11 +1232,25 @@ struct Device {
     Grid g{};
     DBuf<float> F;               // E (3N) then H (3N)
     DBuf<float> ca, cb;          // 3N each, only when more than 256 distinct pairs
-    DBuf<unsigned> cidx;         // N packed coefficient indices
+    DBuf<unsigned> cidx;         // N packed coefficient classes
     DBuf<float2> ctab;           // 3 x 256 (ca, cb)
+    DBuf<int> crow;              // 3 x 256 Debye material row of a class (-1: none)
     bool compressed = false;
+    // Debye currents updated inside the E kernel (fused) or by the post kernel
+    bool fused = false;
+    int npole = 0;
+    int nlisted = 0;             // dispersive edges the post kernel takes
+    DBuf<float> btab;            // fused: beta_p / dt per material row
+    int nrow = 0;
+    int bi0 = 0, bi1 = 0, bj0 = 0, bj1 = 0, bg0 = 0, bg1 = 0;   // J box (k in groups)
+    long long bkp = 0;           // k points of a J box row
+    int npe = 0;                 // poles per point in the J box (npole padded to even)
+    DBuf<long long> d_jb;        // fused: J box index of each listed edge
+    std::vector<long long> jbox_of;   // fused: J box index of every dispersive edge (reference order)
     DBuf<float> axis1d;          // hs[3], es[3], pml b/c for H and E per axis
-    DBuf<int> slots;             // CPML slot tables
+    DBuf<int> slots;             // CPML slot tables, then the z-layout group tables
+    std::vector<int> slots_host;
+    size_t slot_off[2][3] = {}, zo_off[2] = {};   // [is_e][axis] offsets into slots
     DBuf<float> psi;             // all psi arrays
     float *psiE[3][2] = {}, *psiH[3][2] = {};
     size_t psiE_len[3][2] = {}, psiH_len[3][2] = {};
@@ -965,9 +1259,10 @@ struct Device {
     // post-phase edge lists
     DBuf<unsigned> d_idx, sb_idx, sbo, x_idx;
     DBuf<int> sb_src, x_src;
-    DBuf<int> dsb_t, dsb_u, dsrc_t, dsrc_x;   // dispersive edges with SIBC / source slots
-    std::vector<unsigned> d_mat_host;          // material rows of the dispersive edges, no flags
-    std::vector<int> dsb_t_host;               // ascending dispersive edges with an SIBC slot
+    DBuf<int> dsb_t, dsb_u, dsrc_t, dsrc_x;   // listed dispersive edges with SIBC / source slots
+    std::vector<unsigned> d_mat_host;          // material rows of the listed dispersive edges, no flags
+    std::vector<int> dsb_t_host;               // ascending listed dispersive edges with an SIBC slot
+    std::vector<int> listed_of;                // listed slot of each dispersive edge (reference order), -1
     int nsbo = 0;
     bool has_sbsrc = false;
     DBuf<float> J, dtab, Eold, Y, wl, wm;
@@ -975,6 +1270,7 @@ struct Device {
     int dstride = 1;
     DBuf<unsigned char> face;
     DBuf<PostParams> pp;
+    PostParams pp_host{};
     DBuf<Ctl> ctl;
     long long ctl_step = -1;     // what ctl holds once the queued work has run
     int ctl_row = -1;
@@ -1004,6 +1300,7 @@ struct Device {
     // launch shape: x planes per thread in the stencil kernels, set by the autotuner
     // on the first steps unless fixed by COURANT_ISEG_H / COURANT_ISEG_E
     int iseg_h = kIseg, iseg_e = kIseg;
+    int wh = 4, we = 1;          // lane widths (k points per thread) of the H and E updates
     bool tuned = false;
     // timing
     bool profiling = false;
@@ -1041,9 +1338,18 @@ long surface_slot(const cem_sim *s, unsigned long long g) {
     return lo < (size_t)s->nsb && (unsigned long long)s->sb_comp[lo] * s->N + s->sb_idx[lo] == g ? (long)lo : -1;
 }
 
-// Excited sources as (global E index, lumped cs, modal weight) slots; edges that
-// are also dispersive are flagged in d_mat and listed in dsrc_t / dsrc_x, the
-// others go to x_idx.
+// Device index of the reference's global index comp N + q (i si + j sj + k).
+unsigned long long dev_index(const cem_sim *s, const Grid &g, unsigned long long gidx) {
+    const unsigned long long c = gidx / s->N, q = gidx % s->N;
+    const unsigned long long i = q / s->si, r = q % s->si, j = r / s->sj, k = r % s->sj;
+    return c * (unsigned long long)g.N + i * (unsigned long long)g.si + j * (unsigned long long)g.sj + k;
+}
+
+// Excited sources as (device E index, lumped cs, modal weight) slots; edges that
+// are also listed dispersive edges are flagged in d_mat and listed in dsrc_t /
+// dsrc_x, surface edges get sb_src, the others go to x_idx.  Every dispersive
+// edge a source can drive (the mode ports' source planes; lumped-port edges are
+// never dispersive) is a listed edge, whichever port is excited.
 int upload_sources(cem_sim *s, Device *D) {
     std::vector<std::pair<unsigned long long, std::pair<double, double>>> src;
     if (s->src_port >= 0) {
@@ -1077,13 +1383,18 @@ int upload_sources(cem_sim *s, Device *D) {
         const long t = dispersive_slot(s, g);
         const long u = t < 0 ? surface_slot(s, g) : -1;
         if (t >= 0) {
-            dsrc_t.push_back((int)t);
+            const int lt = D->listed_of[t];
+            if (lt < 0) {
+                set_error("source edge missing from the post-phase edge list");
+                return -1;
+            }
+            dsrc_t.push_back(lt);
             dsrc_x.push_back((int)k);
         } else if (u >= 0) {
             sb_src[u] = (int)k;
             D->has_sbsrc = true;
         } else {
-            x_idx.push_back((unsigned)g);
+            x_idx.push_back((unsigned)dev_index(s, D->g, g));
             x_src.push_back((int)k);
         }
     }
@@ -1104,8 +1415,8 @@ int upload_sources(cem_sim *s, Device *D) {
     if (D->dsrc_t.alloc(dsrc_t.size(), D->st) || D->dsrc_x.alloc(dsrc_x.size(), D->st) ||
         D->dsrc_t.upload(dsrc_t.data(), dsrc_t.size(), D->st) || D->dsrc_x.upload(dsrc_x.data(), dsrc_x.size(), D->st))
         return -1;
-    // material rows of the dispersive edges with the SIBC / source flag bits
-    if (s->ndisp) {
+    // material rows of the listed dispersive edges with the SIBC / source flag bits
+    if (D->nlisted) {
         std::vector<unsigned> m(D->d_mat_host);
         for (int t : D->dsb_t_host) m[t] |= kDispSibc;
         for (int t : dsrc_t) m[t] |= kDispSrc;
@@ -1117,70 +1428,236 @@ int upload_sources(cem_sim *s, Device *D) {
     return 0;
 }
 
+// Floats of the psi array (is_e, comp, w) in the device layout (see Pml).
+size_t psi_dev_len(const cem_sim *s, const Device *D, int is_e, int c, int w) {
+    if (!s->has_pml) return 0;
+    const int a = (c + w + 1) % 3;
+    const size_t planes = psi_len(s, is_e, c, w) / pml_per(s, a);
+    const Grid &g = D->g;
+    if (a == 0) return planes * (size_t)g.S4 * 4;
+    if (a == 1) return planes * (size_t)(g.nx + 1) * g.kp;
+    return planes ? (size_t)(g.nx + 1) * (g.ny + 1) * (is_e ? D->pmlE : D->pmlH).zg * 4 : 0;
+}
+
+// Device float index of every entry of the reference's psi array (is_e, comp, w):
+// slab-major, slot * per + in-plane index with the in-plane index running over
+// the other two axes at full node extent, the later axis fastest (sim.c:641-678).
+std::vector<size_t> psi_dev_map(const cem_sim *s, const Device *D, int is_e, int c, int w) {
+    const int a = (c + w + 1) % 3;
+    const size_t per = pml_per(s, a), len = psi_len(s, is_e, c, w);
+    const Grid &g = D->g;
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    std::vector<size_t> m(len);
+    std::vector<int> kof;   // z slab slot -> k
+    if (a == 2) {
+        const int *sl = D->slots_host.data() + D->slot_off[is_e][2];
+        for (int k = 0; k <= nz + is_e; k++)
+            if (sl[k] >= 0) {
+                if ((int)kof.size() <= sl[k]) kof.resize(sl[k] + 1, 0);
+                kof[sl[k]] = k;
+            }
+    }
+    const int *zo = D->slots_host.data() + D->zo_off[is_e];
+    const int zg = (is_e ? D->pmlE : D->pmlH).zg;
+    for (size_t r = 0; r < len; r++) {
+        const size_t slot = r / per, rr = r % per;
+        if (a == 0) {
+            const size_t j = rr / (nz + 1), k = rr % (nz + 1);
+            m[r] = slot * (size_t)g.S4 * 4 + j * g.kp + k;
+        } else if (a == 1) {
+            const size_t i = rr / (nz + 1), k = rr % (nz + 1);
+            m[r] = (slot * (nx + 1) + i) * g.kp + k;
+        } else {
+            const int k = kof[slot];
+            m[r] = (rr * zg + zo[k / 4]) * 4 + (k % 4);
+        }
+    }
+    (void)ny;
+    return m;
+}
+
+// Coefficient classes: per component the (ca, cb) pairs of a grid take a handful
+// of values (one per material mix), so each point stores three 8-bit class
+// numbers; a class also carries the Debye material row of the edges the E kernel
+// updates (row[e] >= 0 for those, -1 elsewhere).  Class 0 of every component is
+// (0, 0, no row): the row padding.  Returns false when a component needs more
+// than 256 classes.
+bool build_classes(const cem_sim *s, const Grid &g, const std::vector<int> &row_of, std::vector<unsigned> &idx,
+                   std::vector<float2> &tab, std::vector<int> &rowtab) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    idx.assign((size_t)g.N, 0u);
+    tab.assign(3 * 256, make_float2(0.f, 0.f));
+    rowtab.assign(3 * 256, -1);
+    size_t t = 0;   // dispersive edges are sorted by (comp, q): walk them with q
+    for (int c = 0; c < 3; c++) {
+        std::vector<std::pair<std::array<unsigned, 3>, unsigned>> seen;
+        seen.push_back({{0u, 0u, 0xffffffffu}, 0u});
+        std::array<unsigned, 3> last_key = {0u, 0u, 0xffffffffu};
+        unsigned last_slot = 0;
+        for (int i = 0; i <= nx; i++)
+            for (int j = 0; j <= ny; j++) {
+                const size_t q0 = (size_t)i * s->si + (size_t)j * s->sj;
+                const size_t d0 = (size_t)i * g.si + (size_t)j * g.sj;
+                for (int k = 0; k <= nz; k++) {
+                    const size_t q = q0 + k;
+                    int row = -1;
+                    while (t < (size_t)s->ndisp && (s->d_comp[t] < c || (s->d_comp[t] == c && s->d_idx[t] < q))) t++;
+                    if (t < (size_t)s->ndisp && s->d_comp[t] == c && s->d_idx[t] == q) row = row_of[t];
+                    const float fa = (float)s->ca[c][q], fb = (float)s->cb[c][q];
+                    std::array<unsigned, 3> key;
+                    std::memcpy(&key[0], &fa, 4);
+                    std::memcpy(&key[1], &fb, 4);
+                    key[2] = (unsigned)row;
+                    unsigned slot = last_slot;
+                    if (key != last_key) {
+                        auto it = std::find_if(seen.begin(), seen.end(), [&](const auto &e) { return e.first == key; });
+                        if (it == seen.end()) {
+                            if (seen.size() == 256) return false;
+                            slot = (unsigned)seen.size();
+                            seen.push_back({key, slot});
+                            tab[c * 256 + slot] = make_float2(fa, fb);
+                            rowtab[c * 256 + slot] = row;
+                        } else {
+                            slot = it->second;
+                        }
+                        last_key = key;
+                        last_slot = slot;
+                    }
+                    idx[d0 + k] |= slot << (8 * c);
+                }
+            }
+    }
+    return true;
+}
+
 int upload_plan(cem_sim *s, Device *D) {
     const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
     Grid &g = D->g;
     g.nx = nx;
     g.ny = ny;
     g.nz = nz;
-    g.S = (ny + 1) * (nz + 1);
-    g.si = (long long)s->si;
-    g.sj = (long long)s->sj;
-    g.N = (long long)s->N;
-    g.nchunk = (g.S + 63) / 64;
+    g.kp = (nz + 1 + 3) / 4 * 4;
+    g.ng = g.kp / 4;
+    g.S4 = (ny + 1) * g.ng;
+    g.sj = g.kp;
+    g.si = (long long)(ny + 1) * g.kp;
+    g.N = (long long)(nx + 1) * g.si;
+    g.si4 = g.si / 4;
+    g.N4 = g.N / 4;
     const size_t N = s->N;
+    const size_t Nd = (size_t)g.N;
     hipStream_t st = D->st;
+    if (3 * Nd > 0xffffffffull) {
+        set_error("grid too large for 32-bit edge indices");
+        return -1;
+    }
+    // device index of a reference point index (same component)
+    auto dq = [&](size_t q) -> size_t {
+        const size_t i = q / s->si, r = q % s->si, j = r / s->sj, k = r % s->sj;
+        return i * (size_t)g.si + j * (size_t)g.sj + k;
+    };
 
-    // fields (zero) and coefficients
-    if (D->F.alloc(6 * N, D->st) || D->F.zero(st)) return -1;
-    // Coefficient compression: the (ca, cb) pairs of each component take few
-    // distinct values (one per material mix), so each point stores three 8-bit
-    // indices into a 256-entry table; more than 256 pairs falls back to arrays.
+    // fields (zero, row padding included)
+    if (D->F.alloc(6 * Nd, D->st) || D->F.zero(st)) return -1;
+
+    // Debye materials: dispersive edges share a handful of (cb, beta_p / dt) rows
+    const int np = s->npole, nd = s->ndisp;
+    D->npole = np;
+    std::vector<unsigned> mat(nd);
+    std::vector<float> tab;
     {
-        std::vector<unsigned> idx(N, 0);
-        std::vector<float2> tab(3 * 256, make_float2(0.f, 0.f));
-        bool ok = std::getenv("COURANT_COEF_ARRAYS") == nullptr;   // switch to force the array path
-        for (int c = 0; c < 3 && ok; c++) {
-            std::vector<std::pair<unsigned long long, unsigned>> seen;   // (ca bits << 32 | cb bits) -> slot
-            unsigned long long last_key = ~0ull;
-            unsigned last_slot = 0;
-            for (size_t q = 0; q < N && ok; q++) {
-                const float fa = (float)s->ca[c][q], fb = (float)s->cb[c][q];
-                unsigned ua, ub;
-                std::memcpy(&ua, &fa, 4);
-                std::memcpy(&ub, &fb, 4);
-                const unsigned long long key = ((unsigned long long)ua << 32) | ub;
-                unsigned slot = last_slot;
-                if (key != last_key) {
-                    auto it = std::find_if(seen.begin(), seen.end(), [&](const auto &e) { return e.first == key; });
-                    if (it == seen.end()) {
-                        if (seen.size() == 256) { ok = false; break; }
-                        slot = (unsigned)seen.size();
-                        seen.push_back({key, slot});
-                        tab[c * 256 + slot] = make_float2(fa, fb);
-                    } else {
-                        slot = it->second;
-                    }
-                    last_key = key;
-                    last_slot = slot;
-                }
-                idx[q] |= slot << (8 * c);
+        std::map<std::vector<float>, unsigned> rows;
+        std::vector<float> key(1 + np);
+        for (int e = 0; e < nd; e++) {
+            key[0] = (float)s->d_cb[e];
+            for (int p = 0; p < np; p++) key[1 + p] = (float)(s->d_beta[(size_t)p * nd + e] * (1.0 / s->dt));
+            auto it = rows.find(key);
+            if (it == rows.end()) {
+                it = rows.emplace(key, (unsigned)rows.size()).first;
+                tab.insert(tab.end(), key.begin(), key.end());
             }
+            mat[e] = it->second;
         }
-        D->compressed = ok;
-        if (ok) {
-            if (D->cidx.alloc(N, D->st) || D->cidx.upload(idx.data(), N, st)) return -1;
-            if (D->ctab.alloc(3 * 256, D->st) || D->ctab.upload(tab.data(), 3 * 256, st)) return -1;
+        if (rows.size() > kDispMat) {
+            set_error("too many Debye materials");
+            return -1;
+        }
+        D->nrow = (int)rows.size();
+    }
+    // Dispersive edges the post kernel must take in reference order with other
+    // physics: surface (SIBC) edges, any mode port's source edges, and the edges
+    // on the x = max face, which the E kernel's i range does not visit.
+    std::vector<uint8_t> linked(nd, 0);
+    for (int e = 0; e < s->nsb; e++) {
+        const long t = dispersive_slot(s, (unsigned long long)s->sb_comp[e] * N + s->sb_idx[e]);
+        if (t >= 0) linked[t] = 1;
+    }
+    for (auto &mp : s->mport)
+        for (size_t e = 0; e < mp.src_idx.size(); e++) {
+            const long t = dispersive_slot(s, (unsigned long long)mp.src_comp[e] * N + mp.src_idx[e]);
+            if (t >= 0) linked[t] = 1;
+        }
+    for (auto &pt : s->port)
+        for (size_t e = 0; e < pt.edge.size(); e++) {
+            const long t = dispersive_slot(s, (unsigned long long)pt.axis * N + pt.edge[e]);
+            if (t >= 0) linked[t] = 1;
+        }
+    int bi0 = nx + 1, bi1 = 0, bj0 = ny + 1, bj1 = 0, bk0 = nz + 1, bk1 = 0;
+    for (int t = 0; t < nd; t++) {
+        const size_t q = s->d_idx[t];
+        const int i = (int)(q / s->si), j = (int)(q % s->si / s->sj), k = (int)(q % s->sj);
+        if (i >= nx) linked[t] = 1;
+        bi0 = std::min(bi0, i);
+        bi1 = std::max(bi1, i + 1);
+        bj0 = std::min(bj0, j);
+        bj1 = std::max(bj1, j + 1);
+        bk0 = std::min(bk0, k);
+        bk1 = std::max(bk1, k + 1);
+    }
+    // the E kernel takes the Debye currents when the poles fit its registers, the
+    // material rows its LDS, and the J box is not much larger than the edge list
+    bool fused = np >= 1 && np <= kFusedPoles && D->nrow <= kDebyeRows && nd > 0 &&
+                 std::getenv("COURANT_DEBYE_LIST") == nullptr && std::getenv("COURANT_COEF_ARRAYS") == nullptr;
+    if (fused) {
+        bk0 = bk0 / 4 * 4;
+        bk1 = (bk1 + 3) / 4 * 4;
+        const size_t box = (size_t)(bi1 - bi0) * (bj1 - bj0) * (bk1 - bk0);
+        if (3 * box > 8 * (size_t)nd + ((size_t)1 << 22)) fused = false;
+    }
+    // coefficient classes, with the Debye rows of the fused edges
+    std::vector<unsigned> cls;
+    std::vector<float2> ctab;
+    std::vector<int> crow;
+    bool compressed = false;
+    if (std::getenv("COURANT_COEF_ARRAYS") == nullptr) {
+        std::vector<int> row_of(nd, -1);
+        if (fused)
+            for (int t = 0; t < nd; t++)
+                if (!linked[t]) row_of[t] = (int)mat[t];
+        compressed = build_classes(s, g, row_of, cls, ctab, crow);
+        if (!compressed && fused) {   // retry with every dispersive edge in the post kernel
+            fused = false;
+            std::fill(row_of.begin(), row_of.end(), -1);
+            compressed = build_classes(s, g, row_of, cls, ctab, crow);
         }
     }
-    if (!D->compressed && (D->ca.alloc(3 * N, D->st) || D->cb.alloc(3 * N, D->st))) return -1;
-    if (!D->compressed) {
-        std::vector<float> tmp(N);
-        for (int c = 0; c < 3; c++) {
-            for (size_t q = 0; q < N; q++) tmp[q] = (float)s->ca[c][q];
-            if (hipMemcpyAsync(D->ca.p + c * N, tmp.data(), N * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return -1;
-            for (size_t q = 0; q < N; q++) tmp[q] = (float)s->cb[c][q];
-            if (hipMemcpyAsync(D->cb.p + c * N, tmp.data(), N * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return -1;
+    if (!compressed) fused = false;
+    D->compressed = compressed;
+    D->fused = fused;
+    if (compressed) {
+        if (D->cidx.alloc(Nd, D->st) || D->cidx.upload(cls.data(), Nd, st)) return -1;
+        if (D->ctab.alloc(3 * 256, D->st) || D->ctab.upload(ctab.data(), 3 * 256, st)) return -1;
+        if (D->crow.alloc(3 * 256, D->st) || D->crow.upload(crow.data(), 3 * 256, st)) return -1;
+        std::vector<unsigned>().swap(cls);
+    } else {
+        if (D->ca.alloc(3 * Nd, D->st) || D->cb.alloc(3 * Nd, D->st)) return -1;
+        std::vector<float> tmp(3 * Nd, 0.f);
+        for (int pass = 0; pass < 2; pass++) {
+            const dvec *arr = pass ? s->cb : s->ca;
+            std::fill(tmp.begin(), tmp.end(), 0.f);
+            for (int c = 0; c < 3; c++)
+                for (size_t q = 0; q < N; q++) tmp[c * Nd + dq(q)] = (float)arr[c][q];
+            if ((pass ? D->cb : D->ca).upload(tmp.data(), 3 * Nd, st)) return -1;
         }
     }
 
@@ -1213,11 +1690,12 @@ int upload_plan(cem_sim *s, Device *D) {
         D->pmlE.c[a] = D->axis1d.p + off_ce[a];
     }
 
-    // CPML slot tables and psi storage (compact layout of sim.c:655-678)
+    // CPML slot tables (compact slab numbering of sim.c:655-678) and, per kernel,
+    // the z-layout place of each k group that holds z-slab points
     std::vector<int> slots;
-    size_t off_sh[3], off_se[3];
-    for (int a = 0; a < 3; a++) {
-        for (int is_e = 0; is_e < 2; is_e++) {
+    size_t off_sh[3], off_se[3], off_zh = 0, off_ze = 0;
+    for (int is_e = 0; is_e < 2; is_e++) {
+        for (int a = 0; a < 3; a++) {
             const int len = s->n[a] + (is_e ? 1 : 0);
             (is_e ? off_se : off_sh)[a] = slots.size();
             std::vector<int> sl(len + 1, -1);
@@ -1231,26 +1709,38 @@ int upload_plan(cem_sim *s, Device *D) {
             }
             slots.insert(slots.end(), sl.begin(), sl.end());
         }
-        D->pmlH.per[a] = D->pmlE.per[a] = (long long)pml_per(s, a);
-    }
-    {
-        const size_t per = pml_per(s, 2);
-        D->pmlH.zpl = per ? (int)(psi_len(s, 0, 0, 1) / per) : 0;
-        D->pmlE.zpl = per ? (int)(psi_len(s, 1, 0, 1) / per) : 0;
+        const size_t zs = (is_e ? off_se : off_sh)[2];
+        const int len = nz + (is_e ? 1 : 0);
+        (is_e ? off_ze : off_zh) = slots.size();
+        int zg = 0;
+        for (int gk = 0; gk < g.ng; gk++) {
+            bool any = false;
+            for (int k = 4 * gk; k < 4 * gk + 4 && k < len; k++) any |= slots[zs + k] >= 0;
+            slots.push_back(any ? zg++ : -1);
+        }
+        (is_e ? D->pmlE : D->pmlH).zg = zg;
     }
     if (D->slots.alloc(slots.size(), D->st) || D->slots.upload(slots.data(), slots.size(), st)) return -1;
+    D->slots_host = slots;
     for (int a = 0; a < 3; a++) {
         D->pmlH.slot[a] = D->slots.p + off_sh[a];
         D->pmlE.slot[a] = D->slots.p + off_se[a];
+        D->slot_off[0][a] = off_sh[a];
+        D->slot_off[1][a] = off_se[a];
     }
+    D->pmlH.zo = D->slots.p + off_zh;
+    D->pmlE.zo = D->slots.p + off_ze;
+    D->zo_off[0] = off_zh;
+    D->zo_off[1] = off_ze;
+    // psi arrays in the device layout (see Pml)
     size_t psi_total = 0;
     for (int c = 0; c < 3; c++)
         for (int w = 0; w < 2; w++) {
-            D->psiE_len[c][w] = psi_len(s, 1, c, w);
-            D->psiH_len[c][w] = psi_len(s, 0, c, w);
+            D->psiE_len[c][w] = psi_dev_len(s, D, 1, c, w);
+            D->psiH_len[c][w] = psi_dev_len(s, D, 0, c, w);
             psi_total += D->psiE_len[c][w] + D->psiH_len[c][w];
         }
-    if (D->psi.alloc(psi_total + 1, D->st) || D->psi.zero(st)) return -1;
+    if (D->psi.alloc(psi_total + 4, D->st) || D->psi.zero(st)) return -1;
     {
         size_t o = 0;
         for (int c = 0; c < 3; c++)
@@ -1262,48 +1752,60 @@ int upload_plan(cem_sim *s, Device *D) {
             }
     }
 
-    // Debye ADE tables
+    // Debye tables.  Listed edges (all of them when the E kernel does not take the
+    // currents) keep the reference's order; their J is the reference's pole-major
+    // array, or, when the E kernel takes the others, the box entry of the edge.
     {
-        if (3 * N > 0xffffffffull) {
-            set_error("grid too large for 32-bit edge indices");
-            return -1;
+        D->listed_of.assign(nd, -1);
+        std::vector<unsigned> lidx, lmat;
+        std::vector<long long> ljb;
+        D->jbox_of.clear();
+        if (fused) {
+            D->bi0 = bi0;
+            D->bi1 = bi1;
+            D->bj0 = bj0;
+            D->bj1 = bj1;
+            D->bg0 = bk0 / 4;
+            D->bg1 = bk1 / 4;
+            D->bkp = bk1 - bk0;
+            D->npe = (np + 1) & ~1;
+            D->jbox_of.resize(nd);
         }
-        // Dispersive edges share a handful of materials: (cb, beta_p / dt) rows of
-        // a table, found by exact FP32 value, replace 4 + 4P bytes per edge.
-        const int np = s->npole;
-        std::vector<unsigned> idx(s->ndisp), mat(s->ndisp);
-        std::vector<float> tab;
-        std::map<std::vector<float>, unsigned> rows;
-        std::vector<float> key(1 + np);
-        for (int e = 0; e < s->ndisp; e++) {
-            idx[e] = (unsigned)(s->d_comp[e] * N + s->d_idx[e]);
-            key[0] = (float)s->d_cb[e];
-            for (int p = 0; p < np; p++) key[1 + p] = (float)(s->d_beta[(size_t)p * s->ndisp + e] * (1.0 / s->dt));
-            auto it = rows.find(key);
-            if (it == rows.end()) {
-                it = rows.emplace(key, (unsigned)rows.size()).first;
-                tab.insert(tab.end(), key.begin(), key.end());
+        for (int t = 0; t < nd; t++) {
+            const size_t q = s->d_idx[t];
+            if (fused) {
+                const int i = (int)(q / s->si), j = (int)(q % s->si / s->sj), k = (int)(q % s->sj);
+                D->jbox_of[t] = ((((long long)(i - bi0) * (bj1 - bj0) + (j - bj0)) * 3 + s->d_comp[t]) * D->bkp +
+                                 (k - bk0)) * D->npe;
             }
-            mat[e] = it->second;
+            if (fused && !linked[t]) continue;
+            D->listed_of[t] = (int)lidx.size();
+            lidx.push_back((unsigned)((size_t)s->d_comp[t] * Nd + dq(q)));
+            lmat.push_back(mat[t]);
+            if (fused) ljb.push_back(D->jbox_of[t]);
         }
+        D->nlisted = (int)lidx.size();
         D->dstride = 1 + np;
-        if (D->d_idx.alloc(s->ndisp, D->st) || D->d_idx.upload(idx.data(), s->ndisp, st)) return -1;
-        if (rows.size() > kDispMat) {
-            set_error("too many Debye materials");
-            return -1;
-        }
-        D->d_mat_host = mat;   // uploaded with its flag bits by upload_sources
-        if (D->d_mat.alloc(s->ndisp, D->st)) return -1;
+        if (D->d_idx.alloc(lidx.size(), D->st) || D->d_idx.upload(lidx.data(), lidx.size(), st)) return -1;
+        D->d_mat_host = lmat;   // uploaded with its flag bits by upload_sources
+        if (D->d_mat.alloc(lmat.size(), D->st)) return -1;
+        if (fused && (D->d_jb.alloc(ljb.size(), D->st) || D->d_jb.upload(ljb.data(), ljb.size(), st))) return -1;
         if (D->dtab.alloc(tab.size() + 1, D->st) || D->dtab.upload(tab.data(), tab.size(), st)) return -1;
-        if (D->Eold.alloc(s->ndisp, D->st) || D->Eold.zero(st)) return -1;
-        const size_t nj = (size_t)np * s->ndisp;
+        if (fused) {
+            std::vector<float> bt((size_t)D->nrow * np);
+            for (int r = 0; r < D->nrow; r++)
+                for (int p = 0; p < np; p++) bt[(size_t)r * np + p] = tab[(size_t)r * (1 + np) + 1 + p];
+            if (D->btab.alloc(bt.size(), D->st) || D->btab.upload(bt.data(), bt.size(), st)) return -1;
+        }
+        if (D->Eold.alloc(lidx.size(), D->st) || D->Eold.zero(st)) return -1;
+        const size_t nj = fused ? (size_t)(bi1 - bi0) * (bj1 - bj0) * 3 * D->bkp * D->npe : (size_t)np * nd;
         if (D->J.alloc(nj, D->st) || D->J.zero(st)) return -1;
         PostParams pp{};
-        pp.ndisp = s->ndisp;
-        pp.npole = s->npole;
+        pp.ndisp = D->nlisted;
+        pp.npole = np;
         pp.nsb = s->nsb;
         pp.nterm = s->sibc_npole;
-        for (int p = 0; p < s->npole; p++) {
+        for (int p = 0; p < np; p++) {
             const double tau = 1.0 / (2 * M_PI * s->pole_f[p]);
             const double x = s->dt / (2 * tau);
             pp.kc[p] = (float)(1.0 / (1.0 + x));          // (1 + k)/2
@@ -1313,6 +1815,7 @@ int upload_plan(cem_sim *s, Device *D) {
             pp.zc[t] = (float)s->sibc_c[t];
             pp.om[t] = (float)(-std::expm1(-s->sibc_a[t] * s->dt));
         }
+        D->pp_host = pp;
         if (D->pp.alloc(1, D->st) || D->pp.upload(&pp, 1, st)) return -1;
         if (D->ctl.alloc(1, D->st) || D->ctl.zero(st)) return -1;
         D->ctl_step = 0;
@@ -1324,10 +1827,11 @@ int upload_plan(cem_sim *s, Device *D) {
         std::vector<int> dsb_u;
         D->dsb_t_host.clear();
         for (int e = 0; e < s->nsb; e++) {   // ascending global index, so t ascends too
-            idx[e] = (unsigned)(s->sb_comp[e] * N + s->sb_idx[e]);
-            const long t = dispersive_slot(s, idx[e]);
+            const unsigned long long gi = (unsigned long long)s->sb_comp[e] * N + s->sb_idx[e];
+            idx[e] = (unsigned)dev_index(s, g, gi);
+            const long t = dispersive_slot(s, gi);
             if (t >= 0) {
-                D->dsb_t_host.push_back((int)t);
+                D->dsb_t_host.push_back(D->listed_of[t]);
                 dsb_u.push_back(e);
             } else {
                 sbo.push_back((unsigned)e);
@@ -1354,7 +1858,7 @@ int upload_plan(cem_sim *s, Device *D) {
         for (auto &p : s->ip) {
             toff.push_back((int)tidx.size());
             for (size_t e = 0; e < p.idx.size(); e++) {
-                tidx.push_back((long long)p.comp[e] * (long long)N + (long long)p.idx[e]);
+                tidx.push_back((long long)dev_index(s, g, (unsigned long long)p.comp[e] * N + p.idx[e]));
                 tw.push_back(p.w[e]);
             }
             post.push_back(1.0);
@@ -1362,7 +1866,7 @@ int upload_plan(cem_sim *s, Device *D) {
         for (auto &p : s->port) {
             toff.push_back((int)tidx.size());
             for (size_t e = 0; e < p.edge.size(); e++) {
-                tidx.push_back((long long)p.axis * (long long)N + (long long)p.edge[e]);
+                tidx.push_back((long long)dev_index(s, g, (unsigned long long)p.axis * N + p.edge[e]));
                 tw.push_back(p.dl[e]);
             }
             post.push_back((double)p.dir);
@@ -1370,7 +1874,7 @@ int upload_plan(cem_sim *s, Device *D) {
         for (auto &p : s->vp) {
             toff.push_back((int)tidx.size());
             for (size_t e = 0; e < p.edge.size(); e++) {
-                tidx.push_back((long long)p.axis * (long long)N + (long long)p.edge[e]);
+                tidx.push_back((long long)dev_index(s, g, (unsigned long long)p.axis * N + p.edge[e]));
                 tw.push_back(p.dl[e]);
             }
             post.push_back(1.0);
@@ -1451,12 +1955,62 @@ EArgs e_args(cem_sim *s, Device *D) {
     a.cb = D->cb.p;
     a.cidx = D->compressed ? D->cidx.p : nullptr;
     a.ctab = D->compressed ? D->ctab.p : nullptr;
+    a.crow = D->compressed ? D->crow.p : nullptr;
     for (int i = 0; i < 3; i++) a.es[i] = D->es[i];
     a.pml = D->pmlE;
     for (int c = 0; c < 3; c++)
         for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiE[c][w];
     a.has_pml = s->has_pml;
+    if (D->fused) {
+        a.J = D->J.p;
+        a.btab = D->btab.p;
+        a.nrow = D->nrow;
+        a.bi0 = D->bi0;
+        a.bi1 = D->bi1;
+        a.bj0 = D->bj0;
+        a.bj1 = D->bj1;
+        a.bg0 = D->bg0;
+        a.bg1 = D->bg1;
+        for (int p = 0; p < D->npole && p < kFusedPoles; p++) {
+            a.kc[p] = D->pp_host.kc[p];
+            a.km1[p] = D->pp_host.km1[p];
+        }
+    }
     return a;
+}
+
+typedef void (*EKernel)(EArgs, int, int, int, int);
+typedef void (*HKernel)(HArgs, int, int, int, int);
+
+// E kernel of this simulation: coefficient classes or arrays, the number of
+// Debye poles it updates itself (0 when the post kernel takes them), lane width
+template <int W>
+EKernel e_kernel_w(const Device *D) {
+    if (!D->compressed) return k_update_e<0, false, W>;
+    switch (D->fused ? D->npole : 0) {
+    case 1: return k_update_e<1, true, W>;
+    case 2: return k_update_e<2, true, W>;
+    case 3: return k_update_e<3, true, W>;
+    case 4: return k_update_e<4, true, W>;
+    case 5: return k_update_e<5, true, W>;
+    case 6: return k_update_e<6, true, W>;
+    case 7: return k_update_e<7, true, W>;
+    case 8: return k_update_e<8, true, W>;
+    default: return k_update_e<0, true, W>;
+    }
+}
+
+EKernel e_kernel(const Device *D) {
+    return D->we == 1 ? e_kernel_w<1>(D) : D->we == 2 ? e_kernel_w<2>(D) : e_kernel_w<4>(D);
+}
+
+HKernel h_kernel(const Device *D) {
+    return D->wh == 1 ? k_update_h<1> : D->wh == 2 ? k_update_h<2> : k_update_h<4>;
+}
+
+// workgroups per i segment of a stencil kernel with lane width w
+int stencil_blocks(const Device *D, int w) {
+    return ((D->g.ny + 1) * (D->g.kp / w) + kBlock - 1) / kBlock;
 }
 
 // Kernel launch on the engine stream; when profiling, the launch carries a start /
@@ -1481,18 +2035,15 @@ void launch(Device *D, KernelTimer &t, F kernel, dim3 grid, dim3 block, A... arg
 void launch_h(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     HArgs a = h_args(s, D);
-    const int npblk = (D->g.S + kBlock - 1) / kBlock, nseg = (i1 - i0 + D->iseg_h - 1) / D->iseg_h;
-    launch(D, D->tH, k_update_h, dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_h, npblk);
+    const int npblk = stencil_blocks(D, D->wh), nseg = (i1 - i0 + D->iseg_h - 1) / D->iseg_h;
+    launch(D, D->tH, h_kernel(D), dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_h, npblk);
 }
 
 void launch_e(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     EArgs a = e_args(s, D);
-    const int npblk = (D->g.S + kBlock - 1) / kBlock, nseg = (i1 - i0 + D->iseg_e - 1) / D->iseg_e;
-    if (D->compressed)
-        launch(D, D->tE, k_update_e<true>, dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_e, npblk);
-    else
-        launch(D, D->tE, k_update_e<false>, dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_e, npblk);
+    const int npblk = stencil_blocks(D, D->we), nseg = (i1 - i0 + D->iseg_e - 1) / D->iseg_e;
+    launch(D, D->tE, e_kernel(D), dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_e, npblk);
 }
 
 void launch_dft(cem_sim *s, Device *D, int direct);
@@ -1506,7 +2057,7 @@ PostArgs post_args(cem_sim *s, Device *D) {
     a.si = D->g.si;
     a.sj = D->g.sj;
     a.pp = D->pp.p;
-    a.ndisp = s->ndisp;
+    a.ndisp = D->nlisted;
     a.nsb = s->nsb;
     a.nsbo = D->nsbo;
     a.nxs = D->nxs;
@@ -1523,6 +2074,8 @@ PostArgs post_args(cem_sim *s, Device *D) {
     a.sbo = D->sbo.p;
     a.sb_src = D->has_sbsrc ? D->sb_src.p : nullptr;
     a.J = D->J.p;
+    a.d_jb = D->fused ? D->d_jb.p : nullptr;
+    a.jstride = D->fused ? 1 : (long long)s->ndisp;
     a.Eold = D->Eold.p;
     a.sb_idx = D->sb_idx.p;
     a.face = D->face.p;
@@ -1566,7 +2119,7 @@ GatherArgs gather_args(cem_sim *s, Device *D, int mid_only) {
 }
 
 int post_blocks(cem_sim *s, Device *D) {
-    return (s->ndisp + D->nsbo + D->nxs + kBlock - 1) / kBlock;
+    return (D->nlisted + D->nsbo + D->nxs + kBlock - 1) / kBlock;
 }
 
 // The device counters follow the host's step and row; after a host-side change
@@ -1647,9 +2200,9 @@ int ensure_vlast(cem_sim *s, Device *D) {
         launch_gather_mid(s, D);
         D->vlast_stale = false;
     }
-    if (D->eold_stale && s->ndisp) {
-        hipLaunchKernelGGL(k_prime_eold, dim3((s->ndisp + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, D->F.p,
-                           D->d_idx.p, D->Eold.p, s->ndisp);
+    if (D->eold_stale && D->nlisted) {
+        hipLaunchKernelGGL(k_prime_eold, dim3((D->nlisted + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, D->F.p,
+                           D->d_idx.p, D->Eold.p, D->nlisted);
     }
     D->eold_stale = false;
     return 0;
@@ -1694,7 +2247,7 @@ int run_chunk(cem_sim *s, Device *D) {
     const EArgs ea = e_args(s, D);
     const PostArgs pa = post_args(s, D);
     const GatherArgs ga = gather_args(s, D, 0);
-    const int geo[4] = {D->iseg_h, D->iseg_e, post_blocks(s, D), (int)D->compressed};
+    const int geo[7] = {D->iseg_h, D->iseg_e, post_blocks(s, D), (int)D->compressed, (int)D->fused, D->wh, D->we};
     std::vector<unsigned char> key;
     for (auto pr : {std::make_pair((const void *)&ha, sizeof ha), std::make_pair((const void *)&ea, sizeof ea),
                     std::make_pair((const void *)&pa, sizeof pa), std::make_pair((const void *)&ga, sizeof ga),
@@ -1709,15 +2262,14 @@ int run_chunk(cem_sim *s, Device *D) {
             D->use_graph = false;
             return -1;
         }
-        const int npblk = (D->g.S + kBlock - 1) / kBlock;
-        const int gh = npblk * ((nx + 1 + D->iseg_h - 1) / D->iseg_h);
-        const int ge = npblk * ((nx + D->iseg_e - 1) / D->iseg_e);
+        const int nbh = stencil_blocks(D, D->wh), nbe = stencil_blocks(D, D->we);
+        const EKernel ek = e_kernel(D);
+        const HKernel hk = h_kernel(D);
+        const int gh = nbh * ((nx + 1 + D->iseg_h - 1) / D->iseg_h);
+        const int ge = nbe * ((nx + D->iseg_e - 1) / D->iseg_e);
         for (int k = 0; k < kGraphSteps; k++) {
-            hipLaunchKernelGGL(k_update_h, dim3(gh), dim3(kBlock), 0, D->st, ha, 0, nx + 1, D->iseg_h, npblk);
-            if (D->compressed)
-                hipLaunchKernelGGL(k_update_e<true>, dim3(ge), dim3(kBlock), 0, D->st, ea, 0, nx, D->iseg_e, npblk);
-            else
-                hipLaunchKernelGGL(k_update_e<false>, dim3(ge), dim3(kBlock), 0, D->st, ea, 0, nx, D->iseg_e, npblk);
+            hipLaunchKernelGGL(hk, dim3(gh), dim3(kBlock), 0, D->st, ha, 0, nx + 1, D->iseg_h, nbh);
+            hipLaunchKernelGGL(ek, dim3(ge), dim3(kBlock), 0, D->st, ea, 0, nx, D->iseg_e, nbe);
             if (geo[2]) hipLaunchKernelGGL(k_post, dim3(geo[2]), dim3(kBlock), 0, D->st, pa);
             hipLaunchKernelGGL(k_gather, dim3(1), dim3(kGatherBlock), 0, D->st, ga);
         }
@@ -1774,6 +2326,11 @@ int cem::dev_ensure(cem_sim *s) {
     }
     D->profiling = std::getenv("COURANT_PROFILE") != nullptr;
     if (const char *v = std::getenv("COURANT_GRAPH")) D->use_graph = std::atoi(v) != 0;
+    for (auto pr : {std::make_pair("COURANT_LANES_H", &D->wh), std::make_pair("COURANT_LANES_E", &D->we)})
+        if (const char *v = std::getenv(pr.first)) {
+            const int w = std::atoi(v);
+            *pr.second = w == 1 || w == 2 ? w : 4;
+        }
     if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v)), D->tuned = true;
     if (const char *v = std::getenv("COURANT_ISEG_E")) D->iseg_e = std::max(1, std::atoi(v)), D->tuned = true;
     if (const char *v = std::getenv("COURANT_AUTOTUNE"))
@@ -1801,6 +2358,9 @@ void cem::dev_destroy(cem_sim *s) {
     D->cb.release();
     D->cidx.release();
     D->ctab.release();
+    D->crow.release();
+    D->btab.release();
+    D->d_jb.release();
     D->axis1d.release();
     D->slots.release();
     D->psi.release();
@@ -1860,10 +2420,11 @@ constexpr int kTuneRounds = 2;
 constexpr long kTuneCost = 1 + (long)kTuneRounds * kTuneNc * kTuneNc * (1 + kTuneSteps);
 
 std::mutex g_tune_mu;
-std::map<std::array<long long, 5>, std::pair<int, int>> g_tune;
+std::map<std::array<long long, 8>, std::pair<int, int>> g_tune;
 
 long autotune(cem_sim *s, Device *D, long budget) {
-    const std::array<long long, 5> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed};
+    const std::array<long long, 8> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
+                                          (long long)D->fused, D->wh, D->we};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
@@ -1990,37 +2551,85 @@ double cem::dev_energy(cem_sim *s) {
     return W;
 }
 
+// Reference layout (i si + j sj + k) <-> device rows of kp floats (padding zero).
+static void rows_to_host(const cem_sim *s, const Grid &g, const float *dev, double *dst) {
+    for (int i = 0; i <= s->n[0]; i++)
+        for (int j = 0; j <= s->n[1]; j++) {
+            const float *r = dev + (size_t)i * g.si + (size_t)j * g.sj;
+            double *o = dst + (size_t)i * s->si + (size_t)j * s->sj;
+            for (int k = 0; k <= s->n[2]; k++) o[k] = r[k];
+        }
+}
+
+static void rows_to_device(const cem_sim *s, const Grid &g, const double *src, float *dev) {
+    for (int i = 0; i <= s->n[0]; i++)
+        for (int j = 0; j <= s->n[1]; j++) {
+            float *r = dev + (size_t)i * g.si + (size_t)j * g.sj;
+            const double *o = src + (size_t)i * s->si + (size_t)j * s->sj;
+            for (int k = 0; k <= s->n[2]; k++) r[k] = (float)o[k];
+        }
+}
+
+static std::vector<float> download_raw(Device *D, const float *src, size_t n) {
+    std::vector<float> t(n);
+    copy_sync(D, t.data(), src, n * sizeof(float), hipMemcpyDeviceToHost);
+    return t;
+}
+
+static void upload_raw(Device *D, float *dst, const std::vector<float> &t) {
+    copy_sync(D, dst, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+}
+
 void cem::dev_field_download(cem_sim *s, int comp, double *dst) {
     Device *D = s->dev;
     if (!D) {
         std::memset(dst, 0, s->N * sizeof(double));
         return;
     }
-    download_f32(D, D->F.p + (size_t)comp * s->N, dst, s->N);
+    const std::vector<float> t = download_raw(D, D->F.p + (size_t)comp * D->g.N, (size_t)D->g.N);
+    rows_to_host(s, D->g, t.data(), dst);
 }
 
-// slab planes of a z-derivative psi array (stored slot-fastest), 0 for every other state
-static int psi_z_planes(const cem_sim *s, const Device *D, int which, int comp, int w) {
-    if ((which != 0 && which != 1) || (comp + w + 1) % 3 != 2 || !s->has_pml) return 0;
-    return which == 0 ? D->pmlE.zpl : D->pmlH.zpl;
-}
-
-static float *state_ptr(cem_sim *s, int which, int comp, int w, size_t *len) {
-    Device *D = s->dev;
-    switch (which) {
-    case 0: *len = D->psiE_len[comp][w]; return D->psiE[comp][w];
-    case 1: *len = D->psiH_len[comp][w]; return D->psiH[comp][w];
-    case 2: *len = (size_t)s->npole * s->ndisp; return D->J.p;
-    case 5: *len = s->N; return D->F.p + (size_t)comp * s->N;
-    default: *len = 0; return nullptr;
-    }
-}
-
+// Checkpoint / state arrays in the reference's layouts (export.c:187-271):
+// which 0 / 1 psi_E / psi_H (comp, w), 2 Debye J pole-major [p ndisp + e],
+// 3 Debye E_old, 4 SIBC states edge-major, 5 field comp.
 void cem::dev_state_download(cem_sim *s, int which, int comp, int w, double *dst) {
     Device *D = s->dev;
+    const Grid &g = D->g;
+    const size_t nd = s->ndisp;
+    if (which == 0 || which == 1) {
+        const int is_e = which == 0;
+        float *p = is_e ? D->psiE[comp][w] : D->psiH[comp][w];
+        const size_t len = is_e ? D->psiE_len[comp][w] : D->psiH_len[comp][w];
+        if (!len) return;
+        const std::vector<float> t = download_raw(D, p, len);
+        const std::vector<size_t> m = psi_dev_map(s, D, is_e, comp, w);
+        for (size_t r = 0; r < m.size(); r++) dst[r] = t[m[r]];
+        return;
+    }
+    if (which == 2) {       // Debye J: the reference's pole-major array or the E kernel's box
+        const size_t np = s->npole;
+        if (!D->fused) {
+            download_f32(D, D->J.p, dst, np * nd);
+            return;
+        }
+        const std::vector<float> t = download_raw(D, D->J.p, D->J.n);
+        for (size_t p = 0; p < np; p++)
+            for (size_t e = 0; e < nd; e++) dst[p * nd + e] = t[D->jbox_of[e] + p];
+        return;
+    }
     if (which == 3) {       // Debye E_old: E of each dispersive edge before the next E update
-        ensure_vlast(s, D);
-        download_f32(D, D->Eold.p, dst, s->ndisp);
+        if (!D->fused) {
+            ensure_vlast(s, D);
+            download_f32(D, D->Eold.p, dst, nd);
+            return;
+        }
+        // the E kernel reads E_old from E itself; listed edges hold the same value
+        const std::vector<float> t = download_raw(D, D->F.p, 3 * (size_t)g.N);
+        for (size_t e = 0; e < nd; e++) {
+            const unsigned long long gi = (unsigned long long)s->d_comp[e] * s->N + s->d_idx[e];
+            dst[e] = t[dev_index(s, g, gi)];
+        }
         return;
     }
     if (which == 4) {       // SIBC states: device term-major -> reference edge-major
@@ -2031,23 +2640,36 @@ void cem::dev_state_download(cem_sim *s, int which, int comp, int w, double *dst
             for (size_t k = 0; k < K; k++) dst[e * K + k] = t[k * n + e];
         return;
     }
-    size_t len = 0;
-    float *p = state_ptr(s, which, comp, w, &len);
-    if (!p) return;
-    const int zpl = psi_z_planes(s, D, which, comp, w);
-    if (!zpl) {
-        download_f32(D, p, dst, len);
-        return;
-    }
-    std::vector<double> t(len);     // device slot-fastest -> reference slab-major
-    download_f32(D, p, t.data(), len);
-    const size_t per = len / zpl;
-    for (size_t r = 0; r < per; r++)
-        for (int z = 0; z < zpl; z++) dst[z * per + r] = t[r * zpl + z];
+    if (which == 5) dev_field_download(s, comp, dst);
 }
 
 void cem::dev_state_upload(cem_sim *s, int which, int comp, int w, const double *src) {
     Device *D = s->dev;
+    const Grid &g = D->g;
+    const size_t nd = s->ndisp;
+    if (which == 0 || which == 1) {
+        const int is_e = which == 0;
+        float *p = is_e ? D->psiE[comp][w] : D->psiH[comp][w];
+        const size_t len = is_e ? D->psiE_len[comp][w] : D->psiH_len[comp][w];
+        if (!len) return;
+        std::vector<float> t(len, 0.f);
+        const std::vector<size_t> m = psi_dev_map(s, D, is_e, comp, w);
+        for (size_t r = 0; r < m.size(); r++) t[m[r]] = (float)src[r];
+        upload_raw(D, p, t);
+        return;
+    }
+    if (which == 2) {
+        const size_t np = s->npole;
+        if (!D->fused) {
+            upload_f32(D, D->J.p, src, np * nd);
+            return;
+        }
+        std::vector<float> t(D->J.n, 0.f);
+        for (size_t p = 0; p < np; p++)
+            for (size_t e = 0; e < nd; e++) t[D->jbox_of[e] + p] = (float)src[p * nd + e];
+        upload_raw(D, D->J.p, t);
+        return;
+    }
     if (which == 3) {        // E_old is taken from E again before the next E update
         D->eold_stale = true;
         return;
@@ -2060,21 +2682,12 @@ void cem::dev_state_upload(cem_sim *s, int which, int comp, int w, const double 
         upload_f32(D, D->Y.p, t.data(), K * n);
         return;
     }
-    size_t len = 0;
-    float *p = state_ptr(s, which, comp, w, &len);
-    if (p) {
-        const int zpl = psi_z_planes(s, D, which, comp, w);
-        if (!zpl) {
-            upload_f32(D, p, src, len);
-        } else {
-            std::vector<double> t(len);
-            const size_t per = len / zpl;
-            for (size_t r = 0; r < per; r++)
-                for (int z = 0; z < zpl; z++) t[r * zpl + z] = src[z * per + r];
-            upload_f32(D, p, t.data(), len);
-        }
+    if (which == 5) {
+        std::vector<float> t((size_t)g.N, 0.f);
+        rows_to_device(s, g, src, t.data());
+        upload_raw(D, D->F.p + (size_t)comp * g.N, t);
+        if (comp < 3) D->vlast_stale = D->eold_stale = true;
     }
-    if (which == 5 && comp < 3) D->vlast_stale = D->eold_stale = true;
 }
 
 void cem::dev_sync_in(cem_sim *s) {
