@@ -13,7 +13,9 @@
  * so results agree with the reference core to the last bit (checked in tests).
  * ORC_REAL = float gives an FP32 emulation of the GPU step; with ORC_HALF_STORE as
  * well, every stored state is rounded to binary16 (the accuracy check of a
- * half-precision storage variant).
+ * half-precision storage variant), and with ORC_HALF_J_ONLY too, only the Debye
+ * currents are (the check of a variant that halves just the largest auxiliary
+ * array).
  */
 #include "oracle.h"
 
@@ -502,7 +504,7 @@ static double mode_wave(const orc_sim *s, double t) {
     return -2 * x * exp(-x * x);
 }
 
-#ifdef ORC_HALF_STORE
+#if defined(ORC_HALF_STORE) && !defined(ORC_HALF_J_ONLY)
 static void half_store_psi(orc_sim *s, int is_e) {
     if (!s->has_pml) return;
     for (int c = 0; c < 3; c++)
@@ -527,7 +529,7 @@ void orc_step(orc_sim *s, long nsteps) {
             }
         }
         step_H(s);
-#ifdef ORC_HALF_STORE
+#if defined(ORC_HALF_STORE) && !defined(ORC_HALF_J_ONLY)
         half_store(s->F + 3 * s->N, 3 * s->N, HALF_SCALE_H);
         half_store_psi(s, 0);
 #endif
@@ -536,7 +538,7 @@ void orc_step(orc_sim *s, long nsteps) {
         for (int p = 0; p < s->nport; p++) s->vold[p] = sum_terms(s, &s->pv[p]);
         for (int e = 0; e < s->ndisp; e++) s->d_Eold[e] = s->F[s->d_idx[e]];
         step_E(s);
-#ifdef ORC_HALF_STORE
+#if defined(ORC_HALF_STORE) && !defined(ORC_HALF_J_ONLY)
         half_store(s->F, 3 * s->N, HALF_SCALE_E);
         half_store_psi(s, 1);
 #endif
@@ -579,7 +581,9 @@ void orc_step(orc_sim *s, long nsteps) {
                 }
             }
         }
-#ifdef ORC_HALF_STORE
+#if defined(ORC_HALF_J_ONLY)
+        half_store(s->d_J, (size_t)s->npole * s->ndisp, HALF_SCALE_J);
+#elif defined(ORC_HALF_STORE)
         half_store(s->F, 3 * s->N, HALF_SCALE_E);
         half_store(s->d_J, (size_t)s->npole * s->ndisp, HALF_SCALE_J);
         half_store(s->sb_y, (size_t)s->nsibc * s->nterm, HALF_SCALE_Y);

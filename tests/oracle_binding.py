@@ -9,6 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_F64 = os.path.join(ROOT, "oracle", "liboracle.so")
 ORACLE_F32 = os.path.join(ROOT, "oracle", "liboracle_f32.so")
 ORACLE_F16S = os.path.join(ROOT, "oracle", "liboracle_f16s.so")   # FP16 storage, FP32 arithmetic
+ORACLE_F16J = os.path.join(ROOT, "oracle", "liboracle_f16j.so")   # only the Debye currents in FP16
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int)
