@@ -2893,6 +2893,25 @@ extern "C" int cem_amd_step_device(cem_sim *s, long n) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Run the launch-shape autotuner now (its steps advance the simulation like
+// ordinary steps); returns the steps it ran, 0 when the shape was already fixed.
+extern "C" long cem_amd_tune(cem_sim *s) {
+    if (!s || !s->finalized || dev_ensure(s)) return -1;
+    Device *D = s->dev;
+    if (D->tuned || D->profiling) return 0;
+    ensure_vlast(s, D);
+    const long n = autotune(s, D, kTuneCost);
+    return hipGetLastError() == hipSuccess ? n : -1;
+}
+
+// Accumulate the buffered samples of the steps run so far into the DFT (queued
+// on the simulation's stream, no host transfer).
+extern "C" int cem_amd_flush_dft(cem_sim *s) {
+    if (!s || !s->dev) return -1;
+    launch_dft(s, s->dev, 0);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // 2x2 renormalisation of nf frequencies on the device, FP64, in place.
 extern "C" int cem_amd_s2_renormalize(double *S, int nf, const double *z1, const double *z2, double zref) {
     if (!S || !z1 || !z2 || nf < 0) {

@@ -129,6 +129,8 @@ AMD_SIGNATURES = {
     "cem_amd_set_profiling": (_I, [_P, _I]),
     "cem_amd_synchronize": (_I, [_P]),
     "cem_amd_step_device": (_I, [_P, _L]),
+    "cem_amd_tune": (_L, [_P]),
+    "cem_amd_flush_dft": (_I, [_P]),
     "cem_amd_copy_bandwidth": (_D, [_S, _I]),
     "cem_amd_s2_renormalize": (_I, [_DP, _I, _DP, _DP, _D]),
     "cem_amd_s2_renormalize_noalloc": (_I, [_DP, _I, _DP, _DP, _D]),
@@ -457,6 +459,14 @@ class Sim:
 
     def synchronize(self):
         return self._ok(self.lib.cem_amd_synchronize(self.h))
+
+    def tune(self):
+        """Run the launch-shape autotuner now; returns the steps it ran."""
+        return self._ok(self.lib.cem_amd_tune(self.h))
+
+    def flush_dft(self):
+        """Queue the DFT of the samples buffered so far (no host transfer)."""
+        return self._ok(self.lib.cem_amd_flush_dft(self.h))
 
     def set_profiling(self, on):
         return self._ok(self.lib.cem_amd_set_profiling(self.h, 1 if on else 0))

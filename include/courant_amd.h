@@ -17,6 +17,18 @@ extern "C" {
  * spectra (timed loops).  Returns 0, or -1 with cem_last_error() set. */
 CEM_API int cem_amd_step_device(cem_sim *s, long n);
 
+/* Run the launch-shape autotuner now instead of on the first long enough
+ * cem_step / cem_amd_step_device call; its steps advance the simulation like
+ * ordinary steps.  Returns the number of steps it ran (0 when the shape was
+ * already fixed), or -1 with cem_last_error() set. */
+CEM_API long cem_amd_tune(cem_sim *s);
+
+/* Queue the running DFT of every sample buffered since the last flush on the
+ * simulation's stream (the engine otherwise flushes every 512 steps and when
+ * spectra are read).  Returns 0, or -1 if the simulation has not run on the
+ * GPU. */
+CEM_API int cem_amd_flush_dft(cem_sim *s);
+
 /* Wait until every queued kernel of this simulation has finished. */
 CEM_API int cem_amd_synchronize(cem_sim *s);
 
