@@ -407,7 +407,10 @@ extern "C" int cem_mode2d_solve(cem_mode2d *m, double f, double neff_guess, cons
             if (order_pec(m, o, g)) B.at(g, g) = big;
             B.at(g, g) -= sigma;
         }
-        if (band_lu(B)) return -1;
+        if (band_lu(B)) {
+            set_error("mode solver: singular pivot in the band LU");
+            return -1;
+        }
         cplx lam_old = 0.0;
         std::vector<cplx> xo(n, cplx(0.0));
         for (int it = 0; it < maxit; it++) {

@@ -1104,7 +1104,10 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
                      out[4], out[5], out[6], out[7], out[8], out[9]);
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
-    if (info) return -1;   // singular pivot: the host path reports the same as a failed LU
+    if (info) {   // singular pivot: a failed LU, as on the host path
+        set_error("mode solver: singular pivot in the band LU");
+        return -1;
+    }
     for (int i = 0; i < n; i++) xout[i] = std::complex<double>(M.hx.p[i].r, M.hx.p[i].i);
     *lam = std::complex<double>(out[0], out[1]);
     return out[2] == 0.0 ? 0 : 1;

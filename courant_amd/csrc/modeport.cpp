@@ -92,28 +92,44 @@ int cem::modeports_finalize(cem_sim *s) {
     // the eigen-solve at warp(f0), retried at scaled effective indices; the ports are
     // independent, so with the GPU solver they solve at once, one host thread and one
     // device context each, as the frequency sweeps do
-    auto solve_f0 = [s](ModePort &p) {
-        for (int t = 0; t < 5; t++)
-            if (!cem_mode2d_solve(p.xs, warp(s, p.def.f0), p.def.neff0 * tries[t], nullptr, 300, 1e-13)) {
+    // Returns 1 when solved, 0 when no try converged, -1 on a solver error (device or
+    // memory, or a singular band), whose message is kept in err: the error buffer is
+    // per thread, so a solving thread hands its message to the caller this way.
+    auto solve_f0 = [s](ModePort &p, std::string &err) {
+        for (int t = 0; t < 5; t++) {
+            const int rc = cem_mode2d_solve(p.xs, warp(s, p.def.f0), p.def.neff0 * tries[t], nullptr, 300, 1e-13);
+            if (rc == 0) {
                 p.def.neff0 *= tries[t];
-                return true;
+                return 1;
             }
-        return false;
+            if (rc < 0) {
+                err = cem_last_error();
+                return -1;
+            }
+        }
+        return 0;
     };
-    std::vector<char> solved(s->mport.size(), 0);
+    std::vector<int> solved(s->mport.size(), 0);
+    std::vector<std::string> errs(s->mport.size());
     if (mode2d_on_gpu() && s->mport.size() > 1) {
         std::vector<std::thread> th;
         for (size_t k = 0; k < s->mport.size(); k++)
-            th.emplace_back([&solve_f0, &solved, s, k]() { solved[k] = solve_f0(s->mport[k]); });
+            th.emplace_back([&solve_f0, &solved, &errs, s, k]() { solved[k] = solve_f0(s->mport[k], errs[k]); });
         for (auto &t : th) t.join();
     } else {
-        for (size_t k = 0; k < s->mport.size(); k++) solved[k] = solve_f0(s->mport[k]);
+        for (size_t k = 0; k < s->mport.size(); k++) solved[k] = solve_f0(s->mport[k], errs[k]);
     }
-    for (char ok : solved)
-        if (!ok) {
+    for (size_t k = 0; k < solved.size(); k++) {
+        if (solved[k] < 0) {
+            const std::string e = "mode port: " + (errs[k].empty() ? std::string("eigen-solver failure") : errs[k]);
+            set_error(e.c_str());
+            return -1;
+        }
+        if (solved[k] == 0) {
             set_error("mode port: eigenmode did not converge");
             return -1;
         }
+    }
     for (auto &p : s->mport) {
         const cem_mode_port_def &d = p.def;
         const int ny = d.j1 - d.j0, nz = d.k1 - d.k0;

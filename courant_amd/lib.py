@@ -157,6 +157,18 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+class _View(np.ndarray):
+    """A numpy view of library-owned memory that keeps the owning Sim alive (the
+    memory is freed when the Sim is closed or collected)."""
+    pass
+
+
+def _owned_view(ptr, n, owner):
+    v = np.ctypeslib.as_array(ptr, shape=(n,)).view(_View)
+    v._owner = owner
+    return v
+
+
 class Courant:
     """A loaded courant.h library (product engine or a checker with the same ABI)."""
 
@@ -362,7 +374,9 @@ class Sim:
     def field(self, comp):
         """Live FP64 view of field component comp (0..2 E, 3..5 H), shape (nx+1, ny+1, nz+1)."""
         ptr = self.lib.cem_field(self.h, comp)
-        return np.ctypeslib.as_array(ptr, shape=(self.N,)).reshape(self.n[0] + 1, self.n[1] + 1, self.n[2] + 1)
+        if not ptr:
+            raise CourantError(self.cem.last_error() or "cem_field returned NULL")
+        return _owned_view(ptr, self.N, self).reshape(self.n[0] + 1, self.n[1] + 1, self.n[2] + 1)
 
     def index(self, i, j, k):
         return self.lib.cem_index(self.h, i, j, k)

@@ -23,6 +23,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <iterator>
+#include <cmath>
+#include <initializer_list>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -206,6 +208,26 @@ struct Call {
     double D(size_t k) const { return num(k); }
     size_t Z(size_t k) const { return (size_t)num(k); }
     void *Ptr(size_t k) const { return in->ptr((uint32_t)num(k)); }
+    // Every pointer argument must be 0 (NULL) or an offset inside a block of the arena
+    // that malloc or a returned array handed out (wasm memory would trap on anything
+    // else); otherwise a RangeError is thrown and the library is not called.
+    bool ptrs_ok(std::initializer_list<size_t> ks) const {
+        for (size_t k : ks) {
+            const double v = num(k);
+            if (v == 0) continue;
+            bool ok = std::isfinite(v) && v > 0 && v == std::floor(v) && v < (double)in->arena_bytes;
+            if (ok) {
+                const uint32_t off = (uint32_t)v;
+                auto it = in->used.upper_bound(off);
+                ok = it != in->used.begin() && off < std::prev(it)->first + std::prev(it)->second;
+            }
+            if (!ok) {
+                napi_throw_range_error(e, nullptr, "courant: pointer argument outside the arena's allocated blocks");
+                return false;
+            }
+        }
+        return true;
+    }
     cem_sim *S(size_t k) const {
         const size_t h = (size_t)num(k);
         return h < in->sims.size() ? in->sims[h] : nullptr;

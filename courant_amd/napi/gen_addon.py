@@ -81,6 +81,9 @@ def wrapper(ret, name, params):
         L.append("    const ModeH *mh = a.MH(0);")
         L.append("    if (!mh) return ret_num(e, 0);")
     call = f"{name}({args})"
+    ptr_args = [str(k) for k, t in enumerate(types) if "*" in t and "cem_sim" not in t and "cem_mode2d" not in t]
+    if ptr_args:
+        L.append(f"    if (!a.ptrs_ok({{{', '.join(ptr_args)}}})) return ret_void(e);")
     r = ret.replace(" ", "")
     if name == "cem_destroy":
         L += ["    a.in->drop_mirrors(s0);", f"    {call};", "    a.release_sim(0);", "    return ret_void(e);"]

@@ -74,6 +74,10 @@ function exportsReport() {
   // error path through the arena string
   out.bad_port = ex.cem_add_lumped_port(s, 2, 10, 20, 6, 12, -1, 50);
   out.error = lastError();
+  // pointer arguments outside the arena's allocated blocks are refused with a RangeError
+  const rejected = (fn) => { try { fn(); return false; } catch (e) { return e instanceof RangeError; } };
+  out.bad_ptr = [out.arena_bytes + 64, -8, 1.5, out.arena_bytes - 64].map(
+    (p) => rejected(() => ex.cem_plan_port(s, 0, p, 0, 0)));
   // allocator: blocks are distinct, freed blocks are reused
   const a = ex.malloc(1000), b = ex.malloc(1000);
   ex.free(a);

@@ -76,6 +76,7 @@ def test_addon_exports_the_whole_abi(cem):
     ide = np.ctypeslib.as_array(cem.lib.cem_plan_array(s.h, 2, 1), shape=(41,))
     assert out["ide_y"] == ide.tolist()
     assert out["bad_port"] == -1 and out["error"] == "already finalized"
+    assert out["bad_ptr"] == [True, True, True, True]
     assert out["alloc"] == {"distinct": True, "reused": True}
 
 
