@@ -319,6 +319,39 @@ extern "C" void cem_mode2d_set_pml(cem_mode2d *m, int which, const double *sigma
     std::copy(alpha, alpha + len, m->pa[which].begin());
 }
 
+// The shared tail of a solve (mode2d.c:376-399): normalise, fix the global phase so the
+// largest component is real, scatter into E_u / E_v, beta = sqrt(lambda) with a
+// positive real part, H = Q E / beta.
+static void finish_solve(cem_mode2d *m, const Coef &c, const Order &o, double w, double f, const std::vector<cplx> &x,
+                         cplx lam, Work &W) {
+    const int n = o.n;
+    // normalise; global phase so the largest component is real
+    double nx2 = 0, amax = 0;
+    cplx ph = 1.0;
+    for (int g = 0; g < n; g++) {
+        nx2 += (x[g] * std::conj(x[g])).real();
+        if (std::abs(x[g]) > amax) {
+            amax = std::abs(x[g]);
+            ph = std::conj(x[g]) / amax;
+        }
+    }
+    double inv = 1 / std::sqrt(nx2);
+    std::fill(m->eu.begin(), m->eu.end(), cplx(0.0));
+    std::fill(m->ev.begin(), m->ev.end(), cplx(0.0));
+    for (int g = 0; g < n; g++) {
+        if (order_pec(m, o, g)) continue;
+        if (o.kind[g] == 0) m->eu[o.idx[g]] = x[g] * inv * ph;
+        else m->ev[o.idx[g]] = x[g] * inv * ph;
+    }
+    cplx bt = std::sqrt(lam);
+    if (bt.real() < 0) bt = -bt;
+    m->beta = bt;
+    m->freq = f;
+    apply_Q(m, c, w, m->eu.data(), m->ev.data(), m->hu.data(), m->hv.data(), W.wk.data());
+    for (auto &v : m->hu) v /= bt;
+    for (auto &v : m->hv) v /= bt;
+}
+
 extern "C" int cem_mode2d_solve(cem_mode2d *m, double f, double neff_guess, const double *start, int maxit,
                                 double tol) {
     const int ny = m->ny, nz = m->nz;
@@ -445,32 +478,76 @@ extern "C" int cem_mode2d_solve(cem_mode2d *m, double f, double neff_guess, cons
             lam_old = lam;
         }
     }
-    // normalise; global phase so the largest component is real
-    double nx2 = 0, amax = 0;
-    cplx ph = 1.0;
-    for (int g = 0; g < n; g++) {
-        nx2 += (x[g] * std::conj(x[g])).real();
-        if (std::abs(x[g]) > amax) {
-            amax = std::abs(x[g]);
-            ph = std::conj(x[g]) / amax;
-        }
-    }
-    double inv = 1 / std::sqrt(nx2);
-    std::fill(m->eu.begin(), m->eu.end(), cplx(0.0));
-    std::fill(m->ev.begin(), m->ev.end(), cplx(0.0));
-    for (int g = 0; g < n; g++) {
-        if (order_pec(m, o, g)) continue;
-        if (o.kind[g] == 0) m->eu[o.idx[g]] = x[g] * inv * ph;
-        else m->ev[o.idx[g]] = x[g] * inv * ph;
-    }
-    cplx bt = std::sqrt(lam);
-    if (bt.real() < 0) bt = -bt;
-    m->beta = bt;
-    m->freq = f;
-    apply_Q(m, c, w, m->eu.data(), m->ev.data(), m->hu.data(), m->hv.data(), W.wk.data());
-    for (auto &v : m->hu) v /= bt;
-    for (auto &v : m->hv) v /= bt;
+    finish_solve(m, c, o, w, f, x, lam, W);
     return rc;
+}
+
+// A batch of solves of one cross-section geometry on the GPU in one launch sequence:
+// ms[b] (distinct objects of the same geometry, e.g. clones) receives the mode at f[b],
+// with shift shift[b] (complex) and start vector start[b] (nullptr: the fixed LCG
+// vector).  The
+// device context of ms[0] runs the batch.  rcs[b]: 0 converged, 1 iteration limit;
+// its[b] (if not null): iterations run.
+// Returns 0, or -1 with the error set.  Each result is the one cem_mode2d_solve gives
+// for the same arguments, up to the rounding of the device's reductions.
+int cem::mode2d_solve_batch(cem_mode2d *const *ms, int nb, const double *f, const std::complex<double> *shift,
+                            const double *const *start, int maxit, double tol, int *rcs, int *its) {
+    if (nb < 1) return 0;
+    cem_mode2d *m0 = ms[0];
+    const int ny = m0->ny, nz = m0->nz;
+    Order o;
+    make_order(m0, o);
+    const int n = o.n;
+    std::vector<Coef> c(nb);
+    std::vector<std::vector<cplx>> x(nb);
+    std::vector<cem::ModeGpuProblem> P(nb);
+    std::vector<cplx> lam(nb);
+    std::vector<cplx *> xo(nb);
+    try {
+        for (int b = 0; b < nb; b++) {
+            const double w = 2 * M_PI * f[b];
+            const cplx sigma = shift[b];
+            make_coef(ms[b], w, c[b]);
+            x[b].assign(n, cplx(0.0));
+            if (start[b]) {
+                const cplx *st = reinterpret_cast<const cplx *>(start[b]);
+                for (int g = 0; g < n; g++) x[b][g] = o.kind[g] == 0 ? st[o.idx[g]] : st[ny * (nz + 1) + o.idx[g]];
+            } else {
+                unsigned sd = 12345u;   // fixed LCG start vector (mode2d.c:345-346)
+                for (int g = 0; g < n; g++) {
+                    sd = sd * 1103515245u + 12345u;
+                    x[b][g] = ((sd >> 8) & 0xffff) / 65536.0 - 0.5;
+                }
+            }
+            cem::ModeGpuProblem &q = P[b];
+            q.ny = ny;
+            q.nz = nz;
+            q.w = w;
+            q.iyc = c[b].iyc.data();
+            q.iyn = c[b].iyn.data();
+            q.izc = c[b].izc.data();
+            q.izn = c[b].izn.data();
+            q.eu = c[b].eu.data();
+            q.ev = c[b].ev.data();
+            q.iw = c[b].iw.data();
+            q.pec_u = m0->pec_u.data();
+            q.pec_v = m0->pec_v.data();
+            q.sigma = sigma;
+            q.big = -1e6 * (std::abs(sigma) + 1.0);
+            q.x0 = x[b].data();
+            q.maxit = maxit;
+            q.tol = tol;
+            xo[b] = x[b].data();
+        }
+    } catch (const std::bad_alloc &) {
+        set_error("out of memory");
+        return -1;
+    }
+    if (cem::mode_gpu_solve_batch(&m0->gpu, P.data(), nb, xo.data(), lam.data(), rcs, its)) return -1;
+    Work W;
+    W.wk.assign((size_t)(ny + 1) * (nz + 1), cplx(0.0));
+    for (int b = 0; b < nb; b++) finish_solve(ms[b], c[b], o, 2 * M_PI * f[b], f[b], x[b], lam[b], W);
+    return 0;
 }
 
 cem_mode2d *cem::mode2d_clone(const cem_mode2d *m) {

@@ -40,7 +40,7 @@ namespace cem {
 namespace {
 
 constexpr double kMu0 = CEM_MU0;
-constexpr int kLuThreads = 256;
+constexpr int kLuThreads = 1024;       // trailing update: one wavefront per column, 16 columns per workgroup share one LDS copy of the panel
 constexpr int kTrailRows = 8;          // rows per lane of a trailing column (panel rows <= 512)
 constexpr int kInvThreads = 512;
 constexpr size_t kRingBytes = 150 * 1024;  // LDS ring of band columns for the triangular solves (gfx950: 160 KiB per workgroup)
@@ -117,6 +117,9 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 struct Geo {
     int ny, nz, nu, nv, nn, nc, n, bw, ldab, wc;
+    // a batch of problems (frequencies) of one geometry solves in one launch
+    // sequence: problem blockIdx.y finds its arrays at these strides (entries)
+    long long sband, svec, scoef, shw, shu, shv, sew;
 };
 
 // dst[t] = src(t) for t = t0, t0 + nt, ... < total: each thread issues U loads
@@ -154,7 +157,22 @@ struct Coef {
     const zc *iyc, *iyn, *izc, *izn, *eu, *ev, *iw;
     const unsigned char *pec_u, *pec_v;
     double w;
+    const double *wv;       // angular frequency of each problem of the batch
 };
+
+// the coefficients of problem b of the batch (the PEC masks are the geometry's)
+__device__ __forceinline__ Coef coef_of(const Geo &g, Coef c, int b) {
+    const long long o = (long long)b * g.scoef;
+    c.iyc += o;
+    c.iyn += o;
+    c.izc += o;
+    c.izn += o;
+    c.eu += o;
+    c.ev += o;
+    c.iw += o;
+    c.w = c.wv[b];
+    return c;
+}
 
 __device__ __forceinline__ zc unit_u(const Geo &g, const Coef &c, int col, int j, int k) {
     return (gu(g, j, k) % g.wc == col && !c.pec_u[j * (g.nz + 1) + k]) ? mk(1.0, 0.0) : mk(0.0, 0.0);
@@ -163,8 +181,16 @@ __device__ __forceinline__ zc unit_v(const Geo &g, const Coef &c, int col, int j
     return (gv(g, j, k) % g.wc == col && !c.pec_v[j * g.nz + k]) ? mk(1.0, 0.0) : mk(0.0, 0.0);
 }
 
+// dst[t] = src[t].r: the real words of (x, 0) pairs
+__global__ void k_real_parts(const zc *src, double *dst, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t].r;
+}
+
 // hw = curl_x of the coloured input (cells)
-__global__ void k_probe_hw(Geo g, Coef c, int col0, int nb, zc *hw) {
+__global__ void k_probe_hw(Geo g, Coef c0, int col0, int nb, zc *hw) {
+    const Coef c = coef_of(g, c0, blockIdx.y);
+    hw += blockIdx.y * g.shw;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nb * g.nc) return;
     const int b = (int)(t / g.nc), q = (int)(t % g.nc);
@@ -175,7 +201,11 @@ __global__ void k_probe_hw(Geo g, Coef c, int col0, int nb, zc *hw) {
 }
 
 // h = Q e: hu at E_v positions, hv at E_u positions (mode2d.c:181-199)
-__global__ void k_probe_h(Geo g, Coef c, int col0, int nb, const zc *hw, zc *hu, zc *hv) {
+__global__ void k_probe_h(Geo g, Coef c0, int col0, int nb, const zc *hw, zc *hu, zc *hv) {
+    const Coef c = coef_of(g, c0, blockIdx.y);
+    hw += blockIdx.y * g.shw;
+    hu += blockIdx.y * g.shu;
+    hv += blockIdx.y * g.shv;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int per = g.nv + g.nu;
     if (t >= (long long)nb * per) return;
@@ -197,7 +227,11 @@ __global__ void k_probe_h(Geo g, Coef c, int col0, int nb, const zc *hw, zc *hu,
 }
 
 // ew at nodes (mode2d.c:203-209)
-__global__ void k_probe_ew(Geo g, Coef c, int nb, const zc *hu, const zc *hv, zc *ew) {
+__global__ void k_probe_ew(Geo g, Coef c0, int nb, const zc *hu, const zc *hv, zc *ew) {
+    const Coef c = coef_of(g, c0, blockIdx.y);
+    hu += blockIdx.y * g.shu;
+    hv += blockIdx.y * g.shv;
+    ew += blockIdx.y * g.sew;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nb * g.nn) return;
     const int b = (int)(t / g.nn), q = (int)(t % g.nn);
@@ -212,7 +246,12 @@ __global__ void k_probe_ew(Geo g, Coef c, int nb, const zc *hu, const zc *hv, zc
 
 // e = P h at every unknown (mode2d.c:210-217), PEC rows zero, scattered into band
 // column g of the colour's unit within bw of the row
-__global__ void k_probe_out(Geo g, Coef c, int col0, int nb, const zc *hu, const zc *hv, const zc *ew, zc *band) {
+__global__ void k_probe_out(Geo g, Coef c0, int col0, int nb, const zc *hu, const zc *hv, const zc *ew, zc *band) {
+    const Coef c = coef_of(g, c0, blockIdx.y);
+    hu += blockIdx.y * g.shu;
+    hv += blockIdx.y * g.shv;
+    ew += blockIdx.y * g.sew;
+    band += blockIdx.y * g.sband;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nb * g.n) return;
     const int b = (int)(t / g.n), r = (int)(t % g.n), col = col0 + b;
@@ -246,7 +285,10 @@ __global__ void k_probe_out(Geo g, Coef c, int col0, int nb, const zc *hu, const
     band[bidx(g, r, gcol)] = y;
 }
 
-__global__ void k_diag(Geo g, Coef c, zc sigma, double big, zc *band) {
+__global__ void k_diag(Geo g, Coef c, const zc *sigmas, const double *bigs, zc *band) {
+    const zc sigma = sigmas[blockIdx.y];
+    const double big = bigs[blockIdx.y];
+    band += blockIdx.y * g.sband;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= g.n) return;
     const int per = 2 * g.nz + 1;
@@ -278,6 +320,9 @@ __global__ void k_diag(Geo g, Coef c, zc sigma, double big, zc *band) {
 // rewritten only after every thread has passed the next column's first barrier.
 template <int NB>
 __global__ void __launch_bounds__(512) k_lu_panel(Geo g, zc *band, int *ipiv, int *info, int j0, int j1) {
+    band += blockIdx.y * g.sband;
+    ipiv += blockIdx.y * g.svec;
+    info += blockIdx.y;
     __shared__ double wbest[2][8];
     __shared__ int wrow[2][8];
     __shared__ zc prow[2][NB], jrow[2][NB];
@@ -373,6 +418,8 @@ __global__ void __launch_bounds__(512) k_lu_panel(Geo g, zc *band, int *ipiv, in
 // panel's multipliers are shared through LDS.
 __global__ void __launch_bounds__(kLuThreads) k_lu_trail(Geo g, zc *band, const int *ipiv, int j0, int j1) {
     extern __shared__ zc L[];
+    band += blockIdx.y * g.sband;
+    ipiv += blockIdx.y * g.svec;
     const int kl = g.bw, kv = 2 * g.bw, nbp = j1 - j0;
     const int R = min(j1 - 1 + kl, g.n - 1) - j0 + 1;
     copy_batched<16>(L, R * nbp, threadIdx.x, kLuThreads, [=](int t) {
@@ -427,10 +474,10 @@ struct InvArgs {
     const zc *band;
     const int *ipiv;
     zc *x, *xo, *y;
-    zc sigma;
+    const zc *sigma;        // shift of each problem of the batch
     int maxit;
     double tol;
-    double *out;   // lam (2), rc, iterations
+    double *out;   // per problem 10 words: lam (2), rc, iterations, phase timers
     int chunk_f, chunk_b;   // band columns per ring slot (forward, back)
     int ring;               // ring size in entries
     int timing;             // accumulate cycle counts of the solve phases into out[4..9]
@@ -670,11 +717,15 @@ __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
     if (threadIdx.x < 8) tmr[threadIdx.x] = 0;
     if (threadIdx.x == 0) *zslot = mk(0.0, 0.0);
     __syncthreads();
-    const int n = a.g.n;
-    zc *__restrict__ X = a.x;
-    zc *__restrict__ XO = a.xo;
-    zc *__restrict__ Y = a.y;
-    zc lam = a.sigma, lam_old = mk(0.0, 0.0);
+    const int n = a.g.n, bq = blockIdx.y;
+    zc *__restrict__ X = a.x + bq * a.g.svec;
+    zc *__restrict__ XO = a.xo + bq * a.g.svec;
+    zc *__restrict__ Y = a.y + bq * a.g.svec;
+    const zc *band = a.band + bq * a.g.sband;
+    const int *ipiv = a.ipiv + bq * a.g.svec;
+    double *out = a.out + bq * 10;
+    const zc sigma = a.sigma[bq];
+    zc lam = sigma, lam_old = mk(0.0, 0.0);
     int rc = 1, it = 0;
     for (it = 0; it < a.maxit; it++) {
         double s = 0.0;
@@ -710,7 +761,7 @@ __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
             Y[q] = X[q];
         }
         __syncthreads();
-        band_solve<SF, SB>(a.g, a.band, a.ipiv, a.y, a.chunk_f, a.chunk_b, ring, rpiv, zslot, a.timing ? tmr : nullptr);
+        band_solve<SF, SB>(a.g, band, ipiv, Y, a.chunk_f, a.chunk_b, ring, rpiv, zslot, a.timing ? tmr : nullptr);
         __syncthreads();
         double mr = 0.0, mi = 0.0;
         #pragma unroll 8
@@ -720,7 +771,7 @@ __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
             mi += t.i;
         }
         const zc mu = mk(block_sum(mr, red), block_sum(mi, red));
-        lam = zadd(a.sigma, zdiv(mk(1.0, 0.0), mu));
+        lam = zadd(sigma, zdiv(mk(1.0, 0.0), mu));
         #pragma unroll 8
         for (int q = threadIdx.x; q < n; q += kInvThreads) X[q] = Y[q];
         __syncthreads();
@@ -731,11 +782,11 @@ __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
         lam_old = lam;
     }
     if (threadIdx.x == 0) {
-        a.out[0] = lam.r;
-        a.out[1] = lam.i;
-        a.out[2] = rc;
-        a.out[3] = it;
-        for (int k = 0; k < 6; k++) a.out[4 + k] = (double)tmr[k];
+        out[0] = lam.r;
+        out[1] = lam.i;
+        out[2] = rc;
+        out[3] = it;
+        for (int k = 0; k < 6; k++) out[4 + k] = (double)tmr[k];
     }
 }
 
@@ -787,8 +838,8 @@ struct HMem {
 };
 
 template <int SF, int SB>
-int run_inverse(size_t lds, hipStream_t st, const InvArgs &ia) {
-    hipLaunchKernelGGL((k_inverse<SF, SB>), dim3(1), dim3(kInvThreads), lds, st, ia);
+int run_inverse(size_t lds, hipStream_t st, const InvArgs &ia, int nb) {
+    hipLaunchKernelGGL((k_inverse<SF, SB>), dim3(1, nb), dim3(kInvThreads), lds, st, ia);
     return 0;
 }
 
@@ -829,14 +880,16 @@ struct ModeGpu {
     DMem<unsigned char> pec;
     DMem<int> ipiv, info;
     DMem<double> out;
+    DMem<zc> prm;                 // per problem: shift, then (big, 0) and (w, 0), then doubles
     HMem<zc> hcoef, hx;           // coefficients; start vector in, eigenvector out
+    HMem<zc> hprm;
     HMem<unsigned char> hpec;
     HMem<double> hout;            // info, then the kernel's ten result words
     // the band LU's launches (two per panel), captured for one geometry and set of
     // buffers and replayed by every later solve of that geometry
     hipGraphExec_t lu = nullptr;
     const void *lu_key[3] = {};
-    int lu_n = -1, lu_bw = -1;
+    int lu_n = -1, lu_bw = -1, lu_b = -1;
     std::chrono::steady_clock::time_point t_last{};   // end of the previous solve (debug timing)
 };
 
@@ -858,6 +911,8 @@ void mode_gpu_release(ModeGpu *m) {
     m->ipiv.release();
     m->info.release();
     m->out.release();
+    m->prm.release();
+    m->hprm.release();
     m->hcoef.release();
     m->hx.release();
     m->hpec.release();
@@ -874,7 +929,9 @@ void mode_gpu_release(ModeGpu *m) {
         }                                                                               \
     } while (0)
 
-int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> *xout, std::complex<double> *lam) {
+int mode_gpu_solve_batch(ModeGpu **ctx, const ModeGpuProblem *P, int nbq, std::complex<double> *const *xout,
+                         std::complex<double> *lam, int *rcs, int *its) {
+    if (nbq < 1) return 0;
     if (!*ctx) {
         *ctx = new ModeGpu();
         // a stream with a CU mask (here every CU) gets a hardware queue of its own; plain
@@ -896,6 +953,12 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
         }
     }
     ModeGpu &M = **ctx;
+    const ModeGpuProblem &p = P[0];   // geometry, masks and iteration limits of the batch
+    for (int b = 1; b < nbq; b++)
+        if (P[b].ny != p.ny || P[b].nz != p.nz) {
+            set_error("GPU mode solver: a batch mixes port geometries");
+            return -1;
+        }
     const auto t_enter = std::chrono::steady_clock::now();
     Geo g;
     g.ny = p.ny;
@@ -931,17 +994,26 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     }
     const int ncoef = p.ny + (p.ny + 1) + p.nz + (p.nz + 1) + g.nu + g.nv + g.nn;
     const int nbatch = std::min(kProbeBatch, g.wc);
+    g.sband = (long long)g.ldab * n;
+    g.svec = n;
+    g.scoef = ncoef;
+    g.shw = (long long)nbatch * g.nc;
+    g.shu = (long long)nbatch * g.nv;
+    g.shv = (long long)nbatch * g.nu;
+    g.sew = (long long)nbatch * g.nn;
+    const size_t B = (size_t)nbq;
     hipStream_t st = M.st;
-    if (!M.band.get((size_t)g.ldab * n, st) || !M.coef.get(ncoef, st) || !M.x.get(n, st) || !M.xo.get(n, st) ||
-        !M.y.get(n, st) || !M.hw.get((size_t)nbatch * g.nc, st) || !M.hu.get((size_t)nbatch * g.nv, st) ||
-        !M.hv.get((size_t)nbatch * g.nu, st) || !M.ew.get((size_t)nbatch * g.nn, st) || !M.pec.get(g.nu + g.nv, st) ||
-        !M.ipiv.get(n, st) || !M.info.get(1, st) || !M.out.get(10, st)) {
+    if (!M.band.get(B * g.sband, st) || !M.coef.get(B * ncoef, st) || !M.x.get(B * n, st) ||
+        !M.xo.get(B * n, st) || !M.y.get(B * n, st) || !M.hw.get(B * g.shw, st) || !M.hu.get(B * g.shu, st) ||
+        !M.hv.get(B * g.shv, st) || !M.ew.get(B * g.sew, st) || !M.pec.get(g.nu + g.nv, st) ||
+        !M.ipiv.get(B * n, st) || !M.info.get(B, st) || !M.out.get(10 * B, st) || !M.prm.get(4 * B, st)) {
         set_error("GPU mode solver: device out of memory");
         return -1;
     }
     // the pinned buffers are rewritten below: no copy of an earlier (failed) call in flight
     MG_CHECK(hipStreamSynchronize(st));
-    if (!M.hcoef.get(ncoef) || !M.hx.get(n) || !M.hpec.get(g.nu + g.nv) || !M.hout.get(11)) {
+    if (!M.hcoef.get(B * ncoef) || !M.hx.get(B * n) || !M.hpec.get(g.nu + g.nv) || !M.hout.get(11 * B) ||
+        !M.hprm.get(4 * B)) {
         set_error("GPU mode solver: cannot allocate pinned host memory");
         return -1;
     }
@@ -950,28 +1022,43 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     if (debug)
         for (auto &e : ev) MG_CHECK(hipEventCreate(&e));
     if (debug) MG_CHECK(hipEventRecord(ev[0], st));
-    // coefficients (host-built, FP64 complex) and PEC masks
-    size_t o = 0;
-    auto put = [&](const std::complex<double> *src, int len) {
-        for (int i = 0; i < len; i++) M.hcoef.p[o + i] = zc{src[i].real(), src[i].imag()};
-        o += len;
-    };
-    put(p.iyc, p.ny);
-    put(p.iyn, p.ny + 1);
-    put(p.izc, p.nz);
-    put(p.izn, p.nz + 1);
-    put(p.eu, g.nu);
-    put(p.ev, g.nv);
-    put(p.iw, g.nn);
-    MG_CHECK(hipMemcpyAsync(M.coef.p, M.hcoef.p, ncoef * sizeof(zc), hipMemcpyHostToDevice, st));
+    // per problem: coefficients (host-built, FP64 complex), start vector, then the
+    // problem parameters (shift, PEC diagonal, angular frequency)
+    for (size_t b = 0; b < B; b++) {
+        const ModeGpuProblem &q = P[b];
+        size_t o = b * ncoef;
+        auto put = [&](const std::complex<double> *src, int len) {
+            for (int i = 0; i < len; i++) M.hcoef.p[o + i] = zc{src[i].real(), src[i].imag()};
+            o += len;
+        };
+        put(q.iyc, q.ny);
+        put(q.iyn, q.ny + 1);
+        put(q.izc, q.nz);
+        put(q.izn, q.nz + 1);
+        put(q.eu, g.nu);
+        put(q.ev, g.nv);
+        put(q.iw, g.nn);
+        for (int i = 0; i < n; i++) M.hx.p[b * n + i] = zc{q.x0[i].real(), q.x0[i].imag()};
+        M.hprm.p[b] = zc{q.sigma.real(), q.sigma.imag()};
+        M.hprm.p[B + b] = zc{q.big, 0.0};
+        M.hprm.p[2 * B + b] = zc{q.w, 0.0};
+    }
+    // the device reads the shifts as zc, the diagonal and frequency as doubles:
+    // (big, 0) and (w, 0) pairs are read through their first word
+    MG_CHECK(hipMemcpyAsync(M.coef.p, M.hcoef.p, B * ncoef * sizeof(zc), hipMemcpyHostToDevice, st));
     std::memcpy(M.hpec.p, p.pec_u, g.nu);
     std::memcpy(M.hpec.p + g.nu, p.pec_v, g.nv);
     MG_CHECK(hipMemcpyAsync(M.pec.p, M.hpec.p, g.nu + g.nv, hipMemcpyHostToDevice, st));
-    for (int i = 0; i < n; i++) M.hx.p[i] = zc{p.x0[i].real(), p.x0[i].imag()};
-    MG_CHECK(hipMemcpyAsync(M.x.p, M.hx.p, n * sizeof(zc), hipMemcpyHostToDevice, st));
-    MG_CHECK(hipMemsetAsync(M.xo.p, 0, n * sizeof(zc), st));
-    MG_CHECK(hipMemsetAsync(M.band.p, 0, (size_t)g.ldab * n * sizeof(zc), st));
-    MG_CHECK(hipMemsetAsync(M.info.p, 0, sizeof(int), st));
+    MG_CHECK(hipMemcpyAsync(M.x.p, M.hx.p, B * n * sizeof(zc), hipMemcpyHostToDevice, st));
+    MG_CHECK(hipMemcpyAsync(M.prm.p, M.hprm.p, 3 * B * sizeof(zc), hipMemcpyHostToDevice, st));
+    // per problem (big, 0) and (w, 0) pairs, compacted to doubles on the device
+    hipLaunchKernelGGL(k_real_parts, dim3((unsigned)((2 * B + 255) / 256)), dim3(256), 0, st,
+                       (const zc *)(M.prm.p + B), (double *)(M.prm.p + 3 * B), (int)(2 * B));
+    const double *bigs = (const double *)(M.prm.p + 3 * B);
+    const double *wv = bigs + B;
+    MG_CHECK(hipMemsetAsync(M.xo.p, 0, B * n * sizeof(zc), st));
+    MG_CHECK(hipMemsetAsync(M.band.p, 0, B * g.sband * sizeof(zc), st));
+    MG_CHECK(hipMemsetAsync(M.info.p, 0, B * sizeof(int), st));
     Coef c;
     c.iyc = M.coef.p;
     c.iyn = c.iyc + p.ny;
@@ -983,26 +1070,28 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     c.pec_u = M.pec.p;
     c.pec_v = M.pec.p + g.nu;
     c.w = p.w;
+    c.wv = wv;
     // assembly by coloured probing
     const int T = 256;
+    const unsigned by = (unsigned)B;
     for (int col0 = 0; col0 < g.wc && col0 < n; col0 += nbatch) {
         const int nbt = std::min(nbatch, std::min(g.wc, n) - col0);
-        hipLaunchKernelGGL(k_probe_hw, dim3((unsigned)(((long long)nbt * g.nc + T - 1) / T)), dim3(T), 0, st, g, c,
-                           col0, nbt, M.hw.p);
-        hipLaunchKernelGGL(k_probe_h, dim3((unsigned)(((long long)nbt * (g.nu + g.nv) + T - 1) / T)), dim3(T), 0, st,
-                           g, c, col0, nbt, M.hw.p, M.hu.p, M.hv.p);
-        hipLaunchKernelGGL(k_probe_ew, dim3((unsigned)(((long long)nbt * g.nn + T - 1) / T)), dim3(T), 0, st, g, c,
+        hipLaunchKernelGGL(k_probe_hw, dim3((unsigned)(((long long)nbt * g.nc + T - 1) / T), by), dim3(T), 0, st, g,
+                           c, col0, nbt, M.hw.p);
+        hipLaunchKernelGGL(k_probe_h, dim3((unsigned)(((long long)nbt * (g.nu + g.nv) + T - 1) / T), by), dim3(T), 0,
+                           st, g, c, col0, nbt, M.hw.p, M.hu.p, M.hv.p);
+        hipLaunchKernelGGL(k_probe_ew, dim3((unsigned)(((long long)nbt * g.nn + T - 1) / T), by), dim3(T), 0, st, g, c,
                            nbt, M.hu.p, M.hv.p, M.ew.p);
-        hipLaunchKernelGGL(k_probe_out, dim3((unsigned)(((long long)nbt * n + T - 1) / T)), dim3(T), 0, st, g, c,
+        hipLaunchKernelGGL(k_probe_out, dim3((unsigned)(((long long)nbt * n + T - 1) / T), by), dim3(T), 0, st, g, c,
                            col0, nbt, M.hu.p, M.hv.p, M.ew.p, M.band.p);
     }
-    hipLaunchKernelGGL(k_diag, dim3((n + T - 1) / T), dim3(T), 0, st, g, c, zc{p.sigma.real(), p.sigma.imag()}, p.big,
-                       M.band.p);
+    hipLaunchKernelGGL(k_diag, dim3((n + T - 1) / T, by), dim3(T), 0, st, g, c, (const zc *)M.prm.p, bigs, M.band.p);
     if (debug) MG_CHECK(hipEventRecord(ev[1], st));
     // band LU, panel by panel: two launches per panel (1897 panels on a 30 k-unknown
-    // port).  Replayed from a graph captured on the first solve of a geometry, so that
-    // solves running at once on other host threads (the port sweeps) do not queue
-    // behind thousands of launch submissions; COURANT_GRAPH=0 launches directly.
+    // port), every problem of the batch in each.  Replayed from a graph captured on the
+    // first solve of a geometry and batch size, so that solves running at once on other
+    // host threads (the port sweeps) do not queue behind thousands of launch
+    // submissions; COURANT_GRAPH=0 launches directly.
     auto launch_lu = [&]() {
         for (int j0 = 0; j0 < n; j0 += nb) {
             const int j1 = std::min(n, j0 + nb);
@@ -1010,14 +1099,14 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
             const size_t lds = (size_t)R * (j1 - j0) * sizeof(zc);
             const dim3 pb((unsigned)((R + 63) / 64 * 64));   // one thread per panel row (R <= 512)
             if (nb > 8)
-                hipLaunchKernelGGL(k_lu_panel<16>, dim3(1), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+                hipLaunchKernelGGL(k_lu_panel<16>, dim3(1, by), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
             else
-                hipLaunchKernelGGL(k_lu_panel<8>, dim3(1), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+                hipLaunchKernelGGL(k_lu_panel<8>, dim3(1, by), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
             const int ncol = std::min(j1 - 1 + kv, n - 1) - j1 + 1;
             if (ncol > 0) {
                 const int wpb = kLuThreads / 64;
-                hipLaunchKernelGGL(k_lu_trail, dim3((ncol + wpb - 1) / wpb), dim3(kLuThreads), lds, st, g, M.band.p,
-                                   M.ipiv.p, j0, j1);
+                hipLaunchKernelGGL(k_lu_trail, dim3((ncol + wpb - 1) / wpb, by), dim3(kLuThreads), lds, st, g,
+                                   M.band.p, M.ipiv.p, j0, j1);
             }
         }
     };
@@ -1026,7 +1115,7 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
         launch_lu();
     } else {
         const void *key[3] = {M.band.p, M.ipiv.p, M.info.p};
-        if (!M.lu || M.lu_n != n || M.lu_bw != kl || std::memcmp(key, M.lu_key, sizeof key) != 0) {
+        if (!M.lu || M.lu_n != n || M.lu_bw != kl || M.lu_b != nbq || std::memcmp(key, M.lu_key, sizeof key) != 0) {
             // one capture at a time in the process (once per context; the sweeps' clones
             // reach their first solve together)
             static std::mutex capture_mu;
@@ -1047,6 +1136,7 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
             std::memcpy(M.lu_key, key, sizeof key);
             M.lu_n = n;
             M.lu_bw = kl;
+            M.lu_b = nbq;
         }
         MG_CHECK(hipGraphLaunch(M.lu, st));
     }
@@ -1058,7 +1148,7 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     ia.x = M.x.p;
     ia.xo = M.xo.p;
     ia.y = M.y.p;
-    ia.sigma = zc{p.sigma.real(), p.sigma.imag()};
+    ia.sigma = (const zc *)M.prm.p;
     ia.maxit = p.maxit;
     ia.tol = p.tol;
     ia.out = M.out.p;
@@ -1069,7 +1159,7 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     // the walk is compiled for the exact register-window sizes; sb is 2 sf - 2 or 2 sf - 1
     int lr = 1;
 #define MG_INV(SFV, SBV) \
-    if (sf == SFV && sb == SBV) lr = run_inverse<SFV, SBV>(inv_lds, st, ia);
+    if (sf == SFV && sb == SBV) lr = run_inverse<SFV, SBV>(inv_lds, st, ia, nbq);
     MG_INV(2, 2) MG_INV(2, 3) MG_INV(3, 4) MG_INV(3, 5) MG_INV(4, 6) MG_INV(4, 7) MG_INV(5, 8) MG_INV(5, 9)
     MG_INV(6, 10) MG_INV(6, 11) MG_INV(7, 12) MG_INV(7, 13) MG_INV(8, 14) MG_INV(8, 15)
 #undef MG_INV
@@ -1079,15 +1169,14 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
     }
     MG_CHECK(hipGetLastError());
     if (debug) MG_CHECK(hipEventRecord(ev[3], st));
-    int *const hinfo = reinterpret_cast<int *>(M.hout.p);
-    const double *const out = M.hout.p + 1;
-    MG_CHECK(hipMemcpyAsync(hinfo, M.info.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    MG_CHECK(hipMemcpyAsync(M.hout.p + 1, M.out.p, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
-    MG_CHECK(hipMemcpyAsync(M.hx.p, M.x.p, n * sizeof(zc), hipMemcpyDeviceToHost, st));
+    int *const hinfo = reinterpret_cast<int *>(M.hout.p);   // B ints in the first B words
+    double *const hres = M.hout.p + B;                      // 10 words per problem
+    MG_CHECK(hipMemcpyAsync(hinfo, M.info.p, B * sizeof(int), hipMemcpyDeviceToHost, st));
+    MG_CHECK(hipMemcpyAsync(hres, M.out.p, 10 * B * sizeof(double), hipMemcpyDeviceToHost, st));
+    MG_CHECK(hipMemcpyAsync(M.hx.p, M.x.p, B * n * sizeof(zc), hipMemcpyDeviceToHost, st));
     if (debug) MG_CHECK(hipEventRecord(ev[4], st));
     const auto t_sync = std::chrono::steady_clock::now();
     MG_CHECK(hipStreamSynchronize(st));
-    const int info = *hinfo;
     if (debug) {
         float t[4];
         for (int k = 0; k < 4; k++) MG_CHECK(hipEventElapsedTime(&t[k], ev[k], ev[k + 1]));
@@ -1097,20 +1186,35 @@ int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> 
         const double between =
             M.t_last.time_since_epoch().count() ? std::chrono::duration<double, std::milli>(t_enter - M.t_last).count() : 0.0;
         M.t_last = t_now;
-        std::fprintf(stderr, "mode_gpu: n %d bw %d nb %d chunks %d/%d | assembly %.2f ms, LU %.2f ms, %d iterations %.2f ms"
-                     " | result copies %.2f ms | call %.1f ms, final wait %.1f ms, host time since the previous call %.1f ms\n",
-                     n, kl, nb, chunk_f, chunk_b, t[0], t[1], (int)out[3] + 1, t[2], t[3], wall, t_wait, between);
+        int itmax = 0;
+        for (size_t b = 0; b < B; b++) itmax = std::max(itmax, (int)hres[10 * b + 3] + 1);
+        std::fprintf(stderr, "mode_gpu: n %d bw %d nb %d chunks %d/%d batch %d | assembly %.2f ms, LU %.2f ms, "
+                     "<= %d iterations %.2f ms | result copies %.2f ms | call %.1f ms, final wait %.1f ms, host time "
+                     "since the previous call %.1f ms\n",
+                     n, kl, nb, chunk_f, chunk_b, nbq, t[0], t[1], itmax, t[2], t[3], wall, t_wait, between);
         std::fprintf(stderr, "mode_gpu: cycles fwd walk %.3g wait %.3g | back walk %.3g wait %.3g | fill fwd %.3g back %.3g\n",
-                     out[4], out[5], out[6], out[7], out[8], out[9]);
+                     hres[4], hres[5], hres[6], hres[7], hres[8], hres[9]);
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
-    if (info) {   // singular pivot: a failed LU, as on the host path
-        set_error("mode solver: singular pivot in the band LU");
-        return -1;
+    for (size_t b = 0; b < B; b++)
+        if (hinfo[b]) {   // singular pivot: a failed LU, as on the host path
+            set_error("mode solver: singular pivot in the band LU");
+            return -1;
+        }
+    for (size_t b = 0; b < B; b++) {
+        for (int i = 0; i < n; i++) xout[b][i] = std::complex<double>(M.hx.p[b * n + i].r, M.hx.p[b * n + i].i);
+        lam[b] = std::complex<double>(hres[10 * b], hres[10 * b + 1]);
+        rcs[b] = hres[10 * b + 2] == 0.0 ? 0 : 1;
+        if (its) its[b] = (int)hres[10 * b + 3] + 1;
     }
-    for (int i = 0; i < n; i++) xout[i] = std::complex<double>(M.hx.p[i].r, M.hx.p[i].i);
-    *lam = std::complex<double>(out[0], out[1]);
-    return out[2] == 0.0 ? 0 : 1;
+    return 0;
+}
+
+int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> *xout, std::complex<double> *lam) {
+    int rc = 1;
+    std::complex<double> *const xo[1] = {xout};
+    if (mode_gpu_solve_batch(ctx, &p, 1, xo, lam, &rc, nullptr)) return -1;
+    return rc;
 }
 
 }  // namespace cem

@@ -44,6 +44,15 @@ bool mode_gpu_enabled();
 // eigenvalue estimate exactly as the host iteration defines them.
 int mode_gpu_solve(ModeGpu **ctx, const ModeGpuProblem &p, std::complex<double> *x, std::complex<double> *lam);
 
+// A batch of nb problems of one geometry (same ny, nz, PEC masks and iteration
+// limits; own coefficients, shift and start vector each) in one launch sequence:
+// problem b runs in the workgroups of grid row b of every kernel.  x[b], lam[b] and
+// rcs[b] (0 converged, 1 iteration limit) receive each result, its[b] (if not
+// null) the iterations run.  Returns 0, or -1 with the error set (a singular band
+// in any problem included).
+int mode_gpu_solve_batch(ModeGpu **ctx, const ModeGpuProblem *p, int nb, std::complex<double> *const *x,
+                         std::complex<double> *lam, int *rcs, int *its);
+
 void mode_gpu_release(ModeGpu *ctx);
 
 }  // namespace cem

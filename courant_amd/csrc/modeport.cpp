@@ -9,6 +9,7 @@
 #include "sim.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -178,6 +179,33 @@ void cem::modeports_free(cem_sim *s) {
     }
 }
 
+// Port quantities at output frequency q from the mode solved on xs
+// (modeport.c:170-195): the discrete propagation constant of the 3D grid from the
+// 2D one, the conductor attenuation by perturbation when SIBC is on, and Z0 = V / I.
+static void record_freq(cem_sim *s, ModePort &p, cem_mode2d *xs, int q) {
+    const cem_mode_port_def &d = p.def;
+    const double dx = s->d[0][d.i_mon];
+    cplx kc(cem_mode2d_beta(xs), -cem_mode2d_alpha(xs));
+    cplx b3 = 2 / dx * std::asin(kc * dx / 2.0);
+    p.kappa[2 * q] = kc.real();
+    p.kappa[2 * q + 1] = kc.imag();
+    p.beta[q] = b3.real();
+    p.alpha[q] = -b3.imag();
+    if (s->sibc_on) {
+        double zr, zi;
+        cem_conductor_zs(s, s->freq[q], &zr, &zi);
+        p.alpha_c[q] = cem_mode2d_conductor_alpha(xs, zr);
+        p.alpha[q] += p.alpha_c[q];
+    }
+    double V[2], Iv[2];
+    cem_mode2d_vi(xs, d.vj - d.j0, d.vk0 - d.k0, d.vk1 - d.k0, d.ja - d.j0, d.jb - d.j0, d.ka - d.k0, d.kb - d.k0, V,
+                  Iv);
+    // cem_mode2d_vi returns -sum Ez dz; the port voltage is vsign * sum Ez dz
+    cplx Z = (double)(-d.vsign) * cplx(V[0], V[1]) / cplx(Iv[0], Iv[1]);
+    p.z0[2 * q] = Z.real();
+    p.z0[2 * q + 1] = Z.imag();
+}
+
 // Per output frequency: solve the port mode, walking outward from the frequency
 // nearest f0 and seeding each solve with the previous mode (modeport.c:146-201).
 // One sweep of the output frequencies outward from q0 (step +1 or -1), each solve
@@ -188,7 +216,6 @@ static int sweep_port(cem_sim *s, ModePort &p, cem_mode2d *xs, int q0, int step,
     const int ny = d.j1 - d.j0, nz = d.k1 - d.k0;
     const int nst = ny * (nz + 1) + (ny + 1) * nz;
     std::vector<double> start(2 * nst);
-    const double dx = s->d[0][d.i_mon];
     cem_mode2d_solve(xs, warp(s, s->freq[q0]), d.neff0, nullptr, 300, 1e-13);
     for (int q = q0; q >= 0 && q < nf; q += step) {
         double fw = warp(s, s->freq[q]);
@@ -196,6 +223,9 @@ static int sweep_port(cem_sim *s, ModePort &p, cem_mode2d *xs, int q0, int step,
         std::memcpy(start.data(), cem_mode2d_eu(xs), 2 * ny * (nz + 1) * sizeof(double));
         std::memcpy(start.data() + 2 * ny * (nz + 1), cem_mode2d_ev(xs), 2 * (ny + 1) * nz * sizeof(double));
         const int rc = cem_mode2d_solve(xs, fw, ng, start.data(), 300, 1e-13);
+        if (std::getenv("COURANT_MODE_DEBUG"))
+            std::fprintf(stderr, "mode sweep: port i_mon %d seq q %d rc %d neff %.9f -> %.9f\n", d.i_mon, q, rc, ng,
+                         cem_mode2d_beta(xs) / (2 * M_PI * mode_freq_hint(xs) / CEM_C0));
         if (rc > 0) {
             set_error("mode port: eigenmode did not converge at an output frequency");
             return -1;
@@ -206,25 +236,7 @@ static int sweep_port(cem_sim *s, ModePort &p, cem_mode2d *xs, int q0, int step,
             return -1;
         }
         if (q == q0 && !write_q0) continue;
-        cplx kc(cem_mode2d_beta(xs), -cem_mode2d_alpha(xs));
-        cplx b3 = 2 / dx * std::asin(kc * dx / 2.0);
-        p.kappa[2 * q] = kc.real();
-        p.kappa[2 * q + 1] = kc.imag();
-        p.beta[q] = b3.real();
-        p.alpha[q] = -b3.imag();
-        if (s->sibc_on) {
-            double zr, zi;
-            cem_conductor_zs(s, s->freq[q], &zr, &zi);
-            p.alpha_c[q] = cem_mode2d_conductor_alpha(xs, zr);
-            p.alpha[q] += p.alpha_c[q];
-        }
-        double V[2], Iv[2];
-        cem_mode2d_vi(xs, d.vj - d.j0, d.vk0 - d.k0, d.vk1 - d.k0, d.ja - d.j0, d.jb - d.j0, d.ka - d.k0,
-                      d.kb - d.k0, V, Iv);
-        // cem_mode2d_vi returns -sum Ez dz; the port voltage is vsign * sum Ez dz
-        cplx Z = (double)(-d.vsign) * cplx(V[0], V[1]) / cplx(Iv[0], Iv[1]);
-        p.z0[2 * q] = Z.real();
-        p.z0[2 * q + 1] = Z.imag();
+        record_freq(s, p, xs, q);
     }
     return 0;
 }
@@ -256,12 +268,259 @@ static int solve_port_modes(cem_sim *s, ModePort &p) {
     return 0;
 }
 
-// With the GPU solver, the sweeps of every unsolved port run at once, one host
-// thread and one device context each (a clone of the port's cross-section for the
-// downward sweep).  Each sweep is the sequence the sequential solve runs, and the
-// device solve is deterministic, so the results are the same; the downward sweep
-// leaves q0 to the upward one, which computes the same value.
+// Frequencies per direction in one batch of the GPU sweep, bounded so that the batch's
+// band storage stays within COURANT_MODE_BATCH_GB (default 24 GiB) per port.
+static int sweep_batch(const ModePort &p) {
+    int k = 32;
+    if (const char *v = std::getenv("COURANT_MODE_BATCH")) k = std::max(1, std::atoi(v));
+    double gb = 24;
+    if (const char *v = std::getenv("COURANT_MODE_BATCH_GB")) gb = std::max(0.1, std::atof(v));
+    const int ny = p.def.j1 - p.def.j0, nz = p.def.k1 - p.def.k0;
+    const double n = (double)ny * (nz + 1) + (double)(ny + 1) * nz, bw = 2.0 * (nz + 2) + 2;
+    const double band = n * (3 * bw + 1) * 16.0;
+    const int fit = (int)std::max(1.0, std::floor(gb * (1ull << 30) / band / 2));
+    return std::min(k, fit);
+}
+
+// The port's sweep on the GPU in batches.  q0 (the output frequency nearest f0) is
+// solved as the sequential sweep solves it (a cold solve, then one seeded by it).  Then
+// each batch takes the next frequencies of each direction (upwards, downwards; at most
+// K, within kSpan of the seed's frequency) and solves them at once, each seeded with
+// the mode of the last frequency solved in its direction: its start vector, and as the
+// shift the square of k0 times its complex effective index, extrapolated linearly in
+// frequency from the last two solved (the sequential sweep seeds each frequency with
+// its neighbour's vector and real index).  Members that have not converged after
+// kPass1 iterations run again from their current vector with their current eigenvalue
+// estimate as the shift (a restarted shift: the distance to the wanted eigenvalue, and
+// so the convergence ratio of inverse iteration, drops by orders of magnitude).  A
+// member whose mode does not continue its seed (overlap below kOverlap, or an index
+// more than kIndexJump away from the guess: another eigenpair) is solved again from its
+// solved neighbour.  Inverse iteration converges to the same eigenpair from any of
+// these seeds, to its tolerance (|d lambda| < 1e-13 |lambda|, phase-aligned change
+// < 1e-10).  K per direction grows while the first passes converge and shrinks when
+// they do not (up to the memory bound of sweep_batch).
+static int sweep_port_batched(cem_sim *s, ModePort &p) {
+    constexpr int kPass1 = 24;
+    constexpr double kOverlap = 0.98, kIndexJump = 0.02, kSpan = 0.5;
+    const int nf = s->nf;
+    const cem_mode_port_def &d = p.def;
+    const int ny = d.j1 - d.j0, nz = d.k1 - d.k0;
+    const size_t nu2 = 2 * (size_t)ny * (nz + 1), nv2 = 2 * (size_t)(ny + 1) * nz;
+    const int q0 = nearest_freq(s, d.f0);
+    const bool debug = std::getenv("COURANT_MODE_DEBUG") != nullptr;
+    cem_mode2d *xs = p.xs;
+    auto fail = [](int rc) {
+        if (rc > 0) {
+            set_error("mode port: eigenmode did not converge at an output frequency");
+        } else {
+            const std::string e = std::string("mode port: ") + cem_last_error();
+            set_error(e.c_str());
+        }
+        return -1;
+    };
+    auto k0 = [](double fw) { return 2 * M_PI * fw / CEM_C0; };
+    // complex effective index of a solved mode: (beta - j alpha) / k0
+    auto index_of = [&](const cem_mode2d *m) {
+        return cplx(cem_mode2d_beta(m), -cem_mode2d_alpha(m)) / k0(mode_freq_hint(m));
+    };
+    auto vec_of = [&](const cem_mode2d *m, std::vector<double> &v) {
+        v.resize(nu2 + nv2);
+        std::memcpy(v.data(), cem_mode2d_eu(m), nu2 * sizeof(double));
+        std::memcpy(v.data() + nu2, cem_mode2d_ev(m), nv2 * sizeof(double));
+    };
+    struct Dir {
+        int step, next, K;
+        std::vector<double> st;   // start vector: the last solved mode of this direction
+        cplx n = 0, n_prev = 0;   // complex effective indices of the last two solved
+        double fa = 0, f_prev = 0;
+        bool has_prev = false;
+        cplx guess(double fw) const {
+            return has_prev && fa != f_prev ? n + (n - n_prev) / (fa - f_prev) * (fw - fa) : n;
+        }
+    };
+    auto take = [&](Dir &dr, const cem_mode2d *m, double fw) {
+        if (dr.fa != 0) {
+            dr.n_prev = dr.n;
+            dr.f_prev = dr.fa;
+            dr.has_prev = true;
+        }
+        dr.n = index_of(m);
+        dr.fa = fw;
+        vec_of(m, dr.st);
+    };
+    // |<u, v>| / (|u| |v|) over the transverse E of two modes stored as (eu, ev)
+    auto overlap = [&](const double *u, const cem_mode2d *m) {
+        const double *eu = cem_mode2d_eu(m), *ev = cem_mode2d_ev(m);
+        double pr = 0, pi = 0, nu = 0, nv = 0;
+        for (size_t t = 0; t < nu2 + nv2; t += 2) {
+            const double *v = t < nu2 ? eu + t : ev + (t - nu2);
+            pr += u[t] * v[0] + u[t + 1] * v[1];
+            pi += u[t] * v[1] - u[t + 1] * v[0];
+            nu += u[t] * u[t] + u[t + 1] * u[t + 1];
+            nv += v[0] * v[0] + v[1] * v[1];
+        }
+        return nu > 0 && nv > 0 ? std::hypot(pr, pi) / std::sqrt(nu * nv) : 0.0;
+    };
+    const double f_q0 = warp(s, s->freq[q0]);
+    cem_mode2d_solve(xs, f_q0, d.neff0, nullptr, 300, 1e-13);
+    {
+        std::vector<double> st;
+        vec_of(xs, st);
+        const double ng = cem_mode2d_beta(xs) / k0(mode_freq_hint(xs));
+        if (const int rc = cem_mode2d_solve(xs, f_q0, ng, st.data(), 300, 1e-13)) return fail(rc);
+    }
+    record_freq(s, p, xs, q0);
+    const int kmax = sweep_batch(p);
+    Dir dirs[2] = {{1, q0 + 1, std::min(8, kmax)}, {-1, q0 - 1, std::min(8, kmax)}};
+    for (auto &dr : dirs) take(dr, xs, f_q0);
+    std::vector<cem_mode2d *> pool;   // result objects of a batch (pool[0] holds the batch's device context)
+    struct Guard {
+        std::vector<cem_mode2d *> &v;
+        ~Guard() {
+            for (auto *x : v) cem_mode2d_destroy(x);
+        }
+    } guard{pool};
+    auto left = [&](const Dir &dr) { return dr.step > 0 ? nf - dr.next : dr.next + 1; };
+    auto count = [&](const Dir &dr) {
+        int n = 0;
+        while (n < dr.K && n < left(dr)) {
+            const double fw = warp(s, s->freq[dr.next + n * dr.step]);
+            if (n > 0 && std::fabs(fw - dr.fa) > kSpan * dr.fa) break;
+            n++;
+        }
+        return n;
+    };
+    // solve `idx` (members of pool) at frequencies f with shifts sh from starts st
+    auto solve = [&](const std::vector<int> &idx, const std::vector<double> &f, const std::vector<cplx> &sh,
+                     const std::vector<const double *> &st, int maxit, std::vector<int> &rc, std::vector<int> &it) {
+        const int m = (int)idx.size();
+        std::vector<cem_mode2d *> ms(m);
+        for (int b = 0; b < m; b++) ms[b] = pool[idx[b]];
+        rc.assign(m, 1);
+        it.assign(m, 0);
+        return mode2d_solve_batch(ms.data(), m, f.data(), sh.data(), st.data(), maxit, 1e-13, rc.data(), it.data());
+    };
+    while (left(dirs[0]) > 0 || left(dirs[1]) > 0) {
+        std::vector<int> qs, di;
+        const int take_n[2] = {count(dirs[0]), count(dirs[1])};
+        for (int k = 0; k < 2; k++)
+            for (int u = 0; u < take_n[k]; u++) {
+                qs.push_back(dirs[k].next + u * dirs[k].step);
+                di.push_back(k);
+            }
+        const int B = (int)qs.size();
+        while ((int)pool.size() < B) pool.push_back(mode2d_clone(xs));
+        std::vector<double> f(B);
+        std::vector<cplx> sh(B), guess(B);
+        std::vector<const double *> start(B);
+        const std::vector<double> seed[2] = {dirs[0].st, dirs[1].st};   // the anchors move while results are taken
+        std::vector<int> all(B);
+        for (int b = 0; b < B; b++) {
+            all[b] = b;
+            f[b] = warp(s, s->freq[qs[b]]);
+            guess[b] = dirs[di[b]].guess(f[b]);
+            sh[b] = k0(f[b]) * k0(f[b]) * guess[b] * guess[b];
+            start[b] = seed[di[b]].data();
+        }
+        std::vector<int> rc1, it1;
+        if (solve(all, f, sh, start, kPass1, rc1, it1)) return fail(-1);
+        // second pass: restarted shifts for the members still converging
+        std::vector<int> rest;
+        for (int b = 0; b < B; b++)
+            if (rc1[b]) rest.push_back(b);
+        std::vector<int> rc(B, 0);
+        int npass2[2] = {0, 0};
+        if (!rest.empty()) {
+            std::vector<double> f2;
+            std::vector<cplx> sh2;
+            std::vector<std::vector<double>> v2(rest.size());
+            std::vector<const double *> st2;
+            for (size_t r = 0; r < rest.size(); r++) {
+                const int b = rest[r];
+                const cplx bt(cem_mode2d_beta(pool[b]), -cem_mode2d_alpha(pool[b]));
+                f2.push_back(f[b]);
+                sh2.push_back(bt * bt);
+                vec_of(pool[b], v2[r]);
+                st2.push_back(v2[r].data());
+                npass2[di[b]]++;
+            }
+            std::vector<int> rc2, it2;
+            if (solve(rest, f2, sh2, st2, 300, rc2, it2)) return fail(-1);
+            for (size_t r = 0; r < rest.size(); r++) rc[rest[r]] = rc2[r];
+        }
+        int nre[2] = {0, 0};
+        for (int b = 0; b < B; b++) {
+            Dir &dr = dirs[di[b]];
+            const cplx nb = index_of(pool[b]);
+            const bool other = std::abs(nb - guess[b]) > kIndexJump * std::abs(guess[b]) ||
+                               overlap(start[b], pool[b]) < kOverlap;
+            if (rc[b] || other) {   // solved again from its solved neighbour
+                nre[di[b]]++;
+                const std::vector<int> one = {b};
+                const cplx g = dr.guess(f[b]);
+                std::vector<int> r1, i1;
+                if (solve(one, {f[b]}, {k0(f[b]) * k0(f[b]) * g * g}, {dr.st.data()}, 300, r1, i1)) return fail(-1);
+                if (debug)
+                    std::fprintf(stderr, "mode sweep: port i_mon %d re-solve q %d (rc %d, other %d) -> rc %d, %d its\n",
+                                 d.i_mon, qs[b], rc[b], (int)other, r1[0], i1[0]);
+                if (r1[0]) return fail(r1[0]);
+            }
+            record_freq(s, p, pool[b], qs[b]);
+            take(dr, pool[b], f[b]);   // batch members of a direction come outward in order
+            if (qs[b] == 0) mode2d_copy_state(xs, pool[b]);   // the cross-section ends where the sequential solve ends
+        }
+        for (int k = 0; k < 2; k++) {
+            Dir &dr = dirs[k];
+            const int n = take_n[k];
+            if (n <= 0) continue;
+            int worst = 0;
+            for (int b = 0; b < B; b++)
+                if (di[b] == k) worst = std::max(worst, it1[b]);
+            if (debug)
+                std::fprintf(stderr, "mode sweep: port i_mon %d dir %+d: %d frequencies from %d, first pass <= %d "
+                             "iterations, %d restarted, %d re-solved\n", d.i_mon, dr.step, n, dr.next, worst,
+                             npass2[k], nre[k]);
+            dr.next += n * dr.step;
+            if (nre[k] || 4 * npass2[k] > n) dr.K = std::max(1, dr.K / 2);
+            else if (npass2[k] == 0 && worst <= 16) dr.K = std::min(kmax, 2 * dr.K);
+        }
+    }
+    return 0;
+}
+
+// With the GPU solver the ports sweep at once, one host thread and one device context
+// each, in frequency batches (sweep_port_batched); COURANT_MODE_SWEEP=seq runs the
+// sequential sweeps instead, each port's upward and downward sweep on a thread of its
+// own (a clone of the cross-section for the downward one).
 static int solve_all_ports(cem_sim *s) {
+    const char *mode = std::getenv("COURANT_MODE_SWEEP");
+    if (!(mode && std::string(mode) == "seq")) {
+        std::vector<ModePort *> ports;
+        for (auto &p : s->mport)
+            if (!p.solved && p.xs) {
+                reset_port(s, p);
+                ports.push_back(&p);
+            }
+        std::vector<int> rc(ports.size(), 0);
+        std::vector<std::string> err(ports.size());
+        std::vector<std::thread> th;
+        for (size_t k = 0; k < ports.size(); k++)
+            th.emplace_back([s, &ports, &rc, &err, k]() {
+                rc[k] = sweep_port_batched(s, *ports[k]);
+                if (rc[k]) err[k] = cem_last_error();
+            });
+        for (auto &t : th) t.join();
+        int r = 0;
+        for (size_t k = 0; k < ports.size(); k++) {
+            if (rc[k]) {
+                set_error(err[k].c_str());
+                r = -1;
+            } else {
+                ports[k]->solved = true;
+            }
+        }
+        return r;
+    }
     struct Job {
         ModePort *p;
         cem_mode2d *xs;

@@ -172,6 +172,9 @@ cem_mode2d *mode2d_clone(const cem_mode2d *m);
 void mode2d_copy_state(cem_mode2d *dst, const cem_mode2d *src);
 // true when cem_mode2d_solve runs on the GPU
 bool mode2d_on_gpu();
+// solves of one cross-section geometry at several frequencies in one GPU batch
+int mode2d_solve_batch(cem_mode2d *const *ms, int nb, const double *f, const std::complex<double> *shift,
+                       const double *const *start, int maxit, double tol, int *rcs, int *its);
 void nnls(int m, int n, const double *A, const double *b, double *x);
 
 // device engine (engine.hip)

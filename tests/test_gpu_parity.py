@@ -154,6 +154,38 @@ def test_lossy_mode_port_waves(cem, ref, gpu_available):
     assert np.abs(s11a - s11b).max() <= 1e-4
 
 
+def test_batched_port_sweep_matches_sequential_sweep(cem, gpu_available, monkeypatch):
+    """The port-mode sweep in frequency batches (three frequencies per direction per batch,
+    each seeded with the last mode solved in its direction) against the sequential sweep
+    (each frequency seeded with its neighbour's mode, modeport.c:146-201), on the same
+    simulation: Z0, beta, alpha within 1e-9 relative, the waves within 1e-9 of max |a|."""
+    print_case("port-mode sweep, frequency batches vs sequential")
+    mini = dict(MODE_MINI, df=2e9)
+    res = {}
+    for mode in ("seq", "batch"):
+        if mode == "seq":
+            monkeypatch.setenv("COURANT_MODE_SWEEP", "seq")
+        else:
+            monkeypatch.delenv("COURANT_MODE_SWEEP", raising=False)
+            monkeypatch.setenv("COURANT_MODE_BATCH", "3")
+        b, ib = S.gcpw(cem, **mini)
+        b.step(400)
+        res[mode] = [(b.mode_port_waves(p), b.mode_port_alpha(p)) for p in ib["ports"]]
+        b.close()
+    monkeypatch.delenv("COURANT_MODE_BATCH", raising=False)
+    for (ws, als), (wb, alb) in zip(res["seq"], res["batch"]):
+        assert len(ws["z0"]) >= 19
+        for k, x, y in (("z0", ws["z0"], wb["z0"]), ("beta", ws["beta"], wb["beta"]), ("alpha", als, alb)):
+            err = np.max(np.abs(x - y) / np.abs(x))
+            print("  %s: max rel diff %.2e" % (k, err))
+            assert err <= 1e-9
+        scale = np.max(np.abs(ws["a"]))
+        for k in ("a", "b"):
+            err = np.max(np.abs(ws[k] - wb[k])) / scale
+            print("  %s: max diff %.2e of max |a|" % (k, err))
+            assert err <= 1e-9
+
+
 def test_device_wave_separation_matches_host_formula(cem, gpu_available, monkeypatch):
     """The FP64 wave separation on the GPU against the same formula on the host, on the
     same simulation (same spectra, same solved port modes): 1e-12 of max |a|."""
