@@ -76,14 +76,18 @@ def kernel_bytes(pc):
              numbers instead of 24 B of ca, cb) per point + psi_E (8 B/entry) + the Debye
              currents it updates, read/write (8 B per pole per dispersive edge; beta / dt
              comes from a per-material table)
+      step : the one-pass step (H and E of a step in one sweep): E and H read once and
+             written once (48 B/pt) + classes (4) + psi_H and psi_E read/write (8 B/entry)
+             + the Debye currents (8 B per pole per dispersive edge)
       post : SIBC states read/write (8 B per term) + index, 4 H faces and weights (36 B)
              per surface edge (and the few dispersive edges that are also surface or
              source edges, not counted)
     """
     h = 36 * pc["N"] + 8 * pc["psiH"]
     e = 40 * pc["N"] + 8 * pc["psiE"] + 8 * pc["npole"] * pc["ndisp"]
+    step = 52 * pc["N"] + 8 * (pc["psiH"] + pc["psiE"]) + 8 * pc["npole"] * pc["ndisp"]
     post = (8 * pc["nterm"] + 36) * pc["nsibc"]
-    return {"H": h, "E": e, "post": post}
+    return {"H": h, "E": e, "step": step, "post": post}
 
 
 def cpu_baseline(workload, target_s=12.0):
@@ -148,6 +152,8 @@ def pmc_traffic(workload, shape):
     the plain copy kernel moving 5e8 B in and 5e8 B out per dispatch.  The counted child runs
     use the launch shape the timed run was tuned to."""
     env = dict(os.environ, COURANT_ISEG_H=str(shape[0]), COURANT_ISEG_E=str(shape[1]))
+    if shape[3]:
+        env["COURANT_ISEG_S"] = str(shape[3])
     tmp = tempfile.mkdtemp(prefix="courant_pmc_", dir="/tmp")
     try:
         out = {}
@@ -160,7 +166,7 @@ def pmc_traffic(workload, shape):
             scale = 0.5e9 / 1024 / cal["cem::k_copy"]
             out[counter] = {"scale": scale, "kib": run}
         kernels = {}
-        for k in ("cem::k_update_h", "cem::k_update_e", "cem::k_post", "cem::k_gather"):
+        for k in ("cem::k_step", "cem::k_update_h", "cem::k_update_e", "cem::k_post", "cem::k_gather"):
             f = out["FETCH_SIZE"]["kib"].get(k)
             w = out["WRITE_SIZE"]["kib"].get(k)
             if f is not None and w is not None:
@@ -337,19 +343,21 @@ def main():
     sim.flush_dft()
     sim.synchronize()
     wall_direct = time.perf_counter() - t1
-    stats = {k: sim.kernel_stats(i) for i, k in enumerate(("H", "E", "gather", "dft", "post"))}
+    stats = {k: sim.kernel_stats(i) for i, k in enumerate(("H", "E", "gather", "dft", "post", "step"))}
     sim.set_profiling(False)
     pc = plan_counts(sim)
     kb = kernel_bytes(pc)
     b8d = step_bytes(pc)
     shape = sim.launch_shape()
+    rows, planes = sim.step_shape()
+    shape = (shape[0], shape[1], shape[2], planes)
     copy_gbs = cem.lib.cem_amd_copy_bandwidth(2 << 30, 10)
     value = cells * args.steps / wall / 1e6
     avg = {k: v[0] / max(v[1], 1) for k, v in stats.items()}
-    dom = max(("H", "E", "post"), key=lambda k: stats[k][0])
+    dom = max(("step", "H", "E", "post"), key=lambda k: stats[k][0])
     achieved = kb[dom] / (avg[dom] * 1e-3) / 1e9
     t_step = wall / args.steps
-    moved = kb["H"] + kb["E"] + kb["post"]
+    moved = (kb["step"] if rows else kb["H"] + kb["E"]) + kb["post"]
     sim.step(1)  # refresh spectra through the public path: probes must be finite
     spec = sim.vprobe(0) if cem.lib.cem_plan_nvprobe(sim.h) > 0 else sim.port_v(0)
     ok = bool(np.all(np.isfinite(spec)))
@@ -370,7 +378,8 @@ def main():
                    "grid_points": pc["N"], "nf": int(len(info["freqs"])), "ndisp": pc["ndisp"],
                    "npole": pc["npole"], "nsibc": pc["nsibc"], "dft": "f64", "setup_s": round(setup_s, 2),
                    "finite": ok},
-        "roofline": {"bound": "hbm", "kernel": {"H": "k_update_h", "E": "k_update_e", "post": "k_post"}[dom],
+        "roofline": {"bound": "hbm",
+                     "kernel": {"step": "k_step", "H": "k_update_h", "E": "k_update_e", "post": "k_post"}[dom],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": kb[dom], "avg_launch_ms": round(avg[dom], 5),
@@ -380,11 +389,13 @@ def main():
                               "frac_moved_model": round(moved / t_step / 1e9 / HBM_PEAK_GBS, 4)},
                      "measured_copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs > 0 else None},
-        "kernels_ms": {k: round(v, 5) for k, v in avg.items()},
-        "kernel_bytes_mb": {k: round(v / 1e6, 1) for k, v in kb.items()},
+        "kernels_ms": {k: round(v, 5) for k, v in avg.items() if stats[k][1]},
+        "kernel_bytes_mb": {k: round(v / 1e6, 1) for k, v in kb.items() if stats[k][1]},
         "launch": {"timed": "graph replay, DFT of the timed steps included",
                    "ms_per_step_direct_launches": round(wall_direct / nprof * 1e3, 5),
-                   "profiled_steps": nprof, "iplanes_per_thread": {"H": shape[0], "E": shape[1]},
+                   "profiled_steps": nprof,
+                   "one_pass_step": {"rows_per_workgroup": rows, "xplanes_per_workgroup": planes} if rows else None,
+                   "iplanes_per_thread_split": None if rows else {"H": shape[0], "E": shape[1]},
                    "autotuned": shape[2]},
     }
     sim.close()

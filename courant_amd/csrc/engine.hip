@@ -50,8 +50,10 @@ namespace cem {
 constexpr int kBlock = 256;        // 4 wavefronts of 64
 constexpr int kIseg = 2;           // default i-planes per thread in the stencil kernels
 constexpr int kSeriesCap = 512;    // steps recorded between DFT flushes
-constexpr int kGraphSteps = 64;
-constexpr int kGatherBlock = 1024;    // steps per captured chunk (divides kSeriesCap)
+constexpr int kGraphSteps = 64;    // steps per captured chunk (even; divides kSeriesCap)
+constexpr int kGatherBlock = 1024;
+constexpr int kStepRows = 8;       // most owned rows per tile of the one-pass step
+constexpr int kStepLanes = 1024;   // most lanes per workgroup of the one-pass step
 
 #define HIP_OK(expr)                                                                  \
     do {                                                                              \
@@ -308,6 +310,59 @@ __device__ __forceinline__ void zerow(float (&x)[W]) {
     for (int e = 0; e < W; e++) x[e] = 0.f;
 }
 
+// One H point (sim.c:579-606) with the CPML recursions of the slabs that hold it
+// (sim.c:680-754), from the point's six E differences: each component only where
+// the reference updates it, with the reference's expression in its order.  Shared
+// by the H sweep and the one-pass step kernel, so both give the same bits.
+__device__ __forceinline__ void h_point(float &hxv, float &hyv, float &hzv, float dz_ey, float dy_ez, float dz_ex,
+                                        float dy_ex, float dx_ez, float dx_ey, float hx, float hy, float hz, bool jin,
+                                        bool kin, bool iin, int sy, float by, float cy, int sx, float bx, float cx,
+                                        bool inz, float bz, float cz, float dth, float &py0, float &py2, float &px1,
+                                        float &px2, float &pz0, float &pz1) {
+    if (jin && kin) {   // Hx: all i
+        float h = hxv - (dy_ez * hy - dz_ey * hz);
+        if (sy >= 0) {
+            const float v = by * py0 + cy * dy_ez;
+            py0 = v;
+            h -= dth * v;
+        }
+        if (inz) {
+            const float v = bz * pz0 + cz * dz_ey;
+            pz0 = v;
+            h += dth * v;
+        }
+        hxv = h;
+    }
+    if (iin && kin) {   // Hy: j <= ny
+        float h = hyv - (dz_ex * hz - dx_ez * hx);
+        if (sx >= 0) {
+            const float v = bx * px1 + cx * dx_ez;
+            px1 = v;
+            h += dth * v;
+        }
+        if (inz) {
+            const float v = bz * pz1 + cz * dz_ex;
+            pz1 = v;
+            h -= dth * v;
+        }
+        hyv = h;
+    }
+    if (iin && jin) {   // Hz: k <= nz
+        float h = hzv - (dx_ey * hx - dy_ex * hy);
+        if (sx >= 0) {
+            const float v = bx * px2 + cx * dx_ey;
+            px2 = v;
+            h -= dth * v;
+        }
+        if (sy >= 0) {
+            const float v = by * py2 + cy * dy_ex;
+            py2 = v;
+            h += dth * v;
+        }
+        hzv = h;
+    }
+}
+
 // H update (sim.c:579-606) fused with the CPML psi_H recursions (sim.c:680-754).
 // One thread per group of an i-segment; the k + 1 neighbour of the group's last
 // point comes from one 32-bit load, the E of plane i+1 is carried in registers
@@ -399,51 +454,9 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
 #pragma unroll
         for (int e = 0; e < W; e++) {
             const float ey_k = e + 1 < W ? ey[(e + 1) % W] : eyk, ex_k = e + 1 < W ? ex[(e + 1) % W] : exk;
-            const float dz_ey = ey_k - ey[e], dy_ez = ezj[e] - ez[e];
-            const float dz_ex = ex_k - ex[e], dy_ex = exj[e] - ex[e];
-            const float dx_ez = ezi[e] - ez[e], dx_ey = eyi[e] - ey[e];
-            if (jin && kin[e]) {   // Hx: all i
-                float h = hxv[e] - (dy_ez * hy - dz_ey * hz[e]);
-                if (sy >= 0) {
-                    const float v = by * py0[e] + cy * dy_ez;
-                    py0[e] = v;
-                    h -= a.dth * v;
-                }
-                if (inz[e]) {
-                    const float v = bz[e] * pz0[e] + cz[e] * dz_ey;
-                    pz0[e] = v;
-                    h += a.dth * v;
-                }
-                hxv[e] = h;
-            }
-            if (iin && kin[e]) {   // Hy: j <= ny
-                float h = hyv[e] - (dz_ex * hz[e] - dx_ez * hx);
-                if (sx >= 0) {
-                    const float v = bx * px1[e] + cx * dx_ez;
-                    px1[e] = v;
-                    h += a.dth * v;
-                }
-                if (inz[e]) {
-                    const float v = bz[e] * pz1[e] + cz[e] * dz_ex;
-                    pz1[e] = v;
-                    h -= a.dth * v;
-                }
-                hyv[e] = h;
-            }
-            if (iin && jin) {      // Hz: k <= nz
-                float h = hzv[e] - (dx_ey * hx - dy_ex * hy);
-                if (sx >= 0) {
-                    const float v = bx * px2[e] + cx * dx_ey;
-                    px2[e] = v;
-                    h -= a.dth * v;
-                }
-                if (sy >= 0) {
-                    const float v = by * py2[e] + cy * dy_ex;
-                    py2[e] = v;
-                    h += a.dth * v;
-                }
-                hzv[e] = h;
-            }
+            h_point(hxv[e], hyv[e], hzv[e], ey_k - ey[e], ezj[e] - ez[e], ex_k - ex[e], exj[e] - ex[e],
+                    ezi[e] - ez[e], eyi[e] - ey[e], hx, hy, hz[e], jin, kin[e], iin, sy, by, cy, sx, bx, cx, inz[e],
+                    bz[e], cz[e], a.dth, py0[e], py2[e], px1[e], px2[e], pz0[e], pz1[e]);
         }
         if (jin) stw<W>(Hx + q, hxv);
         if (iin) stw<W>(Hy + q, hyv);
@@ -467,6 +480,75 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
             ez[e] = ezi[e];
         }
     }
+}
+
+// One E point (sim.c:608-637) with the CPML recursions of the slabs that hold it
+// (sim.c:756-832), from the point's six H differences; each component only where
+// the reference updates it.  Shared by the E sweep and the one-pass step kernel.
+__device__ __forceinline__ void e_point(float &exn, float &eyn, float &ezn, float ex0, float ey0, float ez0, float cax,
+                                        float cbx, float cay, float cby, float caz, float cbz, float dy_hz, float dz_hy,
+                                        float dz_hx, float dx_hz, float dx_hy, float dy_hx, float ex_s, float ey_s,
+                                        float ez_s, bool jb, bool jlt, bool ib_, bool kb, bool kz, int sy, float by,
+                                        float cy, int sx, float bx, float cx, bool inz, float bz, float cz, float &py0,
+                                        float &py2, float &px1, float &px2, float &pz0, float &pz1) {
+    if (jb && kb) {   // Ex
+        float v0 = cax * ex0 + cbx * (dy_hz * ey_s - dz_hy * ez_s);
+        if (sy >= 0) {
+            const float v = by * py0 + cy * dy_hz;
+            py0 = v;
+            v0 += cbx * v;
+        }
+        if (inz) {
+            const float v = bz * pz0 + cz * dz_hy;
+            pz0 = v;
+            v0 -= cbx * v;
+        }
+        exn = v0;
+    }
+    if (jlt && ib_ && kb) {   // Ey
+        float v0 = cay * ey0 + cby * (dz_hx * ez_s - dx_hz * ex_s);
+        if (sx >= 0) {
+            const float v = bx * px1 + cx * dx_hz;
+            px1 = v;
+            v0 -= cby * v;
+        }
+        if (inz) {
+            const float v = bz * pz1 + cz * dz_hx;
+            pz1 = v;
+            v0 += cby * v;
+        }
+        eyn = v0;
+    }
+    if (kz && ib_ && jb) {   // Ez
+        float v0 = caz * ez0 + cbz * (dx_hy * ex_s - dy_hx * ey_s);
+        if (sx >= 0) {
+            const float v = bx * px2 + cx * dx_hy;
+            px2 = v;
+            v0 += cbz * v;
+        }
+        if (sy >= 0) {
+            const float v = by * py2 + cy * dy_hx;
+            py2 = v;
+            v0 -= cbz * v;
+        }
+        ezn = v0;
+    }
+}
+
+// Debye ADE of one dispersive edge of a class with material row bt (beta_p / dt):
+// the correction E -= cb sum (1+k_p)/2 J_p (dispersive.c:93-100) after the bulk
+// update, then J_p += (k_p - 1) J_p + beta_p/dt (E_new - E_old) (dispersive.c:102-112).
+template <int NP>
+__device__ __forceinline__ void debye_point(float &en, float e0, float cb, float *jp, const float *bt,
+                                            const float *kc, const float *km1) {
+    float acc = 0.f;
+#pragma unroll
+    for (int pp = 0; pp < NP; pp++) acc += kc[pp] * jp[pp];
+    const float ev = en - cb * acc;
+    en = ev;
+    const float de = ev - e0;
+#pragma unroll
+    for (int pp = 0; pp < NP; pp++) jp[pp] = jp[pp] + km1[pp] * jp[pp] + bt[pp] * de;
 }
 
 // E update (sim.c:608-637) fused with CPML psi_E (sim.c:756-832) and, for NP > 0,
@@ -638,54 +720,10 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
 #pragma unroll
         for (int e = 0; e < W; e++) {
             const float hx_k = e ? hx[(e + W - 1) % W] : hxk, hy_k = e ? hy[(e + W - 1) % W] : hyk;
-            if (jb && kb[e]) {     // Ex
-                const float cbv = cb[0][e];
-                const float dy_hz = hz[e] - hzj[e], dz_hy = hy[e] - hy_k;
-                float v0 = ca[0][e] * ex0[e] + cbv * (dy_hz * ey_s - dz_hy * ez_s[e]);
-                if (sy >= 0) {
-                    const float v = by * py0[e] + cy * dy_hz;
-                    py0[e] = v;
-                    v0 += cbv * v;
-                }
-                if (inz[e]) {
-                    const float v = bz[e] * pz0[e] + cz[e] * dz_hy;
-                    pz0[e] = v;
-                    v0 -= cbv * v;
-                }
-                exn[e] = v0;
-            }
-            if (j < g.ny && ib_ && kb[e]) {   // Ey
-                const float cbv = cb[1][e];
-                const float dz_hx = hx[e] - hx_k, dx_hz = hz[e] - hzm[e];
-                float v0 = ca[1][e] * ey0[e] + cbv * (dz_hx * ez_s[e] - dx_hz * ex_s);
-                if (sx >= 0) {
-                    const float v = bx * px1[e] + cx * dx_hz;
-                    px1[e] = v;
-                    v0 -= cbv * v;
-                }
-                if (inz[e]) {
-                    const float v = bz[e] * pz1[e] + cz[e] * dz_hx;
-                    pz1[e] = v;
-                    v0 += cbv * v;
-                }
-                eyn[e] = v0;
-            }
-            if (kz[e] && ib_ && jb) {   // Ez
-                const float cbv = cb[2][e];
-                const float dx_hy = hy[e] - hym[e], dy_hx = hx[e] - hxj[e];
-                float v0 = ca[2][e] * ez0[e] + cbv * (dx_hy * ex_s - dy_hx * ey_s);
-                if (sx >= 0) {
-                    const float v = bx * px2[e] + cx * dx_hy;
-                    px2[e] = v;
-                    v0 += cbv * v;
-                }
-                if (sy >= 0) {
-                    const float v = by * py2[e] + cy * dy_hx;
-                    py2[e] = v;
-                    v0 -= cbv * v;
-                }
-                ezn[e] = v0;
-            }
+            e_point(exn[e], eyn[e], ezn[e], ex0[e], ey0[e], ez0[e], ca[0][e], cb[0][e], ca[1][e], cb[1][e], ca[2][e],
+                    cb[2][e], hz[e] - hzj[e], hy[e] - hy_k, hx[e] - hx_k, hz[e] - hzm[e], hy[e] - hym[e],
+                    hx[e] - hxj[e], ex_s, ey_s, ez_s[e], jb, j < g.ny, ib_, kb[e], kz[e], sy, by, cy, sx, bx, cx,
+                    inz[e], bz[e], cz[e], py0[e], py2[e], px1[e], px2[e], pz0[e], pz1[e]);
         }
         if (in_box) {
 #pragma unroll
@@ -695,17 +733,7 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
 #pragma unroll
                 for (int e = 0; e < W; e++) {
                     const int row = s_row[c * 256 + ((cls[e] >> (8 * c)) & 255u)];
-                    if (row < 0) continue;
-                    float *jp = jv[c] + e * NPE;
-                    float acc = 0.f;
-#pragma unroll
-                    for (int pp = 0; pp < NP; pp++) acc += a.kc[pp] * jp[pp];
-                    const float ev = en[e] - cb[c][e] * acc;
-                    en[e] = ev;
-                    const float de = ev - e0[e];
-                    const float *bt = s_bt + row * NP;
-#pragma unroll
-                    for (int pp = 0; pp < NP; pp++) jp[pp] = jp[pp] + a.km1[pp] * jp[pp] + bt[pp] * de;
+                    if (row >= 0) debye_point<NP>(en[e], e0[e], cb[c][e], jv[c] + e * NPE, s_bt + row * NP, a.kc, a.km1);
                 }
 #pragma unroll
                 for (int u = 0; u < NJ; u += CH) {
@@ -736,6 +764,311 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
             hym[e] = hy[e];
             hzm[e] = hz[e];
         }
+    }
+}
+
+// One-pass step: the H update of an x plane and then the E update of the same plane
+// in one sweep over x, so each field component is read once and written once per
+// step (the two sweeps above read E and H twice between them).  The old fields and
+// psi_H are read from one buffer set and the new ones written to the other; psi_E
+// and the Debye currents, which only their own point reads, are updated in place.
+// A workgroup owns a tile of tj rows (j0 .. j0+tj-1) by kw points of k (k0 ..
+// k0+kw-1) and walks iseg planes of x.  Its lanes hold one point each, on tj + 2
+// rows of kw + 2 columns:
+//   row 0 / column 0          (j0 - 1, k0 - 1) H only, for the j - 1 and k - 1
+//                             neighbours of the E update
+//   rows 1 .. tj, columns 1 .. kw   owned: H, E, their stores
+//   row tj + 1 / column kw + 1      publishes its E, the j + 1 / k + 1 neighbour
+//                             of the H update
+// The E of the plane and the new H pass between lanes through LDS; the H of plane
+// i - 1 stays in registers, and a workgroup starting inside x computes that plane's
+// Hy, Hz at its points first.  Every point is updated by h_point / e_point /
+// debye_point, the functions of the two sweeps, so the two paths give the same bits.
+// Measured on the 38.8 M-point four-layer board it moves 3.4 GB a step (the sweeps
+// 4.3 GB) but takes 1.36 ms against their 0.91 ms: at 113 VGPRs one 1024-lane
+// workgroup fills a CU, its two barriers a plane leave the loads of each plane
+// unhidden, and it issues 2.4x the sweeps' VALU work (SGPR spills among it).
+struct SArgs {
+    HArgs h;                // old E, H (h.E, h.H) and old psi_H (h.psi)
+    EArgs e;                // coefficients, psi_E, Debye box (e.E / e.H unused)
+    float *En, *Hn;         // new E (3N), H (3N)
+    float *psiHn[3][2];     // new psi_H
+    int tj, ntj;            // owned rows per tile, tiles over j
+    int kw, ntk, kl;        // owned k points per tile, tiles over k, lanes per row (kw + 2)
+};
+
+template <int NP, bool kCmp>
+__global__ void __launch_bounds__(1024) k_step(SArgs f, int iseg) {
+    extern __shared__ float lds[];
+    const Grid g = f.h.g;
+    const int tj = f.tj, kp = g.kp, kl = f.kl;
+    float *sE = lds;                                    // [3][tj + 2][kl]
+    float *sH = lds + 3 * (tj + 2) * kl;                // [3][tj + 1][kl]
+    float2 *s_tab = (float2 *)(sH + 3 * (tj + 1) * kl); // kCmp: 3 x 256 (ca, cb)
+    int *s_row = (int *)(s_tab + (kCmp ? 768 : 0));     // kCmp, NP > 0: 3 x 256 Debye rows
+    float *s_bt = (float *)(s_row + ((kCmp && NP > 0) ? 768 : 0));   // nrow x NP
+    const EArgs &a = f.e;
+    if (kCmp) {
+        for (int t = threadIdx.x; t < 768; t += blockDim.x) {
+            s_tab[t] = a.ctab[t];
+            if (NP > 0) s_row[t] = a.crow[t];
+        }
+        if (NP > 0)
+            for (int t = threadIdx.x; t < a.nrow * NP; t += blockDim.x) s_bt[t] = a.btab[t];
+    }
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int jt = tile % f.ntj, kt = (tile / f.ntj) % f.ntk, seg = tile / (f.ntj * f.ntk);
+    const int r = threadIdx.x / kl, c = threadIdx.x - r * kl;
+    const int j = jt * tj - 1 + r, kr = kt * f.kw - 1 + c;
+    const bool lane = r < tj + 2 && j >= 0 && j <= g.ny && kr >= 0 && kr < kp;
+    const bool hrow = lane && r <= tj && c <= f.kw;
+    const bool own = lane && r >= 1 && r <= tj && c >= 1 && c <= f.kw;
+    const int k = lane ? kr : 0;
+    const int ib = seg * iseg, ie = min(g.nx + 1, ib + iseg);
+    const float *__restrict__ Exo = f.h.E;
+    const float *__restrict__ Eyo = f.h.E + g.N;
+    const float *__restrict__ Ezo = f.h.E + 2 * g.N;
+    const float *__restrict__ Hxo = f.h.H;
+    const float *__restrict__ Hyo = f.h.H + g.N;
+    const float *__restrict__ Hzo = f.h.H + 2 * g.N;
+    // H-side constants of the lane (k_update_h at one point per lane)
+    const Pml &ph = f.h.pml;
+    const bool jin = j < g.ny, kin = k < g.nz;
+    const float hy_s = lane && jin ? f.h.hs[1][j] : 0.f;
+    const float hz_s = kin ? f.h.hs[2][k] : 0.f;
+    const int syh = f.h.has_pml && lane && jin ? ph.slot[1][j] : -1;
+    const float byh = syh >= 0 ? ph.b[1][j] : 0.f, cyh = syh >= 0 ? ph.c[1][j] : 0.f;
+    const int zoh = f.h.has_pml ? ph.zo[k / 4] : -1;
+    const bool inzh = zoh >= 0 && kin && ph.slot[2][k] >= 0;
+    const float bzh = inzh ? ph.b[2][k] : 0.f, czh = inzh ? ph.c[2][k] : 0.f;
+    const int cn = k + 1 < kp ? c + 1 : c;   // LDS column of the k + 1 neighbour
+    // E-side constants (k_update_e at one point per lane; owned rows only)
+    const Pml &pe = a.pml;
+    const bool jb = j >= 1 && j < g.ny, kb = k >= 1 && k < g.nz, kz = k < g.nz;
+    const float ey_s = own ? a.es[1][j] : 0.f;
+    const float ez_s = k <= g.nz ? a.es[2][k] : 0.f;
+    const int sye = a.has_pml && own ? pe.slot[1][j] : -1;
+    const float bye = sye >= 0 ? pe.b[1][j] : 0.f, cye = sye >= 0 ? pe.c[1][j] : 0.f;
+    const int zoe = a.has_pml ? pe.zo[k / 4] : -1;
+    const bool inze = zoe >= 0 && kz && pe.slot[2][k] >= 0;
+    const float bze = inze ? pe.b[2][k] : 0.f, cze = inze ? pe.c[2][k] : 0.f;
+    const bool box_jk = NP > 0 && own && j >= a.bj0 && j < a.bj1 && k >= 4 * a.bg0 && k < 4 * a.bg1;
+    const int cm = k ? c - 1 : c;            // of the k - 1 neighbour
+    // 32-bit element offsets (3N < 2^32: set-up checks) keep addresses in SGPR base + VGPR offset form
+    const unsigned si = (unsigned)g.si, sj = (unsigned)g.sj, N = (unsigned)g.N;
+    unsigned q = (unsigned)ib * si + (unsigned)(lane ? j : 0) * sj + k;
+    // E of plane ib at the lane's point; H of plane ib - 1 (Hy, Hz) at owned points
+    float ex = 0.f, ey = 0.f, ez = 0.f, hym = 0.f, hzm = 0.f;
+    if (lane) {
+        ex = Exo[q];
+        ey = Eyo[q];
+        ez = Ezo[q];
+    }
+    if (own && ib >= 1) {
+        const unsigned qm = q - si;
+        const int im = ib - 1;
+        const float exm = Exo[qm], eym = Eyo[qm], ezm = Ezo[qm];
+        const float exk = Exo[k + 1 < kp ? qm + 1 : qm];
+        const float exj = jin ? Exo[qm + sj] : 0.f;
+        const int sx = f.h.has_pml ? ph.slot[0][im] : -1;
+        float hxv = 0.f, hyv = Hyo[qm], hzv = Hzo[qm];
+        float py0 = 0.f, py2 = 0.f, px1 = 0.f, px2 = 0.f, pz0 = 0.f, pz1 = 0.f, bx = 0.f, cx = 0.f;
+        if (syh >= 0) py2 = f.h.psi[2][1][((unsigned)syh * (g.nx + 1) + im) * (unsigned)kp + k];
+        if (sx >= 0) {
+            const unsigned ox = (unsigned)sx * si + (unsigned)j * sj + k;
+            px1 = f.h.psi[1][1][ox];
+            px2 = f.h.psi[2][0][ox];
+            bx = ph.b[0][im];
+            cx = ph.c[0][im];
+        }
+        if (zoh >= 0) pz1 = f.h.psi[1][0][(((unsigned)im * (g.ny + 1) + j) * ph.zg + zoh) * 4 + (k & 3)];
+        // Hx is not needed here: its E differences are passed as zero and its result dropped
+        h_point(hxv, hyv, hzv, 0.f, 0.f, exk - exm, exj - exm, ez - ezm, ey - eym, f.h.hs[0][im], hy_s, hz_s,
+                jin, kin, true, syh, byh, cyh, sx, bx, cx, inzh, bzh, czh, f.h.dth, py0, py2, px1, px2, pz0, pz1);
+        hym = hyv;
+        hzm = hzv;
+    }
+    const int rE = tj + 2, rH = tj + 1;
+    for (int i = ib; i < ie; i++, q += si) {
+        const bool iin = i < g.nx;
+        // loads of this plane: E of plane i + 1, old H and psi_H, and the E side's inputs
+        float exi = 0.f, eyi = 0.f, ezi = 0.f;
+        if (lane && iin) {
+            exi = Exo[q + si];
+            eyi = Eyo[q + si];
+            ezi = Ezo[q + si];
+        }
+        float hxv = 0.f, hyv = 0.f, hzv = 0.f;
+        float hpy0 = 0.f, hpy2 = 0.f, hpx1 = 0.f, hpx2 = 0.f, hpz0 = 0.f, hpz1 = 0.f, hbx = 0.f, hcx = 0.f;
+        const int sxh = f.h.has_pml && iin ? ph.slot[0][i] : -1;
+        unsigned hoy = 0, hox = 0, hoz = 0;
+        if (hrow) {
+            hxv = Hxo[q];
+            hyv = Hyo[q];
+            hzv = Hzo[q];
+            if (syh >= 0) {
+                hoy = ((unsigned)syh * (g.nx + 1) + i) * (unsigned)kp + k;
+                hpy0 = f.h.psi[0][0][hoy];
+                hpy2 = f.h.psi[2][1][hoy];
+            }
+            if (sxh >= 0) {
+                hox = (unsigned)sxh * si + (unsigned)j * sj + k;
+                hpx1 = f.h.psi[1][1][hox];
+                hpx2 = f.h.psi[2][0][hox];
+                hbx = ph.b[0][i];
+                hcx = ph.c[0][i];
+            }
+            if (zoh >= 0) {
+                hoz = (((unsigned)i * (g.ny + 1) + j) * ph.zg + zoh) * 4 + (k & 3);
+                hpz0 = f.h.psi[0][1][hoz];
+                hpz1 = f.h.psi[1][0][hoz];
+            }
+        }
+        // E-side inputs of the plane at owned points: classes or coefficients, psi_E,
+        // Debye currents (box layout of EArgs)
+        constexpr int NPE = (NP + 1) & ~1;
+        constexpr int NJ = NP > 0 ? NPE : 2;
+        unsigned cls = 0;
+        float ca[3], cb[3], jv[3][NJ];
+        float *Jl = nullptr;
+        long long jc = 0;
+        bool in_box = false;
+        int sx = -1;
+        float py0 = 0.f, py2 = 0.f, px1 = 0.f, px2 = 0.f, pz0 = 0.f, pz1 = 0.f, bx = 0.f, cx = 0.f;
+        unsigned oy = 0, ox = 0, oz = 0;
+        auto load_e = [&]() {
+            sx = a.has_pml ? pe.slot[0][i] : -1;
+            if (kCmp) {
+                cls = a.cidx[q];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float2 t = s_tab[c * 256 + ((cls >> (8 * c)) & 255u)];
+                    ca[c] = t.x;
+                    cb[c] = t.y;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    ca[c] = a.ca[c * N + q];
+                    cb[c] = a.cb[c * N + q];
+                }
+            }
+            in_box = kCmp && box_jk && i >= a.bi0 && i < a.bi1;
+            if (in_box) {
+                jc = 4LL * (a.bg1 - a.bg0) * NPE;
+                Jl = a.J + ((long long)(i - a.bi0) * (a.bj1 - a.bj0) + (j - a.bj0)) * 3 * jc + (k - 4 * a.bg0) * NPE;
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int u = 0; u < NJ; u += 2) {
+                        float t2[2];
+                        ldw<2>(Jl + c * jc + u, t2);
+                        jv[c][u] = t2[0];
+                        jv[c][u + 1] = t2[1];
+                    }
+            }
+            if (sye >= 0) {
+                oy = ((unsigned)sye * (g.nx + 1) + i) * (unsigned)kp + k;
+                py0 = a.psi[0][0][oy];
+                py2 = a.psi[2][1][oy];
+            }
+            if (sx >= 0) {
+                ox = (unsigned)sx * si + (unsigned)j * sj + k;
+                px1 = a.psi[1][1][ox];
+                px2 = a.psi[2][0][ox];
+                bx = pe.b[0][i];
+                cx = pe.c[0][i];
+            }
+            if (zoe >= 0) {
+                oz = (((unsigned)i * (g.ny + 1) + j) * pe.zg + zoe) * 4 + (k & 3);
+                pz0 = a.psi[0][1][oz];
+                pz1 = a.psi[1][0][oz];
+            }
+        };
+        if (lane) {
+            sE[(0 * rE + r) * kl + c] = ex;
+            sE[(1 * rE + r) * kl + c] = ey;
+            sE[(2 * rE + r) * kl + c] = ez;
+        }
+        __syncthreads();   // E of plane i complete in LDS; last plane's H reads done
+        if (hrow) {
+            const float eyk = sE[(1 * rE + r) * kl + cn], exk = sE[(0 * rE + r) * kl + cn];
+            const float ezj = jin ? sE[(2 * rE + r + 1) * kl + c] : 0.f;
+            const float exj = jin ? sE[(0 * rE + r + 1) * kl + c] : 0.f;
+            h_point(hxv, hyv, hzv, eyk - ey, ezj - ez, exk - ex, exj - ex, ezi - ez, eyi - ey,
+                    iin ? f.h.hs[0][i] : 0.f, hy_s, hz_s, jin, kin, iin, syh, byh, cyh, sxh, hbx, hcx, inzh, bzh, czh,
+                    f.h.dth, hpy0, hpy2, hpx1, hpx2, hpz0, hpz1);
+            sH[(0 * rH + r) * kl + c] = hxv;
+            sH[(1 * rH + r) * kl + c] = hyv;
+            sH[(2 * rH + r) * kl + c] = hzv;
+        }
+        if (own) {
+            f.Hn[q] = hxv;
+            f.Hn[q + N] = hyv;
+            f.Hn[q + 2 * N] = hzv;
+            // every psi_H entry of the new set is written, updated or not
+            if (syh >= 0) {
+                f.psiHn[0][0][hoy] = hpy0;
+                f.psiHn[2][1][hoy] = hpy2;
+            }
+            if (sxh >= 0) {
+                f.psiHn[1][1][hox] = hpx1;
+                f.psiHn[2][0][hox] = hpx2;
+            }
+            if (zoh >= 0) {
+                f.psiHn[0][1][hoz] = hpz0;
+                f.psiHn[1][0][hoz] = hpz1;
+            }
+        }
+        __syncthreads();   // H of plane i complete in LDS
+        if (own) {
+            float exn = ex, eyn = ey, ezn = ez;
+            if (iin) {
+                const bool ib_ = i >= 1;
+                const float hxk = sH[(0 * rH + r) * kl + cm], hyk = sH[(1 * rH + r) * kl + cm];
+                const float hxj = j >= 1 ? sH[(0 * rH + r - 1) * kl + c] : 0.f;
+                const float hzj = j >= 1 ? sH[(2 * rH + r - 1) * kl + c] : 0.f;
+                const float ex_s = a.es[0][i];
+                load_e();
+                e_point(exn, eyn, ezn, ex, ey, ez, ca[0], cb[0], ca[1], cb[1], ca[2], cb[2], hzv - hzj, hyv - hyk,
+                        hxv - hxk, hzv - hzm, hyv - hym, hxv - hxj, ex_s, ey_s, ez_s, jb, jin, ib_, kb, kz, sye, bye,
+                        cye, sx, bx, cx, inze, bze, cze, py0, py2, px1, px2, pz0, pz1);
+                if (in_box) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        float &en = c == 0 ? exn : c == 1 ? eyn : ezn;
+                        const float e0 = c == 0 ? ex : c == 1 ? ey : ez;
+                        const int row = s_row[c * 256 + ((cls >> (8 * c)) & 255u)];
+                        if (row >= 0) debye_point<NP>(en, e0, cb[c], jv[c], s_bt + row * NP, a.kc, a.km1);
+#pragma unroll
+                        for (int u = 0; u < NJ; u += 2) {
+                            const float t2[2] = {jv[c][u], jv[c][u + 1]};
+                            stw<2>(Jl + c * jc + u, t2);
+                        }
+                    }
+                }
+                if (sye >= 0) {
+                    a.psi[0][0][oy] = py0;
+                    a.psi[2][1][oy] = py2;
+                }
+                if (sx >= 0) {
+                    a.psi[1][1][ox] = px1;
+                    a.psi[2][0][ox] = px2;
+                }
+                if (zoe >= 0) {
+                    a.psi[0][1][oz] = pz0;
+                    a.psi[1][0][oz] = pz1;
+                }
+            }
+            f.En[q] = exn;
+            f.En[q + N] = eyn;
+            f.En[q + 2 * N] = ezn;
+            hym = hyv;
+            hzm = hzv;
+        }
+        ex = exi;
+        ey = eyi;
+        ez = ezi;
     }
 }
 
@@ -1230,7 +1563,12 @@ struct KernelTimer {
 struct Device {
     hipStream_t st = nullptr;
     Grid g{};
-    DBuf<float> F;               // E (3N) then H (3N)
+    DBuf<float> F;               // E (3N) then H (3N); two such sets for the one-pass step
+    int par = 0;                 // set holding the current fields (and psi_H)
+    bool onepass = false;        // steps run k_step (H and E in one sweep, set to set)
+    int tj = 0, ntj = 0;         // k_step: owned rows per tile, tiles over j
+    int kw = 0, ntk = 0;         // k_step: owned k points per tile, tiles over k
+    int iseg_s = 8;              // k_step: x planes per workgroup
     DBuf<float> ca, cb;          // 3N each, only when more than 256 distinct pairs
     DBuf<unsigned> cidx;         // N packed coefficient classes
     DBuf<float2> ctab;           // 3 x 256 (ca, cb)
@@ -1252,7 +1590,7 @@ struct Device {
     std::vector<int> slots_host;
     size_t slot_off[2][3] = {}, zo_off[2] = {};   // [is_e][axis] offsets into slots
     DBuf<float> psi;             // all psi arrays
-    float *psiE[3][2] = {}, *psiH[3][2] = {};
+    float *psiE[3][2] = {}, *psiHs[2][3][2] = {};   // psi_H per field set
     size_t psiE_len[3][2] = {}, psiH_len[3][2] = {};
     const float *hs[3] = {}, *es[3] = {};
     Pml pmlH{}, pmlE{};
@@ -1298,13 +1636,13 @@ struct Device {
     DBuf<double> d1d, partial;
     int energy_blocks = 0;
     // launch shape: x planes per thread in the stencil kernels, set by the autotuner
-    // on the first steps unless fixed by COURANT_ISEG_H / COURANT_ISEG_E
+    // on the first steps unless fixed by COURANT_ISEG_H / COURANT_ISEG_E / COURANT_ISEG_S
     int iseg_h = kIseg, iseg_e = kIseg;
     int wh = 4, we = 1;          // lane widths (k points per thread) of the H and E updates
     bool tuned = false;
     // timing
     bool profiling = false;
-    KernelTimer tH, tE, tG, tD, tP;
+    KernelTimer tH, tE, tG, tD, tP, tS;
     // captured chunk of kGraphSteps steps, valid while its launch arguments are unchanged
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
@@ -1557,8 +1895,28 @@ int upload_plan(cem_sim *s, Device *D) {
         return i * (size_t)g.si + j * (size_t)g.sj + k;
     };
 
-    // fields (zero, row padding included)
-    if (D->F.alloc(6 * Nd, D->st) || D->F.zero(st)) return -1;
+    // fields (zero, row padding included).  Full steps run the one-pass step only on
+    // request (COURANT_STEP=onepass at set-up): on the four-layer board it takes 1.36 ms
+    // a step against 0.91 ms for the two sweeps.  Its workgroup holds a tile of tj rows
+    // by kw points of k plus a halo row and a halo column on each side, at most
+    // kStepLanes lanes; whole rows of k (one k tile) are taken when two rows fit,
+    // otherwise the fewest k tiles that leave two rows.
+    {
+        const char *mode = std::getenv("COURANT_STEP");
+        int tj = 0, kw = 0, ntk = 0;
+        for (int n = 1; n <= 8 && tj < 2; n++) {
+            kw = (g.kp + n - 1) / n;
+            tj = std::min(kStepRows, kStepLanes / (kw + 2) - 2);
+            ntk = n;
+        }
+        D->onepass = tj >= 2 && mode && std::strcmp(mode, "onepass") == 0;
+        D->tj = D->onepass ? tj : 0;
+        D->ntj = D->onepass ? (ny + 1 + tj - 1) / tj : 0;
+        D->kw = D->onepass ? kw : 0;
+        D->ntk = D->onepass ? ntk : 0;
+        D->par = 0;
+    }
+    if (D->F.alloc((D->onepass ? 12 : 6) * Nd, D->st) || D->F.zero(st)) return -1;
 
     // Debye materials: dispersive edges share a handful of (cb, beta_p / dt) rows
     const int np = s->npole, nd = s->ndisp;
@@ -1738,7 +2096,7 @@ int upload_plan(cem_sim *s, Device *D) {
         for (int w = 0; w < 2; w++) {
             D->psiE_len[c][w] = psi_dev_len(s, D, 1, c, w);
             D->psiH_len[c][w] = psi_dev_len(s, D, 0, c, w);
-            psi_total += D->psiE_len[c][w] + D->psiH_len[c][w];
+            psi_total += D->psiE_len[c][w] + (D->onepass ? 2 : 1) * D->psiH_len[c][w];
         }
     if (D->psi.alloc(psi_total + 4, D->st) || D->psi.zero(st)) return -1;
     {
@@ -1747,8 +2105,10 @@ int upload_plan(cem_sim *s, Device *D) {
             for (int w = 0; w < 2; w++) {
                 D->psiE[c][w] = D->psi.p + o;
                 o += D->psiE_len[c][w];
-                D->psiH[c][w] = D->psi.p + o;
-                o += D->psiH_len[c][w];
+                for (int set = 0; set < (D->onepass ? 2 : 1); set++) {
+                    D->psiHs[set][c][w] = D->psi.p + o;
+                    o += D->psiH_len[c][w];
+                }
             }
     }
 
@@ -1930,27 +2290,32 @@ int upload_plan(cem_sim *s, Device *D) {
     return 0;
 }
 
-HArgs h_args(cem_sim *s, Device *D) {
+// first float of field set par (E then H)
+float *fset(const Device *D, int par) {
+    return D->F.p + (size_t)par * 6 * (size_t)D->g.N;
+}
+
+HArgs h_args(cem_sim *s, Device *D, int par) {
     HArgs a;
     std::memset(&a, 0, sizeof a);   // padding too: the bytes key the captured graph
     a.g = D->g;
-    a.E = D->F.p;
-    a.H = D->F.p + 3 * D->g.N;
+    a.E = fset(D, par);
+    a.H = fset(D, par) + 3 * D->g.N;
     for (int i = 0; i < 3; i++) a.hs[i] = D->hs[i];
     a.pml = D->pmlH;
     for (int c = 0; c < 3; c++)
-        for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiH[c][w];
+        for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiHs[par][c][w];
     a.dth = (float)(s->dt / CEM_MU0);
     a.has_pml = s->has_pml;
     return a;
 }
 
-EArgs e_args(cem_sim *s, Device *D) {
+EArgs e_args(cem_sim *s, Device *D, int par) {
     EArgs a;
     std::memset(&a, 0, sizeof a);
     a.g = D->g;
-    a.E = D->F.p;
-    a.H = D->F.p + 3 * D->g.N;
+    a.E = fset(D, par);
+    a.H = fset(D, par) + 3 * D->g.N;
     a.ca = D->ca.p;
     a.cb = D->cb.p;
     a.cidx = D->compressed ? D->cidx.p : nullptr;
@@ -2016,9 +2381,9 @@ int stencil_blocks(const Device *D, int w) {
 // Kernel launch on the engine stream; when profiling, the launch carries a start /
 // stop event pair that the runtime stamps with the dispatch's own begin and end.
 template <typename F, typename... A>
-void launch(Device *D, KernelTimer &t, F kernel, dim3 grid, dim3 block, A... args) {
+void launch_lds(Device *D, KernelTimer &t, F kernel, dim3 grid, dim3 block, size_t lds, A... args) {
     if (!D->profiling) {
-        hipLaunchKernelGGL(kernel, grid, block, 0, D->st, args...);
+        hipLaunchKernelGGL(kernel, grid, block, lds, D->st, args...);
         return;
     }
     if (t.used == 4096) t.fold();
@@ -2028,31 +2393,105 @@ void launch(Device *D, KernelTimer &t, F kernel, dim3 grid, dim3 block, A... arg
         HIP_CHECK_VOID(hipEventCreate(&b));
         t.pool.push_back({a, b});
     }
-    hipExtLaunchKernelGGL(kernel, grid, block, 0, D->st, t.pool[t.used].first, t.pool[t.used].second, 0, args...);
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, D->st, t.pool[t.used].first, t.pool[t.used].second, 0, args...);
     t.used++;
+}
+
+template <typename F, typename... A>
+void launch(Device *D, KernelTimer &t, F kernel, dim3 grid, dim3 block, A... args) {
+    launch_lds(D, t, kernel, grid, block, 0, args...);
 }
 
 void launch_h(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
-    HArgs a = h_args(s, D);
+    HArgs a = h_args(s, D, D->par);
     const int npblk = stencil_blocks(D, D->wh), nseg = (i1 - i0 + D->iseg_h - 1) / D->iseg_h;
     launch(D, D->tH, h_kernel(D), dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_h, npblk);
 }
 
 void launch_e(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
-    EArgs a = e_args(s, D);
+    EArgs a = e_args(s, D, D->par);
     const int npblk = stencil_blocks(D, D->we), nseg = (i1 - i0 + D->iseg_e - 1) / D->iseg_e;
     launch(D, D->tE, e_kernel(D), dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_e, npblk);
 }
 
+typedef void (*SKernel)(SArgs, int);
+
+SKernel step_kernel(const Device *D) {
+    if (!D->compressed) return k_step<0, false>;
+    switch (D->fused ? D->npole : 0) {
+    case 1: return k_step<1, true>;
+    case 2: return k_step<2, true>;
+    case 3: return k_step<3, true>;
+    case 4: return k_step<4, true>;
+    case 5: return k_step<5, true>;
+    case 6: return k_step<6, true>;
+    case 7: return k_step<7, true>;
+    case 8: return k_step<8, true>;
+    default: return k_step<0, true>;
+    }
+}
+
+// k_step from field set par to set par ^ 1
+SArgs step_args(cem_sim *s, Device *D, int par) {
+    SArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.h = h_args(s, D, par);
+    a.e = e_args(s, D, par);
+    a.En = fset(D, par ^ 1);
+    a.Hn = fset(D, par ^ 1) + 3 * D->g.N;
+    for (int c = 0; c < 3; c++)
+        for (int w = 0; w < 2; w++) a.psiHn[c][w] = D->psiHs[par ^ 1][c][w];
+    a.tj = D->tj;
+    a.ntj = D->ntj;
+    a.kw = D->kw;
+    a.ntk = D->ntk;
+    a.kl = D->kw + 2;
+    return a;
+}
+
+int step_threads(const Device *D) {
+    return ((D->tj + 2) * (D->kw + 2) + 63) / 64 * 64;
+}
+
+size_t step_lds(const Device *D) {
+    const int kl = D->kw + 2, tj = D->tj;
+    size_t b = (size_t)3 * (2 * tj + 3) * kl * sizeof(float);
+    if (D->compressed) b += 768 * sizeof(float2);
+    if (D->compressed && D->fused && D->npole > 0)
+        b += 768 * sizeof(int) + (size_t)D->nrow * D->npole * sizeof(float);
+    return b;
+}
+
+int step_blocks(const Device *D) {
+    return D->ntj * D->ntk * ((D->g.nx + 1 + D->iseg_s - 1) / D->iseg_s);
+}
+
+// H and E of one step in one pass; the new fields become the current set
+void launch_step(cem_sim *s, Device *D) {
+    SArgs a = step_args(s, D, D->par);
+    launch_lds(D, D->tS, step_kernel(D), dim3(step_blocks(D)), dim3(step_threads(D)), step_lds(D), a, D->iseg_s);
+    D->par ^= 1;
+}
+
+// bulk updates of one full step
+void launch_he(cem_sim *s, Device *D) {
+    if (D->onepass) {
+        launch_step(s, D);
+        return;
+    }
+    launch_h(s, D, 0, s->n[0] + 1);
+    launch_e(s, D, 0, s->n[0]);
+}
+
 void launch_dft(cem_sim *s, Device *D, int direct);
 
-PostArgs post_args(cem_sim *s, Device *D) {
+PostArgs post_args(cem_sim *s, Device *D, int par) {
     PostArgs a;
     std::memset(&a, 0, sizeof a);
-    a.E = D->F.p;
-    a.H = D->F.p + 3 * D->g.N;
+    a.E = fset(D, par);
+    a.H = fset(D, par) + 3 * D->g.N;
     a.N = D->g.N;
     a.si = D->g.si;
     a.sj = D->g.sj;
@@ -2096,10 +2535,10 @@ PostArgs post_args(cem_sim *s, Device *D) {
     return a;
 }
 
-GatherArgs gather_args(cem_sim *s, Device *D, int mid_only) {
+GatherArgs gather_args(cem_sim *s, Device *D, int mid_only, int par) {
     GatherArgs a;
     std::memset(&a, 0, sizeof a);
-    a.F = D->F.p;
+    a.F = fset(D, par);
     a.tidx = D->tidx.p;
     a.tw = D->tw.p;
     a.toff = D->toff.p;
@@ -2136,10 +2575,10 @@ void sync_ctl(cem_sim *s, Device *D) {
 void launch_post(cem_sim *s, Device *D) {
     sync_ctl(s, D);
     if (const int nb = post_blocks(s, D)) {
-        PostArgs a = post_args(s, D);
+        PostArgs a = post_args(s, D, D->par);
         launch(D, D->tP, k_post, dim3(nb), dim3(kBlock), a);
     }
-    GatherArgs g = gather_args(s, D, 0);
+    GatherArgs g = gather_args(s, D, 0, D->par);
     launch(D, D->tG, k_gather, dim3(1), dim3(kGatherBlock), g);
     if (D->rows == 0) D->row0 = s->step;
     D->rows++;
@@ -2191,7 +2630,7 @@ void launch_dft(cem_sim *s, Device *D, int direct) {
 // V of every port before the E update (after a host write of E or a resume)
 void launch_gather_mid(cem_sim *s, Device *D) {
     if (!D->nsamp) return;
-    GatherArgs a = gather_args(s, D, 1);
+    GatherArgs a = gather_args(s, D, 1, D->par);
     hipLaunchKernelGGL(k_gather, dim3(1), dim3(kGatherBlock), 0, D->st, a);
 }
 
@@ -2201,7 +2640,7 @@ int ensure_vlast(cem_sim *s, Device *D) {
         D->vlast_stale = false;
     }
     if (D->eold_stale && D->nlisted) {
-        hipLaunchKernelGGL(k_prime_eold, dim3((D->nlisted + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, D->F.p,
+        hipLaunchKernelGGL(k_prime_eold, dim3((D->nlisted + kBlock - 1) / kBlock), dim3(kBlock), 0, D->st, fset(D, D->par),
                            D->d_idx.p, D->Eold.p, D->nlisted);
     }
     D->eold_stale = false;
@@ -2243,15 +2682,26 @@ void drop_graph(Device *D) {
 int run_chunk(cem_sim *s, Device *D) {
     sync_ctl(s, D);
     const int nx = s->n[0];
-    const HArgs ha = h_args(s, D);
-    const EArgs ea = e_args(s, D);
-    const PostArgs pa = post_args(s, D);
-    const GatherArgs ga = gather_args(s, D, 0);
-    const int geo[7] = {D->iseg_h, D->iseg_e, post_blocks(s, D), (int)D->compressed, (int)D->fused, D->wh, D->we};
+    // launch arguments of the chunk's steps: with the one-pass step the field sets
+    // alternate, so even and odd steps differ (kGraphSteps is even: the chunk ends
+    // on the set it started from)
+    const int p0 = D->par, p1 = D->onepass ? D->par ^ 1 : D->par;
+    const HArgs ha = h_args(s, D, p0);
+    const EArgs ea = e_args(s, D, p0);
+    SArgs sa[2];
+    std::memset(sa, 0, sizeof sa);   // the bytes key the captured graph
+    if (D->onepass) {
+        sa[0] = step_args(s, D, p0);
+        sa[1] = step_args(s, D, p1);
+    }
+    const PostArgs pa[2] = {post_args(s, D, p1), post_args(s, D, p0)};
+    const GatherArgs ga[2] = {gather_args(s, D, 0, p1), gather_args(s, D, 0, p0)};
+    const int geo[10] = {D->iseg_h, D->iseg_e, post_blocks(s, D), (int)D->compressed, (int)D->fused, D->wh, D->we,
+                         (int)D->onepass, D->iseg_s, D->tj};
     std::vector<unsigned char> key;
     for (auto pr : {std::make_pair((const void *)&ha, sizeof ha), std::make_pair((const void *)&ea, sizeof ea),
-                    std::make_pair((const void *)&pa, sizeof pa), std::make_pair((const void *)&ga, sizeof ga),
-                    std::make_pair((const void *)geo, sizeof geo)}) {
+                    std::make_pair((const void *)sa, sizeof sa), std::make_pair((const void *)pa, sizeof pa),
+                    std::make_pair((const void *)ga, sizeof ga), std::make_pair((const void *)geo, sizeof geo)}) {
         const unsigned char *b = (const unsigned char *)pr.first;
         key.insert(key.end(), b, b + pr.second);
     }
@@ -2265,13 +2715,20 @@ int run_chunk(cem_sim *s, Device *D) {
         const int nbh = stencil_blocks(D, D->wh), nbe = stencil_blocks(D, D->we);
         const EKernel ek = e_kernel(D);
         const HKernel hk = h_kernel(D);
+        const SKernel sk = D->onepass ? step_kernel(D) : nullptr;
         const int gh = nbh * ((nx + 1 + D->iseg_h - 1) / D->iseg_h);
         const int ge = nbe * ((nx + D->iseg_e - 1) / D->iseg_e);
         for (int k = 0; k < kGraphSteps; k++) {
-            hipLaunchKernelGGL(hk, dim3(gh), dim3(kBlock), 0, D->st, ha, 0, nx + 1, D->iseg_h, nbh);
-            hipLaunchKernelGGL(ek, dim3(ge), dim3(kBlock), 0, D->st, ea, 0, nx, D->iseg_e, nbe);
-            if (geo[2]) hipLaunchKernelGGL(k_post, dim3(geo[2]), dim3(kBlock), 0, D->st, pa);
-            hipLaunchKernelGGL(k_gather, dim3(1), dim3(kGatherBlock), 0, D->st, ga);
+            const int o = D->onepass ? (k & 1) : 0;
+            if (D->onepass) {
+                hipLaunchKernelGGL(sk, dim3(step_blocks(D)), dim3(step_threads(D)), step_lds(D), D->st, sa[o],
+                                   D->iseg_s);
+            } else {
+                hipLaunchKernelGGL(hk, dim3(gh), dim3(kBlock), 0, D->st, ha, 0, nx + 1, D->iseg_h, nbh);
+                hipLaunchKernelGGL(ek, dim3(ge), dim3(kBlock), 0, D->st, ea, 0, nx, D->iseg_e, nbe);
+            }
+            if (geo[2]) hipLaunchKernelGGL(k_post, dim3(geo[2]), dim3(kBlock), 0, D->st, pa[o]);
+            hipLaunchKernelGGL(k_gather, dim3(1), dim3(kGatherBlock), 0, D->st, ga[o]);
         }
         const hipError_t ce = hipStreamEndCapture(D->st, &graph);
         if (ce != hipSuccess || !graph || hipGraphInstantiate(&D->gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
@@ -2333,6 +2790,7 @@ int cem::dev_ensure(cem_sim *s) {
         }
     if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v)), D->tuned = true;
     if (const char *v = std::getenv("COURANT_ISEG_E")) D->iseg_e = std::max(1, std::atoi(v)), D->tuned = true;
+    if (const char *v = std::getenv("COURANT_ISEG_S")) D->iseg_s = std::max(1, std::atoi(v)), D->tuned = true;
     if (const char *v = std::getenv("COURANT_AUTOTUNE"))
         if (std::atoi(v) == 0) D->tuned = true;
     s->spec_src.assign(2 * s->nf, 0.0);
@@ -2393,7 +2851,7 @@ void cem::dev_destroy(cem_sim *s) {
     D->R.release();
     D->d1d.release();
     D->partial.release();
-    for (KernelTimer *t : {&D->tH, &D->tE, &D->tG, &D->tD, &D->tP}) t->destroy();
+    for (KernelTimer *t : {&D->tH, &D->tE, &D->tG, &D->tD, &D->tP, &D->tS}) t->destroy();
     (void)hipStreamDestroy(D->st);
     delete D;
     s->dev = nullptr;
@@ -2415,78 +2873,106 @@ void cem::dev_sources_changed(cem_sim *s) {
 // boards tunes once).  COURANT_TUNE_DEBUG prints the measured times.
 constexpr int kTuneCand[] = {1, 2, 4, 8};
 constexpr int kTuneNc = sizeof kTuneCand / sizeof kTuneCand[0];
+constexpr int kTuneCandS[] = {4, 8, 16};   // one-pass step: x planes per workgroup
+constexpr int kTuneNs = sizeof kTuneCandS / sizeof kTuneCandS[0];
 constexpr int kTuneSteps = 2;
 constexpr int kTuneRounds = 2;
-constexpr long kTuneCost = 1 + (long)kTuneRounds * kTuneNc * kTuneNc * (1 + kTuneSteps);
+constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * kTuneNc * kTuneNc * (1 + kTuneSteps);
+constexpr long kTuneCostStep = 1 + (long)kTuneRounds * kTuneNs * (1 + kTuneSteps);
+constexpr long kTuneCost = kTuneCostSplit > kTuneCostStep ? kTuneCostSplit : kTuneCostStep;
 
 std::mutex g_tune_mu;
-std::map<std::array<long long, 8>, std::pair<int, int>> g_tune;
+std::map<std::array<long long, 10>, std::array<int, 3>> g_tune;
 
 long autotune(cem_sim *s, Device *D, long budget) {
-    const std::array<long long, 8> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
-                                          (long long)D->fused, D->wh, D->we};
+    const std::array<long long, 10> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
+                                           (long long)D->fused, D->wh, D->we, (long long)D->onepass, (long long)D->tj * 4096 + D->kw};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
         if (it != g_tune.end()) {
-            D->iseg_h = it->second.first;
-            D->iseg_e = it->second.second;
+            D->iseg_h = it->second[0];
+            D->iseg_e = it->second[1];
+            D->iseg_s = it->second[2];
             D->tuned = true;
             return 0;
         }
     }
-    if (budget < kTuneCost) return 0;   // too few steps in this call: tune on a later one
-    const int nx = s->n[0];
+    const long cost = D->onepass ? kTuneCostStep : kTuneCostSplit;
+    if (budget < cost) return 0;        // too few steps in this call: tune on a later one
     auto one_step = [&] {
-        launch_h(s, D, 0, nx + 1);
-        launch_e(s, D, 0, nx);
+        launch_he(s, D);
         launch_post(s, D);
     };
+    auto reset = [&](std::initializer_list<KernelTimer *> ts) {
+        for (KernelTimer *k : ts) {
+            k->fold();
+            k->total_ms = 0;
+            k->count = 0;
+        }
+    };
     one_step();                         // warm-up step (first launches, caches)
-    double t[kTuneNc][kTuneNc];
-    for (int r = 0; r < kTuneRounds; r++)
-        for (int ch = 0; ch < kTuneNc; ch++)
-            for (int ce = 0; ce < kTuneNc; ce++) {
-                D->iseg_h = kTuneCand[ch];
-                D->iseg_e = kTuneCand[ce];
+    const bool dbg = std::getenv("COURANT_TUNE_DEBUG") != nullptr;
+    if (D->onepass) {
+        double t[kTuneNs];
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int c = 0; c < kTuneNs; c++) {
+                D->iseg_s = kTuneCandS[c];
                 one_step();             // settling step, untimed
                 D->profiling = true;
-                for (KernelTimer *k : {&D->tH, &D->tE}) {
-                    k->fold();
-                    k->total_ms = 0;
-                    k->count = 0;
-                }
+                reset({&D->tS});
                 for (int st = 0; st < kTuneSteps; st++) one_step();
-                D->tH.fold();
-                D->tE.fold();
+                D->tS.fold();
                 D->profiling = false;
-                const double v = D->tH.total_ms + D->tE.total_ms;
-                t[ch][ce] = r ? std::min(t[ch][ce], v) : v;
+                t[c] = r ? std::min(t[c], D->tS.total_ms) : D->tS.total_ms;
             }
-    int bh = 0, be = 0;
-    for (int ch = 0; ch < kTuneNc; ch++)
-        for (int ce = 0; ce < kTuneNc; ce++)
-            if (t[ch][ce] < t[bh][be]) bh = ch, be = ce;
-    for (KernelTimer *k : {&D->tH, &D->tE, &D->tG, &D->tD, &D->tP}) {
-        k->fold();
-        k->total_ms = 0;
-        k->count = 0;
-    }
-    D->iseg_h = kTuneCand[bh];
-    D->iseg_e = kTuneCand[be];
-    D->tuned = true;
-    if (std::getenv("COURANT_TUNE_DEBUG")) {
-        std::fprintf(stderr, "courant autotune H+E us per step (rows H, columns E = 1 2 4 8):\n");
-        for (int ch = 0; ch < kTuneNc; ch++) {
-            std::fprintf(stderr, "  H %d:", kTuneCand[ch]);
-            for (int ce = 0; ce < kTuneNc; ce++) std::fprintf(stderr, " %7.1f", t[ch][ce] / kTuneSteps * 1e3);
-            std::fprintf(stderr, "\n");
+        int b = 0;
+        for (int c = 0; c < kTuneNs; c++)
+            if (t[c] < t[b]) b = c;
+        D->iseg_s = kTuneCandS[b];
+        if (dbg) {
+            std::fprintf(stderr, "courant autotune one-pass step us (x planes per workgroup):");
+            for (int c = 0; c < kTuneNs; c++) std::fprintf(stderr, " %d: %.1f", kTuneCandS[c], t[c] / kTuneSteps * 1e3);
+            std::fprintf(stderr, " -> %d\n", D->iseg_s);
         }
-        std::fprintf(stderr, "  -> H %d, E %d\n", D->iseg_h, D->iseg_e);
+    } else {
+        double t[kTuneNc][kTuneNc];
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int ch = 0; ch < kTuneNc; ch++)
+                for (int ce = 0; ce < kTuneNc; ce++) {
+                    D->iseg_h = kTuneCand[ch];
+                    D->iseg_e = kTuneCand[ce];
+                    one_step();         // settling step, untimed
+                    D->profiling = true;
+                    reset({&D->tH, &D->tE});
+                    for (int st = 0; st < kTuneSteps; st++) one_step();
+                    D->tH.fold();
+                    D->tE.fold();
+                    D->profiling = false;
+                    const double v = D->tH.total_ms + D->tE.total_ms;
+                    t[ch][ce] = r ? std::min(t[ch][ce], v) : v;
+                }
+        int bh = 0, be = 0;
+        for (int ch = 0; ch < kTuneNc; ch++)
+            for (int ce = 0; ce < kTuneNc; ce++)
+                if (t[ch][ce] < t[bh][be]) bh = ch, be = ce;
+        D->iseg_h = kTuneCand[bh];
+        D->iseg_e = kTuneCand[be];
+        if (dbg) {
+            std::fprintf(stderr, "courant autotune H+E us per step (rows H, columns E = 1 2 4 8):\n");
+            for (int ch = 0; ch < kTuneNc; ch++) {
+                std::fprintf(stderr, "  H %d:", kTuneCand[ch]);
+                for (int ce = 0; ce < kTuneNc; ce++) std::fprintf(stderr, " %7.1f", t[ch][ce] / kTuneSteps * 1e3);
+                std::fprintf(stderr, "\n");
+            }
+            std::fprintf(stderr, "  -> H %d, E %d\n", D->iseg_h, D->iseg_e);
+        }
     }
+    reset({&D->tH, &D->tE, &D->tG, &D->tD, &D->tP, &D->tS});
+    D->tuned = true;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = {D->iseg_h, D->iseg_e};
-    return kTuneCost;
+    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s};
+    return cost;
 }
 
 void cem::dev_step(cem_sim *s, long nsteps) {
@@ -2502,8 +2988,7 @@ void cem::dev_step(cem_sim *s, long nsteps) {
             it += kGraphSteps;
             continue;
         }
-        launch_h(s, D, 0, nx + 1);
-        launch_e(s, D, 0, nx);
+        launch_he(s, D);
         launch_post(s, D);
         it++;
     }
@@ -2527,8 +3012,8 @@ double cem::dev_energy(cem_sim *s) {
     Device *D = s->dev;
     EnergyArgs a{};
     a.g = D->g;
-    a.E = D->F.p;
-    a.H = D->F.p + 3 * D->g.N;
+    a.E = fset(D, D->par);
+    a.H = fset(D, D->par) + 3 * D->g.N;
     a.cb = D->cb.p;
     a.cidx = D->compressed ? D->cidx.p : nullptr;
     a.ctab = D->compressed ? D->ctab.p : nullptr;
@@ -2586,7 +3071,7 @@ void cem::dev_field_download(cem_sim *s, int comp, double *dst) {
         std::memset(dst, 0, s->N * sizeof(double));
         return;
     }
-    const std::vector<float> t = download_raw(D, D->F.p + (size_t)comp * D->g.N, (size_t)D->g.N);
+    const std::vector<float> t = download_raw(D, fset(D, D->par) + (size_t)comp * D->g.N, (size_t)D->g.N);
     rows_to_host(s, D->g, t.data(), dst);
 }
 
@@ -2599,7 +3084,7 @@ void cem::dev_state_download(cem_sim *s, int which, int comp, int w, double *dst
     const size_t nd = s->ndisp;
     if (which == 0 || which == 1) {
         const int is_e = which == 0;
-        float *p = is_e ? D->psiE[comp][w] : D->psiH[comp][w];
+        float *p = is_e ? D->psiE[comp][w] : D->psiHs[D->par][comp][w];
         const size_t len = is_e ? D->psiE_len[comp][w] : D->psiH_len[comp][w];
         if (!len) return;
         const std::vector<float> t = download_raw(D, p, len);
@@ -2625,7 +3110,7 @@ void cem::dev_state_download(cem_sim *s, int which, int comp, int w, double *dst
             return;
         }
         // the E kernel reads E_old from E itself; listed edges hold the same value
-        const std::vector<float> t = download_raw(D, D->F.p, 3 * (size_t)g.N);
+        const std::vector<float> t = download_raw(D, fset(D, D->par), 3 * (size_t)g.N);
         for (size_t e = 0; e < nd; e++) {
             const unsigned long long gi = (unsigned long long)s->d_comp[e] * s->N + s->d_idx[e];
             dst[e] = t[dev_index(s, g, gi)];
@@ -2649,7 +3134,7 @@ void cem::dev_state_upload(cem_sim *s, int which, int comp, int w, const double 
     const size_t nd = s->ndisp;
     if (which == 0 || which == 1) {
         const int is_e = which == 0;
-        float *p = is_e ? D->psiE[comp][w] : D->psiH[comp][w];
+        float *p = is_e ? D->psiE[comp][w] : D->psiHs[D->par][comp][w];
         const size_t len = is_e ? D->psiE_len[comp][w] : D->psiH_len[comp][w];
         if (!len) return;
         std::vector<float> t(len, 0.f);
@@ -2685,7 +3170,7 @@ void cem::dev_state_upload(cem_sim *s, int which, int comp, int w, const double 
     if (which == 5) {
         std::vector<float> t((size_t)g.N, 0.f);
         rows_to_device(s, g, src, t.data());
-        upload_raw(D, D->F.p + (size_t)comp * g.N, t);
+        upload_raw(D, fset(D, D->par) + (size_t)comp * g.N, t);
         if (comp < 3) D->vlast_stale = D->eold_stale = true;
     }
 }
@@ -2862,8 +3347,8 @@ void cem::dev_set_step(cem_sim *s, long step) {
 // ---------------------------------------------------------------------------
 extern "C" int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count) {
     if (!s || !s->dev) return -1;
-    KernelTimer *t[5] = {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD, &s->dev->tP};
-    if (which < 0 || which > 4) return -1;
+    KernelTimer *t[6] = {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD, &s->dev->tP, &s->dev->tS};
+    if (which < 0 || which > 5) return -1;
     t[which]->fold();
     *total_ms = t[which]->total_ms;
     *count = t[which]->count;
@@ -2873,7 +3358,7 @@ extern "C" int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, lon
 extern "C" int cem_amd_set_profiling(cem_sim *s, int on) {
     if (!s || !s->finalized || dev_ensure(s)) return -1;
     s->dev->profiling = on != 0;
-    for (KernelTimer *t : {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD, &s->dev->tP}) {
+    for (KernelTimer *t : {&s->dev->tH, &s->dev->tE, &s->dev->tG, &s->dev->tD, &s->dev->tP, &s->dev->tS}) {
         t->fold();
         t->total_ms = 0;
         t->count = 0;
@@ -3033,6 +3518,15 @@ extern "C" int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e) {
     if (iseg_h) *iseg_h = s->dev->iseg_h;
     if (iseg_e) *iseg_e = s->dev->iseg_e;
     return s->dev->tuned ? 1 : 0;
+}
+
+// Shape of the one-pass step kernel: rows per tile and x planes per workgroup;
+// rows 0 when the simulation runs the two sweeps.
+extern "C" int cem_amd_step_shape(cem_sim *s, int *rows, int *planes) {
+    if (!s || !s->finalized || dev_ensure(s)) return -1;
+    *rows = s->dev->onepass ? s->dev->tj : 0;
+    *planes = s->dev->onepass ? s->dev->iseg_s : 0;
+    return 0;
 }
 
 extern "C" double cem_amd_copy_bandwidth(size_t bytes, int iters) {

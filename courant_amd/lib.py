@@ -135,6 +135,7 @@ AMD_SIGNATURES = {
     "cem_amd_s2_renormalize": (_I, [_DP, _I, _DP, _DP, _D]),
     "cem_amd_s2_renormalize_noalloc": (_I, [_DP, _I, _DP, _DP, _D]),
     "cem_amd_launch_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cem_amd_step_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 
@@ -491,8 +492,16 @@ class Sim:
         rc = self._ok(self.lib.cem_amd_launch_shape(self.h, C.byref(h), C.byref(e)))
         return h.value, e.value, rc == 1
 
+    def step_shape(self):
+        """(grid rows per workgroup, x planes per workgroup) of the one-pass step kernel;
+        (0, 0) when full steps run the separate H and E updates."""
+        r, p = C.c_int(), C.c_int()
+        self._ok(self.lib.cem_amd_step_shape(self.h, C.byref(r), C.byref(p)))
+        return r.value, p.value
+
     def kernel_stats(self, which):
-        """(total GPU ms, launches) of kernel `which`: 0 H update, 1 E update, 2 gather, 3 DFT, 4 post."""
+        """(total GPU ms, launches) of kernel `which`: 0 H update, 1 E update, 2 gather, 3 DFT, 4 post,
+        5 one-pass step (H and E in one kernel)."""
         t, n = C.c_double(), C.c_long()
         self._ok(self.lib.cem_amd_kernel_stats(self.h, which, C.byref(t), C.byref(n)))
         return t.value, n.value

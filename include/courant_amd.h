@@ -38,8 +38,10 @@ CEM_API int cem_amd_set_profiling(cem_sim *s, int on);
 
 /* Total GPU time (ms) and launch count of one kernel since profiling was
  * switched on: which = 0 H update, 1 E update, 2 probe gather, 3 DFT,
- * 4 post-phase edge physics (Debye, SIBC, sources).  Profiling launches
- * every kernel directly (no graph replay). */
+ * 4 post-phase edge physics (Debye, SIBC, sources), 5 one-pass step (H and E
+ * updates of a step in one kernel; full steps use it in place of 0 and 1 when
+ * cem_amd_step_shape reports rows > 0).  Profiling launches every kernel
+ * directly (no graph replay). */
 CEM_API int cem_amd_kernel_stats(cem_sim *s, int which, double *total_ms, long *count);
 
 /* 2x2 S-matrix renormalisation of nf frequencies on the GPU in FP64, in place:
@@ -63,6 +65,15 @@ CEM_API int cem_amd_s2_renormalize_noalloc(double *S, int nf, const double *z1, 
  * E updates).  Returns 1 once the autotuner (or COURANT_ISEG_H / _E) has fixed
  * it, 0 before, -1 if the simulation has not run on the GPU. */
 CEM_API int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e);
+
+/* Shape of the one-pass step kernel (H and E updates of a step in one sweep,
+ * field sets alternating), which full steps (cem_step, cem_run) use when
+ * COURANT_STEP=onepass is in the environment at set-up: rows = grid rows (y)
+ * owned by one workgroup, planes = x planes it walks.  rows = planes = 0 when
+ * full steps run the separate H and E updates (the default).  The phase calls
+ * cem_mt_H / cem_mt_E always run the separate updates.  Returns 0, or -1 if the
+ * simulation cannot run on the GPU. */
+CEM_API int cem_amd_step_shape(cem_sim *s, int *rows, int *planes);
 
 /* Stream-copy bandwidth of the current GPU in GB/s (read + write bytes), best
  * of `iters` copies of `bytes` between two device buffers; -1 on error. */
