@@ -149,9 +149,9 @@ struct EArgs {
     float *psi[3][2];       // psiE[c][w]
     int has_pml;
     // Debye ADE of the dispersive edges this kernel updates.  J is held over a box
-    // of the grid, [i - bi0][j - bj0][comp][k - 4 bg0][pole], the poles of a point
-    // padded to an even count: a lane reads its points' currents with 8- or 16-byte
-    // loads, and one (i, j) row of the box is one contiguous block
+    // of the grid, [i - bi0][j - bj0][comp][pole][k - 4 bg0]: the lanes of a wave
+    // read one pole of consecutive points as one contiguous run, and one (i, j) row
+    // of the box is one contiguous block
     float *J;
     const float *btab;      // per material row: beta_p / dt for each pole
     int nrow;
@@ -621,24 +621,21 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
         // Debye currents of the group first, every component and pole at once, so
         // that their loads are in flight together with the field loads
         const bool in_box = kCmp && NP > 0 && box_jk && i >= a.bi0 && i < a.bi1;
-        constexpr int NPE = (NP + 1) & ~1;            // poles per point, padded to even
-        constexpr int NJ = NP > 0 ? W * NPE : 2;      // floats of one component per lane
-        constexpr int CH = (W >= 2 && NJ % 4 == 0) ? 4 : 2;
-        float jv[3][NJ];
+        constexpr int NPE = NP > 0 ? NP : 1;          // poles per point
+        float jv[3][W * NPE];                         // [comp][point e * NPE + pole]
         float *Jl = nullptr;
-        long long jc = 0;                             // component stride in the box row
+        long long bkp = 0;                            // pole stride in the box row
         if (in_box) {
-            const long long bkp = 4 * (a.bg1 - a.bg0);
-            jc = bkp * NPE;
-            Jl = a.J + ((long long)(i - a.bi0) * (a.bj1 - a.bj0) + (j - a.bj0)) * 3 * jc + (k0 - 4 * a.bg0) * NPE;
+            bkp = 4 * (a.bg1 - a.bg0);
+            Jl = a.J + ((long long)(i - a.bi0) * (a.bj1 - a.bj0) + (j - a.bj0)) * 3 * NPE * bkp + (k0 - 4 * a.bg0);
 #pragma unroll
             for (int c = 0; c < 3; c++)
 #pragma unroll
-                for (int u = 0; u < NJ; u += CH) {
-                    float t[CH];
-                    ldw<CH>(Jl + c * jc + u, t);
+                for (int pp = 0; pp < NPE; pp++) {
+                    float t[W];
+                    ldw<W>(Jl + (c * NPE + pp) * bkp, t);
 #pragma unroll
-                    for (int v = 0; v < CH; v++) jv[c][u + v] = t[v];
+                    for (int e = 0; e < W; e++) jv[c][e * NPE + pp] = t[e];
                 }
         }
         float hx[W], hy[W], hz[W], hxj[W], hzj[W];
@@ -736,11 +733,11 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     if (row >= 0) debye_point<NP>(en[e], e0[e], cb[c][e], jv[c] + e * NPE, s_bt + row * NP, a.kc, a.km1);
                 }
 #pragma unroll
-                for (int u = 0; u < NJ; u += CH) {
-                    float t[CH];
+                for (int pp = 0; pp < NPE; pp++) {
+                    float t[W];
 #pragma unroll
-                    for (int v = 0; v < CH; v++) t[v] = jv[c][u + v];
-                    stw<CH>(Jl + c * jc + u, t);
+                    for (int e = 0; e < W; e++) t[e] = jv[c][e * NPE + pp];
+                    stw<W>(Jl + (c * NPE + pp) * bkp, t);
                 }
             }
         }
@@ -926,12 +923,11 @@ __global__ void __launch_bounds__(1024) k_step(SArgs f, int iseg) {
         }
         // E-side inputs of the plane at owned points: classes or coefficients, psi_E,
         // Debye currents (box layout of EArgs)
-        constexpr int NPE = (NP + 1) & ~1;
-        constexpr int NJ = NP > 0 ? NPE : 2;
+        constexpr int NPE = NP > 0 ? NP : 1;
         unsigned cls = 0;
-        float ca[3], cb[3], jv[3][NJ];
+        float ca[3], cb[3], jv[3][NPE];
         float *Jl = nullptr;
-        long long jc = 0;
+        long long bkp = 0;
         bool in_box = false;
         int sx = -1;
         float py0 = 0.f, py2 = 0.f, px1 = 0.f, px2 = 0.f, pz0 = 0.f, pz1 = 0.f, bx = 0.f, cx = 0.f;
@@ -955,17 +951,12 @@ __global__ void __launch_bounds__(1024) k_step(SArgs f, int iseg) {
             }
             in_box = kCmp && box_jk && i >= a.bi0 && i < a.bi1;
             if (in_box) {
-                jc = 4LL * (a.bg1 - a.bg0) * NPE;
-                Jl = a.J + ((long long)(i - a.bi0) * (a.bj1 - a.bj0) + (j - a.bj0)) * 3 * jc + (k - 4 * a.bg0) * NPE;
+                bkp = 4LL * (a.bg1 - a.bg0);
+                Jl = a.J + ((long long)(i - a.bi0) * (a.bj1 - a.bj0) + (j - a.bj0)) * 3 * NPE * bkp + (k - 4 * a.bg0);
 #pragma unroll
                 for (int c = 0; c < 3; c++)
 #pragma unroll
-                    for (int u = 0; u < NJ; u += 2) {
-                        float t2[2];
-                        ldw<2>(Jl + c * jc + u, t2);
-                        jv[c][u] = t2[0];
-                        jv[c][u + 1] = t2[1];
-                    }
+                    for (int pp = 0; pp < NPE; pp++) jv[c][pp] = Jl[(c * NPE + pp) * bkp];
             }
             if (sye >= 0) {
                 oy = ((unsigned)sye * (g.nx + 1) + i) * (unsigned)kp + k;
@@ -1041,10 +1032,7 @@ __global__ void __launch_bounds__(1024) k_step(SArgs f, int iseg) {
                         const int row = s_row[c * 256 + ((cls >> (8 * c)) & 255u)];
                         if (row >= 0) debye_point<NP>(en, e0, cb[c], jv[c], s_bt + row * NP, a.kc, a.km1);
 #pragma unroll
-                        for (int u = 0; u < NJ; u += 2) {
-                            const float t2[2] = {jv[c][u], jv[c][u + 1]};
-                            stw<2>(Jl + c * jc + u, t2);
-                        }
+                        for (int pp = 0; pp < NPE; pp++) Jl[(c * NPE + pp) * bkp] = jv[c][pp];
                     }
                 }
                 if (sye >= 0) {
@@ -1581,10 +1569,9 @@ struct Device {
     DBuf<float> btab;            // fused: beta_p / dt per material row
     int nrow = 0;
     int bi0 = 0, bi1 = 0, bj0 = 0, bj1 = 0, bg0 = 0, bg1 = 0;   // J box (k in groups)
-    long long bkp = 0;           // k points of a J box row
-    int npe = 0;                 // poles per point in the J box (npole padded to even)
+    long long bkp = 0;           // k points of a J box row (the pole stride)
     DBuf<long long> d_jb;        // fused: J box index of each listed edge
-    std::vector<long long> jbox_of;   // fused: J box index of every dispersive edge (reference order)
+    std::vector<long long> jbox_of;   // fused: J box index of pole 0 of every dispersive edge (reference order)
     DBuf<float> axis1d;          // hs[3], es[3], pml b/c for H and E per axis
     DBuf<int> slots;             // CPML slot tables, then the z-layout group tables
     std::vector<int> slots_host;
@@ -2128,15 +2115,14 @@ int upload_plan(cem_sim *s, Device *D) {
             D->bg0 = bk0 / 4;
             D->bg1 = bk1 / 4;
             D->bkp = bk1 - bk0;
-            D->npe = (np + 1) & ~1;
             D->jbox_of.resize(nd);
         }
         for (int t = 0; t < nd; t++) {
             const size_t q = s->d_idx[t];
             if (fused) {
                 const int i = (int)(q / s->si), j = (int)(q % s->si / s->sj), k = (int)(q % s->sj);
-                D->jbox_of[t] = ((((long long)(i - bi0) * (bj1 - bj0) + (j - bj0)) * 3 + s->d_comp[t]) * D->bkp +
-                                 (k - bk0)) * D->npe;
+                D->jbox_of[t] = (((long long)(i - bi0) * (bj1 - bj0) + (j - bj0)) * 3 + s->d_comp[t]) * np * D->bkp +
+                                (k - bk0);
             }
             if (fused && !linked[t]) continue;
             D->listed_of[t] = (int)lidx.size();
@@ -2158,7 +2144,7 @@ int upload_plan(cem_sim *s, Device *D) {
             if (D->btab.alloc(bt.size(), D->st) || D->btab.upload(bt.data(), bt.size(), st)) return -1;
         }
         if (D->Eold.alloc(lidx.size(), D->st) || D->Eold.zero(st)) return -1;
-        const size_t nj = fused ? (size_t)(bi1 - bi0) * (bj1 - bj0) * 3 * D->bkp * D->npe : (size_t)np * nd;
+        const size_t nj = fused ? (size_t)(bi1 - bi0) * (bj1 - bj0) * 3 * D->bkp * np : (size_t)np * nd;
         if (D->J.alloc(nj, D->st) || D->J.zero(st)) return -1;
         PostParams pp{};
         pp.ndisp = D->nlisted;
@@ -2514,7 +2500,7 @@ PostArgs post_args(cem_sim *s, Device *D, int par) {
     a.sb_src = D->has_sbsrc ? D->sb_src.p : nullptr;
     a.J = D->J.p;
     a.d_jb = D->fused ? D->d_jb.p : nullptr;
-    a.jstride = D->fused ? 1 : (long long)s->ndisp;
+    a.jstride = D->fused ? D->bkp : (long long)s->ndisp;
     a.Eold = D->Eold.p;
     a.sb_idx = D->sb_idx.p;
     a.face = D->face.p;
@@ -3100,7 +3086,7 @@ void cem::dev_state_download(cem_sim *s, int which, int comp, int w, double *dst
         }
         const std::vector<float> t = download_raw(D, D->J.p, D->J.n);
         for (size_t p = 0; p < np; p++)
-            for (size_t e = 0; e < nd; e++) dst[p * nd + e] = t[D->jbox_of[e] + p];
+            for (size_t e = 0; e < nd; e++) dst[p * nd + e] = t[D->jbox_of[e] + p * D->bkp];
         return;
     }
     if (which == 3) {       // Debye E_old: E of each dispersive edge before the next E update
@@ -3151,7 +3137,7 @@ void cem::dev_state_upload(cem_sim *s, int which, int comp, int w, const double 
         }
         std::vector<float> t(D->J.n, 0.f);
         for (size_t p = 0; p < np; p++)
-            for (size_t e = 0; e < nd; e++) t[D->jbox_of[e] + p] = (float)src[p * nd + e];
+            for (size_t e = 0; e < nd; e++) t[D->jbox_of[e] + p * D->bkp] = (float)src[p * nd + e];
         upload_raw(D, D->J.p, t);
         return;
     }
