@@ -7,6 +7,13 @@
 //   node harness.js line <addon>      a small lumped-port line stepped on the GPU
 //   node harness.js workers <addon>   the export check in two worker threads while the
 //                                     main thread holds a plan and data in its own arena
+//   node harness.js board <addon> <trace.json>
+//                                     the command-line board run (web/tools/run-board.ts:53-85)
+//                                     on a mode-port board: each excitation's set-up calls
+//                                     replayed from the trace, then stepped in chunks of 500
+//                                     to the energy-decay stop, the two ports' waves, the
+//                                     2 x 2 S matrix, its 50-ohm renormalisation per frequency
+//                                     and the Touchstone text
 //
 // Prints one JSON object.
 'use strict';
@@ -88,6 +95,118 @@ function exportsReport() {
   return out;
 }
 
+
+// Replays recorded courant.h calls (tests/napi_trace.py): scalars as given, simulation
+// handles by creation order, arrays and structs copied into arena blocks, output
+// buffers allocated with their initial bytes and read back after the call.
+function replay(calls) {
+  const sims = [], results = [], outputs = [];
+  for (const [name, args, res] of calls) {
+    const blocks = [], outs = [];
+    const vals = args.map((a) => {
+      if ('i' in a) return a.i;
+      if ('d' in a) return a.d;
+      if ('sim' in a) return sims[a.sim];
+      if ('null' in a) return 0;
+      const raw = Buffer.from('in' in a ? a.in : a.init, 'base64');
+      const p = ex.malloc(Math.max(raw.length, 8));
+      if (!p) throw new Error('arena out of memory');
+      new Uint8Array(ex.memory.buffer, p, raw.length).set(raw);
+      blocks.push(p);
+      if ('out' in a) outs.push([p, a.out]);
+      return p;
+    });
+    const r = ex[name](...vals);
+    if (name === 'cem_create') {
+      if (!r && r !== 0) throw new Error(`cem_create: ${lastError()}`);
+      sims.push(r);
+      results.push({ sim: sims.length - 1 });
+    } else {
+      results.push(typeof r === 'number' ? r : null);
+    }
+    outputs.push(outs.map(([p, n]) => Buffer.from(new Uint8Array(ex.memory.buffer, p, n)).toString('base64')));
+    blocks.forEach((p) => ex.free(p));
+  }
+  return { sims, results, outputs };
+}
+
+const cdiv = (x, y) => {   // run-board.ts's div
+  const d = y.re * y.re + y.im * y.im;
+  return { re: (x.re * y.re + x.im * y.im) / d, im: (x.im * y.re - x.re * y.im) / d };
+};
+
+function modePortWaves(s, port, nf) {   // courant.ts modePortWaves
+  const pa = ex.malloc(16 * nf), pb = ex.malloc(16 * nf), pz = ex.malloc(8 * nf), pbeta = ex.malloc(8 * nf);
+  check(ex.cem_mode_port_waves(s, port, pa, pb, pz, pbeta), 'mode_port_waves');
+  const a = readF64(pa, 2 * nf), b = readF64(pb, 2 * nf);
+  const w = {
+    a: Array.from({ length: nf }, (_, i) => ({ re: a[2 * i], im: a[2 * i + 1] })),
+    b: Array.from({ length: nf }, (_, i) => ({ re: b[2 * i], im: b[2 * i + 1] })),
+    z0: readF64(pz, nf),
+  };
+  [pa, pb, pz, pbeta].forEach((p) => ex.free(p));
+  return w;
+}
+
+function renormalize2(S, z1, z2, zref) {   // courant.ts renormalize2
+  const p = allocF64(S.flatMap((c) => [c.re, c.im]));
+  ex.cem_s2_renormalize(p, z1, z2, zref);
+  const v = readF64(p, 8);
+  ex.free(p);
+  return [0, 1, 2, 3].map((k) => ({ re: v[2 * k], im: v[2 * k + 1] }));
+}
+
+function jsTouchstone(f, s11, s21, s12, s22, title, version, date) {   // web/src/app/touchstone.ts:7-23
+  const lines = [
+    `! Courant EM ${version} — ${title}`,
+    `! ${date}; reference 50; both ports excited`,
+    '# HZ S RI R 50',
+  ];
+  f.forEach((fk, i) => {
+    const a = s11[i], b = s21[i], c = s12[i], d = s22[i];
+    lines.push([fk, a.re, a.im, b.re, b.im, c.re, c.im, d.re, d.im].map((v, k) => (k ? v.toExponential(9) : v.toExponential(6))).join(' '));
+  });
+  return lines.join('\n') + '\n';
+}
+
+function runBoard(trace) {
+  const waves = [], runs = [];
+  let steps = 0;
+  for (const run of trace.runs) {
+    const rep = replay(run.calls);
+    const s = rep.sims[0];
+    const dt = ex.cem_dt(s), nf = ex.cem_nfreq(s);
+    let peak = 0;
+    const energies = [];
+    for (;;) {   // run-board.ts:64-72
+      check(ex.cem_step(s, 500), 'step');
+      const W = ex.cem_energy(s);
+      energies.push(W);
+      peak = Math.max(peak, W);
+      const n = ex.cem_step_count(s);
+      if ((n * dt > trace.tstop && W < trace.decay * peak) || n * dt > trace.tmax) break;
+    }
+    waves.push({ w1: modePortWaves(s, run.ports[0], nf), w2: modePortWaves(s, run.ports[1], nf) });
+    steps += ex.cem_step_count(s);
+    runs.push({ results: rep.results, outputs: rep.outputs, energies, steps: ex.cem_step_count(s) });
+    ex.cem_destroy(s);
+  }
+  const { w1, w2 } = waves[0];
+  const s11 = w1.b.map((b, i) => cdiv(b, w1.a[i])), s21 = w2.b.map((b, i) => cdiv(b, w1.a[i]));
+  const s12 = waves[1].w1.b.map((b, i) => cdiv(b, waves[1].w2.a[i]));
+  const s22 = waves[1].w2.b.map((b, i) => cdiv(b, waves[1].w2.a[i]));
+  const r50 = s11.map((a, i) => renormalize2([a, s12[i], s21[i], s22[i]], w1.z0[i], w2.z0[i], 50));
+  const flat = (v) => v.flatMap((c) => [c.re, c.im]);
+  const version = readString(ex.cem_version());
+  return {
+    runs, steps, z1: w1.z0, z2: w2.z0,
+    s11: flat(s11), s21: flat(s21), s12: flat(s12), s22: flat(s22),
+    s50: r50.map((r) => flat(r)),
+    touchstone: jsTouchstone(trace.f, r50.map((r) => r[0]), r50.map((r) => r[2]), r50.map((r) => r[1]),
+      r50.map((r) => r[3]), trace.title, version, new Date().toISOString()),
+  };
+}
+
 function emit(out) { process.stdout.write(JSON.stringify(out) + '\n'); }
 
 let out = {};
@@ -140,6 +259,8 @@ if (!isMainThread) {
   // step-then-read through the same held view
   out.ez_probe = ez[ex.cem_index(s, 30, 20, 9)];
   ex.cem_destroy(s);
+} else if (mode === 'board') {
+  out = runBoard(JSON.parse(require('fs').readFileSync(process.argv[4], 'utf8')));
 } else {
   throw new Error(`unknown mode ${mode}`);
 }

@@ -12,6 +12,13 @@ tests/napi/harness.js the way that wrapper drives the wasm module.
   * GPU: a lumped-port line stepped in chunks of 500 through the addon, with a field
     written through a held view before stepping and read through it after, gives
     bit-identical spectra, energy and field values to the same calls made through ctypes
+  * GPU: the command-line board run (web/tools/run-board.ts:53-85) through the addon on the
+    lossy mode-port GCPW: both excitations (their set-up calls recorded from the Python
+    scene builder and replayed), stepped in chunks of 500 to the energy-decay stop, the
+    mode-port waves of both ports, the 2 x 2 S matrix, its per-frequency 50-ohm
+    renormalisation (cem_s2_renormalize) and the Touchstone text: every return value,
+    energy, wave ratio and renormalised entry bit-identical to the ctypes path, and the
+    Touchstone text identical to the Python writer's apart from the date line
 """
 import json
 import os
@@ -104,3 +111,77 @@ def test_addon_line_matches_the_ctypes_path(cem, gpu_available):
         assert np.array_equal(np.array(out[key]), interleave(got)), key
     assert out["ez_probe"] == s.field(2).reshape(-1)[s.index(30, 20, 9)]
     assert np.max(np.abs(out["v2"])) > 0
+
+
+MODE_MINI = dict(hmin=40e-6, hmax=100e-6, dx=100e-6, length=1.5e-3, feed=1.2e-3, mon=0.6e-3, yair=0.3e-3,
+                 zair=0.4e-3, ports="mode", lossy=True, tcu=35e-6, mask=25e-6, dc=False, df=4e9)
+
+
+def cdiv(x, y):
+    """run-board.ts's complex division, elementwise on complex arrays."""
+    d = y.real * y.real + y.imag * y.imag
+    return (x.real * y.real + x.imag * y.imag) / d + 1j * ((x.imag * y.real - x.real * y.imag) / d)
+
+
+@needs_addon
+@pytest.mark.gpu
+def test_addon_board_run_matches_the_ctypes_path(cem, gpu_available, tmp_path):
+    from courant_amd import scenes as S
+    from courant_amd import touchstone
+    from napi_trace import Recorder
+    ver = subprocess.run([NODE, "--version"], stdout=subprocess.PIPE, text=True).stdout.strip()
+    print(f"\n[gpu case] node addon ({ver}): board run, lossy mode-port GCPW, both excitations", flush=True)
+    rec_cem = L.Courant()
+    rec = Recorder(rec_cem.lib)
+    rec_cem.lib = rec
+    decay = 1e-5
+    runs, waves, energies, steps = [], [], [], []
+    for excite in (1, 2):
+        rec.calls, rec.handles, rec.active = [], {}, True
+        s, info = S.gcpw(rec_cem, **dict(MODE_MINI, excite=excite))
+        rec.active = False
+        tstop, tmax = 9 * info["tau"], 4000 * s.dt
+        peak, ws = 0.0, []
+        while True:
+            s.lib.cem_step(s.h, 500)
+            w = s.lib.cem_energy(s.h)
+            ws.append(w)
+            peak = max(peak, w)
+            n = s.lib.cem_step_count(s.h)
+            if (n * s.dt > tstop and w < decay * peak) or n * s.dt > tmax:
+                break
+        p1, p2 = info["ports"]
+        waves.append((s.mode_port_waves(p1), s.mode_port_waves(p2)))
+        energies.append(ws)
+        steps.append(n)
+        runs.append({"calls": rec.calls, "ports": [p1, p2]})
+        s.close()
+    f = [float(v) for v in info["freqs"]]
+    title = "lossy mode-port GCPW, planes at the port references"
+    trace = {"runs": runs, "tstop": tstop, "tmax": tmax, "decay": decay, "f": f, "title": title}
+    path = tmp_path / "trace.json"
+    path.write_text(json.dumps(trace))
+    r = subprocess.run([NODE, HARNESS, "board", ADDON, str(path)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    for k, run in enumerate(runs):
+        got = out["runs"][k]
+        assert got["results"] == [c[2] for c in run["calls"]], k
+        assert got["outputs"] == [c[3] for c in run["calls"]], k
+        assert got["energies"] == energies[k] and got["steps"] == steps[k], k
+    (u1, u2), (v1, v2) = waves
+    Sm = {"s11": cdiv(u1["b"], u1["a"]), "s21": cdiv(u2["b"], u1["a"]),
+          "s12": cdiv(v1["b"], v2["a"]), "s22": cdiv(v2["b"], v2["a"])}
+    for key, val in Sm.items():
+        assert np.array_equal(np.array(out[key]), interleave(val)), key
+    assert out["z1"] == u1["z0"].tolist() and out["z2"] == u2["z0"].tolist()
+    s50 = np.array([cem.renormalize2([Sm["s11"][i], Sm["s12"][i], Sm["s21"][i], Sm["s22"][i]],
+                                     u1["z0"][i], u2["z0"][i], 50.0) for i in range(len(f))])
+    assert np.array_equal(np.array(out["s50"]), np.stack([s50.real, s50.imag], -1).reshape(len(f), 8))
+    py = touchstone.write_s2p(f, s50.reshape(-1, 2, 2), title, cem.version(), full=True, renormalised=True)
+    js, pl = out["touchstone"].split("\n"), py.split("\n")
+    assert len(js) == len(pl) == len(f) + 4
+    assert js[0] == pl[0] and js[2:] == pl[2:]
+    assert js[1].endswith("; reference 50; both ports excited")
+    assert steps[0] > 500 and np.all(np.abs(s50) < 1.5)
