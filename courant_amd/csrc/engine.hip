@@ -2694,6 +2694,7 @@ int run_chunk(cem_sim *s, Device *D) {
     if (!D->gexec || key != D->gkey) {
         drop_graph(D);
         hipGraph_t graph = nullptr;
+        CaptureLock capture;   // no stream-ordered allocation in any thread during the capture
         if (hipStreamBeginCapture(D->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             D->use_graph = false;
             return -1;

@@ -790,6 +790,14 @@ __global__ void __launch_bounds__(kInvThreads) k_inverse(InvArgs a) {
     }
 }
 
+// the last failed device allocation of this thread: its HIP error and size, for the
+// solver's error message
+thread_local char t_alloc_err[160];
+
+inline void alloc_failure(hipError_t e, size_t bytes) {
+    std::snprintf(t_alloc_err, sizeof t_alloc_err, "%s (%zu bytes)", hipGetErrorString(e), bytes);
+}
+
 template <typename T>
 struct DMem {
     T *p = nullptr;
@@ -799,8 +807,10 @@ struct DMem {
         if (count <= n) return true;
         release();
         owner = st;
-        if (dev_alloc((void **)&p, count * sizeof(T), st) != hipSuccess) {
+        const hipError_t e = dev_alloc((void **)&p, count * sizeof(T), st);
+        if (e != hipSuccess) {
             p = nullptr;
+            alloc_failure(e, count * sizeof(T));
             return false;
         }
         n = count;
@@ -823,6 +833,7 @@ struct HMem {
     bool get(size_t count) {
         if (count <= n) return true;
         release();
+        std::shared_lock<std::shared_mutex> lock(capture_mutex());
         if (hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
             p = nullptr;
             return false;
@@ -831,7 +842,10 @@ struct HMem {
         return true;
     }
     void release() {
-        if (p) (void)hipHostFree(p);
+        if (p) {
+            std::shared_lock<std::shared_mutex> lock(capture_mutex());
+            (void)hipHostFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -1007,7 +1021,9 @@ int mode_gpu_solve_batch(ModeGpu **ctx, const ModeGpuProblem *P, int nbq, std::c
         !M.xo.get(B * n, st) || !M.y.get(B * n, st) || !M.hw.get(B * g.shw, st) || !M.hu.get(B * g.shu, st) ||
         !M.hv.get(B * g.shv, st) || !M.ew.get(B * g.sew, st) || !M.pec.get(g.nu + g.nv, st) ||
         !M.ipiv.get(B * n, st) || !M.info.get(B, st) || !M.out.get(10 * B, st) || !M.prm.get(4 * B, st)) {
-        set_error("GPU mode solver: device out of memory");
+        char m[240];
+        std::snprintf(m, sizeof m, "GPU mode solver: device allocation failed: %s", t_alloc_err);
+        set_error(m);
         return -1;
     }
     // the pinned buffers are rewritten below: no copy of an earlier (failed) call in flight
@@ -1116,10 +1132,9 @@ int mode_gpu_solve_batch(ModeGpu **ctx, const ModeGpuProblem *P, int nbq, std::c
     } else {
         const void *key[3] = {M.band.p, M.ipiv.p, M.info.p};
         if (!M.lu || M.lu_n != n || M.lu_bw != kl || M.lu_b != nbq || std::memcmp(key, M.lu_key, sizeof key) != 0) {
-            // one capture at a time in the process (once per context; the sweeps' clones
-            // reach their first solve together)
-            static std::mutex capture_mu;
-            std::lock_guard<std::mutex> lock(capture_mu);
+            // one capture at a time in the process, and no allocation in any thread during
+            // it (once per context; the sweeps' clones reach their first solve together)
+            CaptureLock lock;
             if (M.lu) (void)hipGraphExecDestroy(M.lu);
             M.lu = nullptr;
             hipGraph_t graph = nullptr;
