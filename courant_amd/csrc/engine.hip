@@ -310,57 +310,68 @@ __device__ __forceinline__ void zerow(float (&x)[W]) {
     for (int e = 0; e < W; e++) x[e] = 0.f;
 }
 
-// One H point (sim.c:579-606) with the CPML recursions of the slabs that hold it
-// (sim.c:680-754), from the point's six E differences: each component only where
-// the reference updates it, with the reference's expression in its order.  Shared
-// by the H sweep and the one-pass step kernel, so both give the same bits.
+// One H component at one point (sim.c:579-606) with the CPML recursions of the slabs
+// that hold it (sim.c:680-754), from the point's E differences, with the reference's
+// expression in its order.  Every kernel that updates H calls these, so all of them
+// give the same bits.
+__device__ __forceinline__ void hx_point(float &hxv, float dz_ey, float dy_ez, float hy, float hz, int sy, float by,
+                                         float cy, bool inz, float bz, float cz, float dth, float &py0, float &pz0) {
+    float h = hxv - (dy_ez * hy - dz_ey * hz);
+    if (sy >= 0) {
+        const float v = by * py0 + cy * dy_ez;
+        py0 = v;
+        h -= dth * v;
+    }
+    if (inz) {
+        const float v = bz * pz0 + cz * dz_ey;
+        pz0 = v;
+        h += dth * v;
+    }
+    hxv = h;
+}
+
+__device__ __forceinline__ void hy_point(float &hyv, float dz_ex, float dx_ez, float hx, float hz, int sx, float bx,
+                                         float cx, bool inz, float bz, float cz, float dth, float &px1, float &pz1) {
+    float h = hyv - (dz_ex * hz - dx_ez * hx);
+    if (sx >= 0) {
+        const float v = bx * px1 + cx * dx_ez;
+        px1 = v;
+        h += dth * v;
+    }
+    if (inz) {
+        const float v = bz * pz1 + cz * dz_ex;
+        pz1 = v;
+        h -= dth * v;
+    }
+    hyv = h;
+}
+
+__device__ __forceinline__ void hz_point(float &hzv, float dx_ey, float dy_ex, float hx, float hy, int sx, float bx,
+                                         float cx, int sy, float by, float cy, float dth, float &px2, float &py2) {
+    float h = hzv - (dx_ey * hx - dy_ex * hy);
+    if (sx >= 0) {
+        const float v = bx * px2 + cx * dx_ey;
+        px2 = v;
+        h -= dth * v;
+    }
+    if (sy >= 0) {
+        const float v = by * py2 + cy * dy_ex;
+        py2 = v;
+        h += dth * v;
+    }
+    hzv = h;
+}
+
+// One H point: each component only where the reference updates it (Hx: j < ny, k < nz;
+// Hy: i < nx, k < nz; Hz: i < nx, j < ny).
 __device__ __forceinline__ void h_point(float &hxv, float &hyv, float &hzv, float dz_ey, float dy_ez, float dz_ex,
                                         float dy_ex, float dx_ez, float dx_ey, float hx, float hy, float hz, bool jin,
                                         bool kin, bool iin, int sy, float by, float cy, int sx, float bx, float cx,
                                         bool inz, float bz, float cz, float dth, float &py0, float &py2, float &px1,
                                         float &px2, float &pz0, float &pz1) {
-    if (jin && kin) {   // Hx: all i
-        float h = hxv - (dy_ez * hy - dz_ey * hz);
-        if (sy >= 0) {
-            const float v = by * py0 + cy * dy_ez;
-            py0 = v;
-            h -= dth * v;
-        }
-        if (inz) {
-            const float v = bz * pz0 + cz * dz_ey;
-            pz0 = v;
-            h += dth * v;
-        }
-        hxv = h;
-    }
-    if (iin && kin) {   // Hy: j <= ny
-        float h = hyv - (dz_ex * hz - dx_ez * hx);
-        if (sx >= 0) {
-            const float v = bx * px1 + cx * dx_ez;
-            px1 = v;
-            h += dth * v;
-        }
-        if (inz) {
-            const float v = bz * pz1 + cz * dz_ex;
-            pz1 = v;
-            h -= dth * v;
-        }
-        hyv = h;
-    }
-    if (iin && jin) {   // Hz: k <= nz
-        float h = hzv - (dx_ey * hx - dy_ex * hy);
-        if (sx >= 0) {
-            const float v = bx * px2 + cx * dx_ey;
-            px2 = v;
-            h -= dth * v;
-        }
-        if (sy >= 0) {
-            const float v = by * py2 + cy * dy_ex;
-            py2 = v;
-            h += dth * v;
-        }
-        hzv = h;
-    }
+    if (jin && kin) hx_point(hxv, dz_ey, dy_ez, hy, hz, sy, by, cy, inz, bz, cz, dth, py0, pz0);
+    if (iin && kin) hy_point(hyv, dz_ex, dx_ez, hx, hz, sx, bx, cx, inz, bz, cz, dth, px1, pz1);
+    if (iin && jin) hz_point(hzv, dx_ey, dy_ex, hx, hy, sx, bx, cx, sy, by, cy, dth, px2, py2);
 }
 
 // H update (sim.c:579-606) fused with the CPML psi_H recursions (sim.c:680-754).
@@ -1060,6 +1071,281 @@ __global__ void __launch_bounds__(1024) k_step(SArgs f, int iseg) {
     }
 }
 
+// One-pass step without barriers.  Each lane owns one point (j, k) of the (j, k)
+// plane, walks an x segment, and at every plane computes the new H it needs itself:
+// H at (i, j, k) in full (stored), and Hx, Hz at (i, j - 1, k) again (the row below
+// is another lane's; recomputing its two components from the old fields costs VALU
+// work but no memory traffic or synchronisation).  The H of (i, j, k - 1) comes from
+// the lane below by a cross-lane shift: a wavefront's lanes hold 63 consecutive
+// points of the flattened (j, k) plane after one halo lane, which computes the H of
+// the point before them and owns nothing.  The H of plane i - 1 is carried in
+// registers; a segment starting inside x computes that plane's Hy, Hz first.  Old E,
+// H and psi_H are read from one buffer set and the new values written to the other
+// (so no lane sees another's writes); psi_E and the Debye currents are updated in
+// place by their owner.  Every value is computed by hx/hy/hz_point, e_point and
+// debye_point, the functions of the two sweeps, so all paths give the same bits.
+// Measured on the 38.8 M-point four-layer board: 1.02 ms a step against 0.93 ms for
+// the two sweeps; it moves the one-pass step's bytes but issues ~1.7x the sweeps'
+// instructions per point (the recomputed row, 64-bit address arithmetic, scalar
+// register spills of its large argument block) at 127 VGPRs (4 waves per SIMD).
+template <int NP, bool kCmp>
+__global__ void __launch_bounds__(kBlock) k_step_rows(SArgs f, int iseg, int npblk) {
+    __shared__ float2 s_tab[kCmp ? 768 : 1];
+    __shared__ int s_row[(kCmp && NP > 0) ? 768 : 1];
+    __shared__ float s_bt[(kCmp && NP > 0) ? kDebyeRows * NP : 1];
+    const EArgs &a = f.e;
+    if (kCmp) {
+        for (int t = threadIdx.x; t < 768; t += kBlock) {
+            s_tab[t] = a.ctab[t];
+            if (NP > 0) s_row[t] = a.crow[t];
+        }
+        if (NP > 0)
+            for (int t = threadIdx.x; t < a.nrow * NP; t += kBlock) s_bt[t] = a.btab[t];
+        __syncthreads();
+    }
+    const Grid g = f.h.g;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int pblk = tile % npblk, seg = tile / npblk;
+    const int lane = threadIdx.x & 63;
+    const int P = (g.ny + 1) * g.kp;
+    const int p0 = (pblk * (kBlock / 64) + (int)(threadIdx.x >> 6)) * 63;   // first owned point of the wave
+    if (p0 >= P) return;                                                   // whole wavefronts only
+    const int p = p0 + lane - 1;
+    const bool valid = p >= 0 && p < P;
+    const bool own = valid && lane >= 1;
+    const int pc = valid ? p : 0;
+    const int j = pc / g.kp, k = pc - j * g.kp;
+    const int ib = seg * iseg, ie = min(g.nx + 1, ib + iseg);
+    const float *__restrict__ Exo = f.h.E;
+    const float *__restrict__ Eyo = f.h.E + g.N;
+    const float *__restrict__ Ezo = f.h.E + 2 * g.N;
+    const float *__restrict__ Hxo = f.h.H;
+    const float *__restrict__ Hyo = f.h.H + g.N;
+    const float *__restrict__ Hzo = f.h.H + 2 * g.N;
+    const Pml &ph = f.h.pml;
+    const Pml &pe = a.pml;
+    // rows j (owned) and j - 1 (Hx, Hz recomputed); column k
+    const bool jin = j < g.ny, kin = k < g.nz, jm_ok = j >= 1;
+    const int jm = jm_ok ? j - 1 : 0;
+    const float hy_s = jin ? f.h.hs[1][j] : 0.f, hy_m = jm_ok ? f.h.hs[1][jm] : 0.f;
+    const float hz_s = kin ? f.h.hs[2][k] : 0.f;
+    const int syh = f.h.has_pml && jin ? ph.slot[1][j] : -1;
+    const float byh = syh >= 0 ? ph.b[1][j] : 0.f, cyh = syh >= 0 ? ph.c[1][j] : 0.f;
+    const int sym = f.h.has_pml && jm_ok ? ph.slot[1][jm] : -1;
+    const float bym = sym >= 0 ? ph.b[1][jm] : 0.f, cym = sym >= 0 ? ph.c[1][jm] : 0.f;
+    const int zoh = f.h.has_pml ? ph.zo[k / 4] : -1;
+    const bool inzh = zoh >= 0 && kin && ph.slot[2][k] >= 0;
+    const float bzh = inzh ? ph.b[2][k] : 0.f, czh = inzh ? ph.c[2][k] : 0.f;
+    const bool jb = j >= 1 && j < g.ny, kb = k >= 1 && k < g.nz, kz = k < g.nz;
+    const float ey_s = a.es[1][j];
+    const float ez_s = k <= g.nz ? a.es[2][k] : 0.f;
+    const int sye = a.has_pml ? pe.slot[1][j] : -1;
+    const float bye = sye >= 0 ? pe.b[1][j] : 0.f, cye = sye >= 0 ? pe.c[1][j] : 0.f;
+    const int zoe = a.has_pml ? pe.zo[k / 4] : -1;
+    const bool inze = zoe >= 0 && kz && pe.slot[2][k] >= 0;
+    const float bze = inze ? pe.b[2][k] : 0.f, cze = inze ? pe.c[2][k] : 0.f;
+    const bool box_jk = NP > 0 && own && j >= a.bj0 && j < a.bj1 && k >= 4 * a.bg0 && k < 4 * a.bg1;
+    // 32-bit element offsets (3N < 2^32: set-up checks)
+    const unsigned si = (unsigned)g.si, sj = (unsigned)g.sj, N = (unsigned)g.N;
+    const unsigned kn = k + 1 < g.kp ? 1u : 0u;          // the k + 1 neighbour (itself at a row's end)
+    const unsigned dm = jm_ok ? sj : 0u;                  // row j - 1 (row j itself at j = 0, unused)
+    const unsigned dp = jin ? sj : 0u;                    // row j + 1 (row j itself at j = ny, unused)
+    unsigned q = (unsigned)ib * si + (unsigned)j * sj + k;
+    // H of plane ib - 1 (Hy, Hz) at the owned point, for the x differences of E at ib
+    float hyp = 0.f, hzp = 0.f;
+    if (own && ib >= 1) {
+        const unsigned qm = q - si;
+        const int im = ib - 1;
+        const float exm = Exo[qm], eym = Eyo[qm], ezm = Ezo[qm];
+        const float exk = Exo[qm + kn], exj = Exo[qm + dp];
+        const float eyi = Eyo[q], ezi = Ezo[q];
+        const int sx = f.h.has_pml ? ph.slot[0][im] : -1;
+        const float bx = sx >= 0 ? ph.b[0][im] : 0.f, cx = sx >= 0 ? ph.c[0][im] : 0.f;
+        const float hx_s = f.h.hs[0][im];
+        hyp = Hyo[qm];
+        hzp = Hzo[qm];
+        float px1 = 0.f, px2 = 0.f, pz1 = 0.f, py2 = 0.f;
+        if (syh >= 0) py2 = f.h.psi[2][1][((unsigned)syh * (g.nx + 1) + im) * (unsigned)g.kp + k];
+        if (sx >= 0) {
+            const unsigned ox = (unsigned)sx * si + (unsigned)j * sj + k;
+            px1 = f.h.psi[1][1][ox];
+            px2 = f.h.psi[2][0][ox];
+        }
+        if (zoh >= 0) pz1 = f.h.psi[1][0][(((unsigned)im * (g.ny + 1) + j) * ph.zg + zoh) * 4 + (k & 3)];
+        if (kin) hy_point(hyp, exk - exm, ezi - ezm, hx_s, hz_s, sx, bx, cx, inzh, bzh, czh, f.h.dth, px1, pz1);
+        if (jin) hz_point(hzp, eyi - eym, exj - exm, hx_s, hy_s, sx, bx, cx, syh, byh, cyh, f.h.dth, px2, py2);
+    }
+    // E of plane ib carried into the loop: Ey, Ez of row j, Ey of row j - 1
+    float ey = Eyo[q], ez = Ezo[q], eym = Eyo[q - dm];
+    for (int i = ib; i < ie; i++, q += si) {
+        const bool iin = i < g.nx;
+        const unsigned qm = q - dm;
+        // Debye currents first, so that their loads are in flight with the field loads
+        const bool in_box = kCmp && NP > 0 && box_jk && iin && i >= a.bi0 && i < a.bi1;
+        constexpr int NPE = NP > 0 ? NP : 1;
+        float jv[3][NPE];
+        float *Jl = nullptr;
+        long long bkp = 0;
+        if (in_box) {
+            bkp = 4 * (a.bg1 - a.bg0);
+            Jl = a.J + ((long long)(i - a.bi0) * (a.bj1 - a.bj0) + (j - a.bj0)) * 3 * NPE * bkp + (k - 4 * a.bg0);
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int pp = 0; pp < NPE; pp++) jv[c][pp] = Jl[(c * NPE + pp) * bkp];
+        }
+        // old E of planes i (rows j - 1, j, j + 1; k + 1) and i + 1 (rows j - 1, j)
+        const float ex = Exo[q], exk = Exo[q + kn], eyk = Eyo[q + kn];
+        const float exm = Exo[qm], ezm = Ezo[qm], eykm = Eyo[qm + kn];
+        const float exj = Exo[q + dp], ezj = Ezo[q + dp];
+        float eyi = 0.f, ezi = 0.f, eyim = 0.f;
+        if (iin) {
+            eyi = Eyo[q + si];
+            ezi = Ezo[q + si];
+            eyim = Eyo[qm + si];
+        }
+        // old H of rows j and j - 1
+        float hxv = Hxo[q], hyv = Hyo[q], hzv = Hzo[q];
+        float hxm = Hxo[qm], hzm = Hzo[qm];
+        const float hx_s = iin ? f.h.hs[0][i] : 0.f;
+        const int sxh = f.h.has_pml && iin ? ph.slot[0][i] : -1;
+        const float bxh = sxh >= 0 ? ph.b[0][i] : 0.f, cxh = sxh >= 0 ? ph.c[0][i] : 0.f;
+        // old psi_H of rows j and j - 1
+        float hpy0 = 0.f, hpy2 = 0.f, hpx1 = 0.f, hpx2 = 0.f, hpz0 = 0.f, hpz1 = 0.f;
+        float mpy0 = 0.f, mpy2 = 0.f, mpx2 = 0.f, mpz0 = 0.f;
+        unsigned hoy = 0, hox = 0, hoz = 0;
+        if (syh >= 0) {
+            hoy = ((unsigned)syh * (g.nx + 1) + i) * (unsigned)g.kp + k;
+            hpy0 = f.h.psi[0][0][hoy];
+            hpy2 = f.h.psi[2][1][hoy];
+        }
+        if (sym >= 0) {
+            const unsigned o = ((unsigned)sym * (g.nx + 1) + i) * (unsigned)g.kp + k;
+            mpy0 = f.h.psi[0][0][o];
+            mpy2 = f.h.psi[2][1][o];
+        }
+        if (sxh >= 0) {
+            hox = (unsigned)sxh * si + (unsigned)j * sj + k;
+            hpx1 = f.h.psi[1][1][hox];
+            hpx2 = f.h.psi[2][0][hox];
+            mpx2 = f.h.psi[2][0][hox - dm];
+        }
+        if (zoh >= 0) {
+            hoz = (((unsigned)i * (g.ny + 1) + j) * ph.zg + zoh) * 4 + (k & 3);
+            hpz0 = f.h.psi[0][1][hoz];
+            hpz1 = f.h.psi[1][0][hoz];
+            mpz0 = f.h.psi[0][1][hoz - (jm_ok ? (unsigned)ph.zg * 4 : 0u)];
+        }
+        // E-side inputs: classes or coefficients, psi_E
+        const int sx = a.has_pml && iin ? pe.slot[0][i] : -1;
+        unsigned cls = 0;
+        float ca[3], cb[3];
+        if (kCmp) {
+            cls = a.cidx[q];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float2 t = s_tab[c * 256 + ((cls >> (8 * c)) & 255u)];
+                ca[c] = t.x;
+                cb[c] = t.y;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                ca[c] = a.ca[c * N + q];
+                cb[c] = a.cb[c * N + q];
+            }
+        }
+        float py0 = 0.f, py2 = 0.f, px1 = 0.f, px2 = 0.f, pz0 = 0.f, pz1 = 0.f, bx = 0.f, cx = 0.f;
+        unsigned oy = 0, ox = 0, oz = 0;
+        if (iin) {
+            if (sye >= 0) {
+                oy = ((unsigned)sye * (g.nx + 1) + i) * (unsigned)g.kp + k;
+                py0 = a.psi[0][0][oy];
+                py2 = a.psi[2][1][oy];
+            }
+            if (sx >= 0) {
+                ox = (unsigned)sx * si + (unsigned)j * sj + k;
+                px1 = a.psi[1][1][ox];
+                px2 = a.psi[2][0][ox];
+                bx = pe.b[0][i];
+                cx = pe.c[0][i];
+            }
+            if (zoe >= 0) {
+                oz = (((unsigned)i * (g.ny + 1) + j) * pe.zg + zoe) * 4 + (k & 3);
+                pz0 = a.psi[0][1][oz];
+                pz1 = a.psi[1][0][oz];
+            }
+        }
+        // new H of row j, and Hx, Hz of row j - 1
+        h_point(hxv, hyv, hzv, eyk - ey, ezj - ez, exk - ex, exj - ex, ezi - ez, eyi - ey, hx_s, hy_s, hz_s, jin, kin,
+                iin, syh, byh, cyh, sxh, bxh, cxh, inzh, bzh, czh, f.h.dth, hpy0, hpy2, hpx1, hpx2, hpz0, hpz1);
+        if (jm_ok) {
+            if (kin) hx_point(hxm, eykm - eym, ez - ezm, hy_m, hz_s, sym, bym, cym, inzh, bzh, czh, f.h.dth, mpy0, mpz0);
+            if (iin) hz_point(hzm, eyim - eym, ex - exm, hx_s, hy_m, sxh, bxh, cxh, sym, bym, cym, f.h.dth, mpx2, mpy2);
+        }
+        // H of (j, k - 1) from the lane below (the point itself at k = 0: unused there)
+        const float hxk0 = __shfl_up(hxv, 1u, 64), hyk0 = __shfl_up(hyv, 1u, 64);
+        const float hxk = k ? hxk0 : hxv, hyk = k ? hyk0 : hyv;
+        if (own) {
+            f.Hn[q] = hxv;
+            f.Hn[q + N] = hyv;
+            f.Hn[q + 2 * N] = hzv;
+            // every psi_H entry of the new set is written, updated or not
+            if (syh >= 0) {
+                f.psiHn[0][0][hoy] = hpy0;
+                f.psiHn[2][1][hoy] = hpy2;
+            }
+            if (sxh >= 0) {
+                f.psiHn[1][1][hox] = hpx1;
+                f.psiHn[2][0][hox] = hpx2;
+            }
+            if (zoh >= 0) {
+                f.psiHn[0][1][hoz] = hpz0;
+                f.psiHn[1][0][hoz] = hpz1;
+            }
+            float exn = ex, eyn = ey, ezn = ez;
+            if (iin) {
+                const bool ib_ = i >= 1;
+                const float hxj = jm_ok ? hxm : 0.f, hzj = jm_ok ? hzm : 0.f;
+                e_point(exn, eyn, ezn, ex, ey, ez, ca[0], cb[0], ca[1], cb[1], ca[2], cb[2], hzv - hzj, hyv - hyk,
+                        hxv - hxk, hzv - hzp, hyv - hyp, hxv - hxj, a.es[0][i], ey_s, ez_s, jb, jin, ib_, kb, kz, sye,
+                        bye, cye, sx, bx, cx, inze, bze, cze, py0, py2, px1, px2, pz0, pz1);
+                if (in_box) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        float &en = c == 0 ? exn : c == 1 ? eyn : ezn;
+                        const float e0 = c == 0 ? ex : c == 1 ? ey : ez;
+                        const int row = s_row[c * 256 + ((cls >> (8 * c)) & 255u)];
+                        if (row >= 0) debye_point<NP>(en, e0, cb[c], jv[c], s_bt + row * NP, a.kc, a.km1);
+#pragma unroll
+                        for (int pp = 0; pp < NPE; pp++) Jl[(c * NPE + pp) * bkp] = jv[c][pp];
+                    }
+                }
+                if (sye >= 0) {
+                    a.psi[0][0][oy] = py0;
+                    a.psi[2][1][oy] = py2;
+                }
+                if (sx >= 0) {
+                    a.psi[1][1][ox] = px1;
+                    a.psi[2][0][ox] = px2;
+                }
+                if (zoe >= 0) {
+                    a.psi[0][1][oz] = pz0;
+                    a.psi[1][0][oz] = pz1;
+                }
+            }
+            f.En[q] = exn;
+            f.En[q + N] = eyn;
+            f.En[q + 2 * N] = ezn;
+        }
+        hyp = hyv;
+        hzp = hzv;
+        ey = eyi;
+        ez = ezi;
+        eym = eyim;
+    }
+}
+
 // surface-impedance overwrite of the E edge of SIBC slot u (sibc.c:256-269):
 // J from up to four H faces, then the recursive-convolution terms
 __device__ __forceinline__ float sibc_edge(const PostArgs &a, const PostParams &pp, int u) {
@@ -1554,6 +1840,7 @@ struct Device {
     DBuf<float> F;               // E (3N) then H (3N); two such sets for the one-pass step
     int par = 0;                 // set holding the current fields (and psi_H)
     bool onepass = false;        // steps run k_step (H and E in one sweep, set to set)
+    bool steprows = false;       // ... as k_step_rows (no tiles: row j - 1 recomputed per lane)
     int tj = 0, ntj = 0;         // k_step: owned rows per tile, tiles over j
     int kw = 0, ntk = 0;         // k_step: owned k points per tile, tiles over k
     int iseg_s = 8;              // k_step: x planes per workgroup
@@ -1897,6 +2184,8 @@ int upload_plan(cem_sim *s, Device *D) {
             ntk = n;
         }
         D->onepass = tj >= 2 && mode && std::strcmp(mode, "onepass") == 0;
+        D->steprows = mode && std::strcmp(mode, "rows") == 0;
+        if (D->steprows) D->onepass = true, tj = 1, kw = g.kp, ntk = 1;
         D->tj = D->onepass ? tj : 0;
         D->ntj = D->onepass ? (ny + 1 + tj - 1) / tj : 0;
         D->kw = D->onepass ? kw : 0;
@@ -2403,6 +2692,29 @@ void launch_e(cem_sim *s, Device *D, int i0, int i1) {
 }
 
 typedef void (*SKernel)(SArgs, int);
+typedef void (*RKernel)(SArgs, int, int);
+
+RKernel rows_kernel(const Device *D) {
+    if (!D->compressed) return k_step_rows<0, false>;
+    switch (D->fused ? D->npole : 0) {
+    case 1: return k_step_rows<1, true>;
+    case 2: return k_step_rows<2, true>;
+    case 3: return k_step_rows<3, true>;
+    case 4: return k_step_rows<4, true>;
+    case 5: return k_step_rows<5, true>;
+    case 6: return k_step_rows<6, true>;
+    case 7: return k_step_rows<7, true>;
+    case 8: return k_step_rows<8, true>;
+    default: return k_step_rows<0, true>;
+    }
+}
+
+// k_step_rows: workgroups of kBlock lanes over the (j, k) plane (63 owned points per
+// wavefront) per x segment
+int rows_blocks_per_seg(const Device *D) {
+    const int waves = ((D->g.ny + 1) * D->g.kp + 62) / 63;
+    return (waves + kBlock / 64 - 1) / (kBlock / 64);
+}
 
 SKernel step_kernel(const Device *D) {
     if (!D->compressed) return k_step<0, false>;
@@ -2438,10 +2750,12 @@ SArgs step_args(cem_sim *s, Device *D, int par) {
 }
 
 int step_threads(const Device *D) {
+    if (D->steprows) return kBlock;
     return ((D->tj + 2) * (D->kw + 2) + 63) / 64 * 64;
 }
 
 size_t step_lds(const Device *D) {
+    if (D->steprows) return 0;
     const int kl = D->kw + 2, tj = D->tj;
     size_t b = (size_t)3 * (2 * tj + 3) * kl * sizeof(float);
     if (D->compressed) b += 768 * sizeof(float2);
@@ -2451,13 +2765,17 @@ size_t step_lds(const Device *D) {
 }
 
 int step_blocks(const Device *D) {
+    if (D->steprows) return rows_blocks_per_seg(D) * ((D->g.nx + 1 + D->iseg_s - 1) / D->iseg_s);
     return D->ntj * D->ntk * ((D->g.nx + 1 + D->iseg_s - 1) / D->iseg_s);
 }
 
 // H and E of one step in one pass; the new fields become the current set
 void launch_step(cem_sim *s, Device *D) {
     SArgs a = step_args(s, D, D->par);
-    launch_lds(D, D->tS, step_kernel(D), dim3(step_blocks(D)), dim3(step_threads(D)), step_lds(D), a, D->iseg_s);
+    if (D->steprows)
+        launch(D, D->tS, rows_kernel(D), dim3(step_blocks(D)), dim3(kBlock), a, D->iseg_s, rows_blocks_per_seg(D));
+    else
+        launch_lds(D, D->tS, step_kernel(D), dim3(step_blocks(D)), dim3(step_threads(D)), step_lds(D), a, D->iseg_s);
     D->par ^= 1;
 }
 
@@ -2702,12 +3020,16 @@ int run_chunk(cem_sim *s, Device *D) {
         const int nbh = stencil_blocks(D, D->wh), nbe = stencil_blocks(D, D->we);
         const EKernel ek = e_kernel(D);
         const HKernel hk = h_kernel(D);
-        const SKernel sk = D->onepass ? step_kernel(D) : nullptr;
+        const SKernel sk = D->onepass && !D->steprows ? step_kernel(D) : nullptr;
+        const RKernel rk = D->steprows ? rows_kernel(D) : nullptr;
         const int gh = nbh * ((nx + 1 + D->iseg_h - 1) / D->iseg_h);
         const int ge = nbe * ((nx + D->iseg_e - 1) / D->iseg_e);
         for (int k = 0; k < kGraphSteps; k++) {
             const int o = D->onepass ? (k & 1) : 0;
-            if (D->onepass) {
+            if (D->steprows) {
+                hipLaunchKernelGGL(rk, dim3(step_blocks(D)), dim3(kBlock), 0, D->st, sa[o], D->iseg_s,
+                                   rows_blocks_per_seg(D));
+            } else if (D->onepass) {
                 hipLaunchKernelGGL(sk, dim3(step_blocks(D)), dim3(step_threads(D)), step_lds(D), D->st, sa[o],
                                    D->iseg_s);
             } else {
@@ -2873,7 +3195,7 @@ std::map<std::array<long long, 10>, std::array<int, 3>> g_tune;
 
 long autotune(cem_sim *s, Device *D, long budget) {
     const std::array<long long, 10> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
-                                           (long long)D->fused, D->wh, D->we, (long long)D->onepass, (long long)D->tj * 4096 + D->kw};
+                                           (long long)D->fused, D->wh, D->we, (long long)D->onepass + 2 * D->steprows, (long long)D->tj * 4096 + D->kw};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);

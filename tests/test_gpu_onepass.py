@@ -1,5 +1,6 @@
-"""The one-pass step kernel (H and E updates of a step in one sweep, field sets alternating;
-COURANT_STEP=onepass) against the two separate sweeps that full steps run by default: bit-identical fields, psi and Debye
+"""The one-pass step kernels (H and E updates of a step in one sweep, field sets alternating:
+k_step with LDS tiles, COURANT_STEP=onepass; k_step_rows, which recomputes row j - 1's H per
+lane, COURANT_STEP=rows) against the two separate sweeps: bit-identical fields, psi and Debye
 state (the checkpoint bytes), port and probe spectra, after every stretch of a schedule
 that mixes graph-replayed chunks, single steps, host reads and a host write; on scenes with
 CPML on five faces, fused and post-kernel Debye currents, SIBC, lumped and mode ports, the
@@ -66,24 +67,25 @@ def compare(a, b, tag):
     assert np.array_equal(a.source_spectrum(), b.source_spectrum()), tag
 
 
+@pytest.mark.parametrize("mode", ["onepass", "rows"])
 @pytest.mark.parametrize("scene", sorted(SCENES))
 @pytest.mark.parametrize("variant", ["classes", "arrays", "debye_list"])
-def test_onepass_step_matches_separate_sweeps(cem, gpu_available, monkeypatch, scene, variant):
+def test_onepass_step_matches_separate_sweeps(cem, gpu_available, monkeypatch, scene, variant, mode):
     if variant != "classes" and scene != "debye_sibc":
         pytest.skip("coefficient arrays and post-kernel Debye run on the CPML + Debye + SIBC scene")
-    print_case(f"one-pass step vs separate sweeps: {scene}, {variant}")
+    print_case(f"one-pass step ({mode}) vs separate sweeps: {scene}, {variant}")
     if variant == "arrays":
         monkeypatch.setenv("COURANT_COEF_ARRAYS", "1")
     if variant == "debye_list":
         monkeypatch.setenv("COURANT_DEBYE_LIST", "1")
-    monkeypatch.setenv("COURANT_STEP", "onepass")
+    monkeypatch.setenv("COURANT_STEP", mode)
     a = SCENES[scene](cem)
     a.energy()          # set-up reads the switches when the sim first runs on the GPU
     monkeypatch.delenv("COURANT_STEP")
     b = SCENES[scene](cem)
     b.energy()
     rows, planes = a.step_shape()
-    assert rows >= 2 and planes >= 1, (rows, planes)
+    assert rows >= (2 if mode == "onepass" else 1) and planes >= 1, (rows, planes)
     assert b.step_shape() == (0, 0)
     for n in (200, 37, 64, 1, 450):
         a.step(n)
@@ -99,15 +101,16 @@ def test_onepass_step_matches_separate_sweeps(cem, gpu_available, monkeypatch, s
     compare(a, b, "after host write")
 
 
-def test_onepass_launch_shape_is_bit_identical(cem, gpu_available, monkeypatch):
+@pytest.mark.parametrize("mode", ["onepass", "rows"])
+def test_onepass_launch_shape_is_bit_identical(cem, gpu_available, monkeypatch, mode):
     """The autotuner picks the x planes per workgroup of the one-pass step on the first steps;
     a run with a fixed, different shape gives bit-identical fields and spectra."""
-    print_case("one-pass step: autotuned vs fixed planes per workgroup")
-    monkeypatch.setenv("COURANT_STEP", "onepass")
+    print_case(f"one-pass step ({mode}): autotuned vs fixed planes per workgroup")
+    monkeypatch.setenv("COURANT_STEP", mode)
     a = S.checkpoint_scene(cem)
     a.step(300)
     rows, planes = a.step_shape()
-    assert rows >= 2 and planes in (4, 8, 16)
+    assert rows >= (2 if mode == "onepass" else 1) and planes in (4, 8, 16)
     monkeypatch.setenv("COURANT_ISEG_S", "3" if planes != 3 else "5")
     b = S.checkpoint_scene(cem)
     b.step(300)
