@@ -13,26 +13,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
-#include <mutex>
-#include <shared_mutex>
 
 namespace cem {
-
-// Stream capture and stream-ordered allocation exclude each other in the process.  With
-// one thread capturing a graph on its stream (thread-local capture mode), hipMallocAsync
-// on another thread's stream has returned hipErrorStreamCaptureUnsupported ("operation
-// not permitted when stream is capturing") on this runtime: two port sweeps, one
-// capturing its LU graph while the other allocated its next batch.  Captures hold this
-// lock exclusively from begin to end; allocations and frees hold it shared.
-inline std::shared_mutex &capture_mutex() {
-    static std::shared_mutex m;
-    return m;
-}
-
-// held by a thread for the whole of one stream capture
-struct CaptureLock {
-    std::unique_lock<std::shared_mutex> lock{capture_mutex()};
-};
 
 inline void pool_keep_resident() {
     static const bool done = [] {
@@ -49,14 +31,11 @@ inline void pool_keep_resident() {
 
 inline hipError_t dev_alloc(void **p, size_t bytes, hipStream_t st) {
     pool_keep_resident();
-    std::shared_lock<std::shared_mutex> lock(capture_mutex());
     return hipMallocAsync(p, bytes, st);
 }
 
 inline void dev_free(void *p, hipStream_t st) {
-    if (!p) return;
-    std::shared_lock<std::shared_mutex> lock(capture_mutex());
-    (void)hipFreeAsync(p, st);
+    if (p) (void)hipFreeAsync(p, st);
 }
 
 }  // namespace cem

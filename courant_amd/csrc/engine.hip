@@ -41,6 +41,7 @@
 
 #include "courant_amd.h"
 #include "devmem.hpp"
+#include "graph.hpp"
 #include "sim.hpp"
 
 using namespace cem;
@@ -3011,12 +3012,8 @@ int run_chunk(cem_sim *s, Device *D) {
     }
     if (!D->gexec || key != D->gkey) {
         drop_graph(D);
-        hipGraph_t graph = nullptr;
-        CaptureLock capture;   // no stream-ordered allocation in any thread during the capture
-        if (hipStreamBeginCapture(D->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            D->use_graph = false;
-            return -1;
-        }
+        // the chunk's launches as a chain of kernel nodes (no stream capture)
+        KernelChain chain;
         const int nbh = stencil_blocks(D, D->wh), nbe = stencil_blocks(D, D->we);
         const EKernel ek = e_kernel(D);
         const HKernel hk = h_kernel(D);
@@ -3024,29 +3021,25 @@ int run_chunk(cem_sim *s, Device *D) {
         const RKernel rk = D->steprows ? rows_kernel(D) : nullptr;
         const int gh = nbh * ((nx + 1 + D->iseg_h - 1) / D->iseg_h);
         const int ge = nbe * ((nx + D->iseg_e - 1) / D->iseg_e);
+        const int rb = D->steprows ? rows_blocks_per_seg(D) : 0;
         for (int k = 0; k < kGraphSteps; k++) {
             const int o = D->onepass ? (k & 1) : 0;
             if (D->steprows) {
-                hipLaunchKernelGGL(rk, dim3(step_blocks(D)), dim3(kBlock), 0, D->st, sa[o], D->iseg_s,
-                                   rows_blocks_per_seg(D));
+                chain.add(rk, dim3(step_blocks(D)), dim3(kBlock), 0, sa[o], D->iseg_s, rb);
             } else if (D->onepass) {
-                hipLaunchKernelGGL(sk, dim3(step_blocks(D)), dim3(step_threads(D)), step_lds(D), D->st, sa[o],
-                                   D->iseg_s);
+                chain.add(sk, dim3(step_blocks(D)), dim3(step_threads(D)), (unsigned)step_lds(D), sa[o], D->iseg_s);
             } else {
-                hipLaunchKernelGGL(hk, dim3(gh), dim3(kBlock), 0, D->st, ha, 0, nx + 1, D->iseg_h, nbh);
-                hipLaunchKernelGGL(ek, dim3(ge), dim3(kBlock), 0, D->st, ea, 0, nx, D->iseg_e, nbe);
+                chain.add(hk, dim3(gh), dim3(kBlock), 0, ha, 0, nx + 1, D->iseg_h, nbh);
+                chain.add(ek, dim3(ge), dim3(kBlock), 0, ea, 0, nx, D->iseg_e, nbe);
             }
-            if (geo[2]) hipLaunchKernelGGL(k_post, dim3(geo[2]), dim3(kBlock), 0, D->st, pa[o]);
-            hipLaunchKernelGGL(k_gather, dim3(1), dim3(kGatherBlock), 0, D->st, ga[o]);
+            if (geo[2]) chain.add(k_post, dim3(geo[2]), dim3(kBlock), 0, pa[o]);
+            chain.add(k_gather, dim3(1), dim3(kGatherBlock), 0, ga[o]);
         }
-        const hipError_t ce = hipStreamEndCapture(D->st, &graph);
-        if (ce != hipSuccess || !graph || hipGraphInstantiate(&D->gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
+        if (!chain.instantiate(&D->gexec)) {
             D->gexec = nullptr;
             D->use_graph = false;   // fall back to direct launches for this simulation
             return -1;
         }
-        (void)hipGraphDestroy(graph);
         D->gkey.swap(key);
     }
     if (hipGraphLaunch(D->gexec, D->st) != hipSuccess) {

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "devmem.hpp"
+#include "graph.hpp"
 #include "mode_gpu.hpp"
 #include "sim.hpp"
 
@@ -833,7 +834,6 @@ struct HMem {
     bool get(size_t count) {
         if (count <= n) return true;
         release();
-        std::shared_lock<std::shared_mutex> lock(capture_mutex());
         if (hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
             p = nullptr;
             return false;
@@ -842,10 +842,7 @@ struct HMem {
         return true;
     }
     void release() {
-        if (p) {
-            std::shared_lock<std::shared_mutex> lock(capture_mutex());
-            (void)hipHostFree(p);
-        }
+        if (p) (void)hipHostFree(p);
         p = nullptr;
         n = 0;
     }
@@ -1104,48 +1101,46 @@ int mode_gpu_solve_batch(ModeGpu **ctx, const ModeGpuProblem *P, int nbq, std::c
     hipLaunchKernelGGL(k_diag, dim3((n + T - 1) / T, by), dim3(T), 0, st, g, c, (const zc *)M.prm.p, bigs, M.band.p);
     if (debug) MG_CHECK(hipEventRecord(ev[1], st));
     // band LU, panel by panel: two launches per panel (1897 panels on a 30 k-unknown
-    // port), every problem of the batch in each.  Replayed from a graph captured on the
+    // port), every problem of the batch in each.  Replayed from a graph built on the
     // first solve of a geometry and batch size, so that solves running at once on other
     // host threads (the port sweeps) do not queue behind thousands of launch
-    // submissions; COURANT_GRAPH=0 launches directly.
-    auto launch_lu = [&]() {
+    // submissions; COURANT_GRAPH=0 launches directly.  `put` launches a kernel or
+    // appends it to the graph.
+    auto launch_lu = [&](auto &&put) {
         for (int j0 = 0; j0 < n; j0 += nb) {
             const int j1 = std::min(n, j0 + nb);
             const int R = std::min(j1 - 1 + kl, n - 1) - j0 + 1;
-            const size_t lds = (size_t)R * (j1 - j0) * sizeof(zc);
+            const unsigned lds = (unsigned)((size_t)R * (j1 - j0) * sizeof(zc));
             const dim3 pb((unsigned)((R + 63) / 64 * 64));   // one thread per panel row (R <= 512)
             if (nb > 8)
-                hipLaunchKernelGGL(k_lu_panel<16>, dim3(1, by), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+                put(k_lu_panel<16>, dim3(1, by), pb, 0u, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
             else
-                hipLaunchKernelGGL(k_lu_panel<8>, dim3(1, by), pb, 0, st, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
+                put(k_lu_panel<8>, dim3(1, by), pb, 0u, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
             const int ncol = std::min(j1 - 1 + kv, n - 1) - j1 + 1;
             if (ncol > 0) {
                 const int wpb = kLuThreads / 64;
-                hipLaunchKernelGGL(k_lu_trail, dim3((ncol + wpb - 1) / wpb, by), dim3(kLuThreads), lds, st, g,
-                                   M.band.p, M.ipiv.p, j0, j1);
+                put(k_lu_trail, dim3((ncol + wpb - 1) / wpb, by), dim3(kLuThreads), lds, g, M.band.p, M.ipiv.p, j0,
+                    j1);
             }
         }
     };
     const char *gv = std::getenv("COURANT_GRAPH");
     if (gv && std::atoi(gv) == 0) {
-        launch_lu();
+        launch_lu([&](auto kernel, dim3 grid, dim3 block, unsigned lds, auto... args) {
+            hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+        });
     } else {
         const void *key[3] = {M.band.p, M.ipiv.p, M.info.p};
         if (!M.lu || M.lu_n != n || M.lu_bw != kl || M.lu_b != nbq || std::memcmp(key, M.lu_key, sizeof key) != 0) {
-            // one capture at a time in the process, and no allocation in any thread during
-            // it (once per context; the sweeps' clones reach their first solve together)
-            CaptureLock lock;
             if (M.lu) (void)hipGraphExecDestroy(M.lu);
             M.lu = nullptr;
-            hipGraph_t graph = nullptr;
-            MG_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            launch_lu();
-            MG_CHECK(hipStreamEndCapture(st, &graph));
-            const hipError_t e = hipGraphInstantiate(&M.lu, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) {
+            KernelChain chain;
+            launch_lu([&](auto kernel, dim3 grid, dim3 block, unsigned lds, auto... args) {
+                chain.add(kernel, grid, block, lds, args...);
+            });
+            if (!chain.instantiate(&M.lu)) {
                 M.lu = nullptr;
-                set_error("GPU mode solver: cannot instantiate the LU graph");
+                set_error("GPU mode solver: cannot build the LU graph");
                 return -1;
             }
             std::memcpy(M.lu_key, key, sizeof key);
