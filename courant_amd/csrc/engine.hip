@@ -122,6 +122,12 @@ struct HArgs {
     float *psi[3][2];       // psiH[c][w]
     float dth;
     int has_pml;
+    // pair: a second right-hand side of the same plan (the other excitation of a 2-port),
+    // whose fields and psi_H these are; odd tiles of the launch update it
+    int pair;
+    const float *E2;
+    float *H2;
+    float *psi2[3][2];
 };
 
 // Per-edge physics parameters, resident on the device so the post kernel's
@@ -158,6 +164,12 @@ struct EArgs {
     int nrow;
     int bi0, bi1, bj0, bj1, bg0, bg1;
     float kc[kFusedPoles], km1[kFusedPoles];
+    // pair: the second right-hand side's fields, psi_E and Debye currents (odd tiles)
+    int pair;
+    float *E2;
+    const float *H2;
+    float *psi2[3][2];
+    float *J2;
 };
 
 // Device-side step and series-row counters: every per-step kernel argument is
@@ -384,7 +396,18 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, int iseg, int npblk) {
     const Grid g = a.g;
     const int ngw = g.kp / W;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (a.pair) {   // the two right-hand sides of a tile are neighbouring tiles (one XCD)
+        if (tile & 1) {
+            a.E = a.E2;
+            a.H = a.H2;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int w = 0; w < 2; w++) a.psi[c][w] = a.psi2[c][w];
+        }
+        tile >>= 1;
+    }
     const int pblk = tile % npblk, seg = tile / npblk;
     const int p = pblk * kBlock + threadIdx.x;
     if (p >= (g.ny + 1) * ngw) return;
@@ -587,7 +610,19 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     }
     const Grid g = a.g;
     const int ngw = g.kp / W;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (a.pair) {   // the two right-hand sides of a tile are neighbouring tiles (one XCD)
+        if (tile & 1) {
+            a.E = a.E2;
+            a.H = a.H2;
+            a.J = a.J2;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int w = 0; w < 2; w++) a.psi[c][w] = a.psi2[c][w];
+        }
+        tile >>= 1;
+    }
     const int pblk = tile % npblk, seg = tile / npblk;
     const int p = pblk * kBlock + threadIdx.x;
     if (p >= (g.ny + 1) * ngw) return;
@@ -1842,6 +1877,8 @@ struct Device {
     int par = 0;                 // set holding the current fields (and psi_H)
     bool onepass = false;        // steps run k_step (H and E in one sweep, set to set)
     bool steprows = false;       // ... as k_step_rows (no tiles: row j - 1 recomputed per lane)
+    Device *partner = nullptr;   // while a pair steps: the second right-hand side (same plan),
+    cem_sim *partner_sim = nullptr;   // whose H and E updates share this device's launches
     int tj = 0, ntj = 0;         // k_step: owned rows per tile, tiles over j
     int kw = 0, ntk = 0;         // k_step: owned k points per tile, tiles over k
     int iseg_s = 8;              // k_step: x planes per workgroup
@@ -2678,18 +2715,42 @@ void launch(Device *D, KernelTimer &t, F kernel, dim3 grid, dim3 block, A... arg
     launch_lds(D, t, kernel, grid, block, 0, args...);
 }
 
+// The H / E arguments of a paired step: the partner's fields, psi and Debye currents
+// as the second right-hand side (its plan, coefficient tables and lists are this
+// device's; a pair runs the split sweeps, field set 0)
+void pair_h(HArgs &a, const Device *P) {
+    if (!P) return;
+    a.pair = 1;
+    a.E2 = fset(P, P->par);
+    a.H2 = fset(P, P->par) + 3 * P->g.N;
+    for (int c = 0; c < 3; c++)
+        for (int w = 0; w < 2; w++) a.psi2[c][w] = P->psiHs[P->par][c][w];
+}
+
+void pair_e(EArgs &a, const Device *P) {
+    if (!P) return;
+    a.pair = 1;
+    a.E2 = fset(P, P->par);
+    a.H2 = fset(P, P->par) + 3 * P->g.N;
+    for (int c = 0; c < 3; c++)
+        for (int w = 0; w < 2; w++) a.psi2[c][w] = P->psiE[c][w];
+    a.J2 = P->fused ? P->J.p : nullptr;
+}
+
 void launch_h(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     HArgs a = h_args(s, D, D->par);
+    pair_h(a, D->partner);
     const int npblk = stencil_blocks(D, D->wh), nseg = (i1 - i0 + D->iseg_h - 1) / D->iseg_h;
-    launch(D, D->tH, h_kernel(D), dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_h, npblk);
+    launch(D, D->tH, h_kernel(D), dim3(npblk * nseg * (1 + a.pair)), dim3(kBlock), a, i0, i1, D->iseg_h, npblk);
 }
 
 void launch_e(cem_sim *s, Device *D, int i0, int i1) {
     if (i1 <= i0) return;
     EArgs a = e_args(s, D, D->par);
+    pair_e(a, D->partner);
     const int npblk = stencil_blocks(D, D->we), nseg = (i1 - i0 + D->iseg_e - 1) / D->iseg_e;
-    launch(D, D->tE, e_kernel(D), dim3(npblk * nseg), dim3(kBlock), a, i0, i1, D->iseg_e, npblk);
+    launch(D, D->tE, e_kernel(D), dim3(npblk * nseg * (1 + a.pair)), dim3(kBlock), a, i0, i1, D->iseg_e, npblk);
 }
 
 typedef void (*SKernel)(SArgs, int);
@@ -2891,6 +2952,7 @@ void launch_post(cem_sim *s, Device *D) {
     D->ctl_step = s->step;
     D->ctl_row = D->rows;
     if (D->rows == kSeriesCap) launch_dft(s, D, 0);
+    if (D->partner) launch_post(D->partner_sim, D->partner);
 }
 
 void launch_dft(cem_sim *s, Device *D, int direct) {
@@ -2986,13 +3048,28 @@ void drop_graph(Device *D) {
 // the host after the chunk that fills the series.
 int run_chunk(cem_sim *s, Device *D) {
     sync_ctl(s, D);
+    Device *P = D->partner;
+    cem_sim *ps = D->partner_sim;
+    if (P) sync_ctl(ps, P);
     const int nx = s->n[0];
     // launch arguments of the chunk's steps: with the one-pass step the field sets
     // alternate, so even and odd steps differ (kGraphSteps is even: the chunk ends
     // on the set it started from)
     const int p0 = D->par, p1 = D->onepass ? D->par ^ 1 : D->par;
-    const HArgs ha = h_args(s, D, p0);
-    const EArgs ea = e_args(s, D, p0);
+    HArgs ha = h_args(s, D, p0);
+    EArgs ea = e_args(s, D, p0);
+    pair_h(ha, P);
+    pair_e(ea, P);
+    // the partner's post-phase and probe kernels (a pair runs the split sweeps: one field set)
+    PostArgs pb;
+    GatherArgs gb;
+    std::memset(&pb, 0, sizeof pb);
+    std::memset(&gb, 0, sizeof gb);
+    if (P) {
+        pb = post_args(ps, P, P->par);
+        gb = gather_args(ps, P, 0, P->par);
+    }
+    const int nbp = P ? post_blocks(ps, P) : 0;
     SArgs sa[2];
     std::memset(sa, 0, sizeof sa);   // the bytes key the captured graph
     if (D->onepass) {
@@ -3006,7 +3083,9 @@ int run_chunk(cem_sim *s, Device *D) {
     std::vector<unsigned char> key;
     for (auto pr : {std::make_pair((const void *)&ha, sizeof ha), std::make_pair((const void *)&ea, sizeof ea),
                     std::make_pair((const void *)sa, sizeof sa), std::make_pair((const void *)pa, sizeof pa),
-                    std::make_pair((const void *)ga, sizeof ga), std::make_pair((const void *)geo, sizeof geo)}) {
+                    std::make_pair((const void *)ga, sizeof ga), std::make_pair((const void *)geo, sizeof geo),
+                    std::make_pair((const void *)&pb, sizeof pb), std::make_pair((const void *)&gb, sizeof gb),
+                    std::make_pair((const void *)&nbp, sizeof nbp)}) {
         const unsigned char *b = (const unsigned char *)pr.first;
         key.insert(key.end(), b, b + pr.second);
     }
@@ -3029,11 +3108,15 @@ int run_chunk(cem_sim *s, Device *D) {
             } else if (D->onepass) {
                 chain.add(sk, dim3(step_blocks(D)), dim3(step_threads(D)), (unsigned)step_lds(D), sa[o], D->iseg_s);
             } else {
-                chain.add(hk, dim3(gh), dim3(kBlock), 0, ha, 0, nx + 1, D->iseg_h, nbh);
-                chain.add(ek, dim3(ge), dim3(kBlock), 0, ea, 0, nx, D->iseg_e, nbe);
+                chain.add(hk, dim3(gh * (1 + ha.pair)), dim3(kBlock), 0, ha, 0, nx + 1, D->iseg_h, nbh);
+                chain.add(ek, dim3(ge * (1 + ea.pair)), dim3(kBlock), 0, ea, 0, nx, D->iseg_e, nbe);
             }
             if (geo[2]) chain.add(k_post, dim3(geo[2]), dim3(kBlock), 0, pa[o]);
             chain.add(k_gather, dim3(1), dim3(kGatherBlock), 0, ga[o]);
+            if (P) {
+                if (nbp) chain.add(k_post, dim3(nbp), dim3(kBlock), 0, pb);
+                chain.add(k_gather, dim3(1), dim3(kGatherBlock), 0, gb);
+            }
         }
         if (!chain.instantiate(&D->gexec)) {
             D->gexec = nullptr;
@@ -3048,12 +3131,17 @@ int run_chunk(cem_sim *s, Device *D) {
         D->use_graph = false;
         return -1;
     }
-    if (D->rows == 0) D->row0 = s->step;
-    D->rows += kGraphSteps;
-    s->step += kGraphSteps;
-    D->ctl_step = s->step;
-    D->ctl_row = D->rows;
-    if (D->rows == kSeriesCap) launch_dft(s, D, 0);
+    for (auto rhs : {std::make_pair(s, D), std::make_pair(ps, P)}) {
+        cem_sim *t = rhs.first;
+        Device *T = rhs.second;
+        if (!T) continue;
+        if (T->rows == 0) T->row0 = t->step;
+        T->rows += kGraphSteps;
+        t->step += kGraphSteps;
+        T->ctl_step = t->step;
+        T->ctl_row = T->rows;
+        if (T->rows == kSeriesCap) launch_dft(t, T, 0);
+    }
     return 0;
 }
 
@@ -3188,7 +3276,7 @@ std::map<std::array<long long, 10>, std::array<int, 3>> g_tune;
 
 long autotune(cem_sim *s, Device *D, long budget) {
     const std::array<long long, 10> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
-                                           (long long)D->fused, D->wh, D->we, (long long)D->onepass + 2 * D->steprows, (long long)D->tj * 4096 + D->kw};
+                                           (long long)D->fused, D->wh, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr), (long long)D->tj * 4096 + D->kw};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
@@ -3280,6 +3368,7 @@ long autotune(cem_sim *s, Device *D, long budget) {
 void cem::dev_step(cem_sim *s, long nsteps) {
     Device *D = s->dev;
     ensure_vlast(s, D);
+    if (D->partner) ensure_vlast(D->partner_sim, D->partner);
     const int nx = s->n[0];
     long it = 0;
     if (!D->tuned && !D->profiling) it += autotune(s, D, nsteps);
@@ -3677,6 +3766,47 @@ extern "C" int cem_amd_synchronize(cem_sim *s) {
 extern "C" int cem_amd_step_device(cem_sim *s, long n) {
     if (!s || !s->finalized || dev_ensure(s)) return -1;
     dev_step(s, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Advance both right-hand sides of a pair (b an excitation clone of a, same step) n
+// steps: each step's H and E updates of a and b run as one launch each (odd tiles b),
+// reading a's coefficient classes, Debye tables and CPML profiles; the post-phase and
+// probe kernels run per right-hand side; everything goes on a's stream (chunks of
+// steps as one graph), and b's later work on its own stream is ordered after it.
+extern "C" int cem_amd_step_pair(cem_sim *a, cem_sim *b, long n) {
+    if (!a || !b || a == b || !a->finalized || !b->finalized) {
+        set_error("step_pair needs two finalized simulations");
+        return -1;
+    }
+    if (a->plan_id != b->plan_id) {
+        set_error("step_pair needs an excitation clone of the same plan");
+        return -1;
+    }
+    if (dev_ensure(a) || dev_ensure(b)) return -1;
+    Device *A = a->dev, *B = b->dev;
+    if (A->onepass || B->onepass || a->step != b->step || A->rows != B->rows || A->par != B->par) {
+        set_error("step_pair: both simulations must run the split sweeps at the same step");
+        return -1;
+    }
+    if (n <= 0) return 0;
+    HIP_OK(hipStreamSynchronize(B->st));   // b's uploads and earlier steps are done
+    const hipStream_t bst = B->st;
+    B->st = A->st;
+    A->partner = B;                       // a's launch shape (tuned on the pair's first
+    A->partner_sim = b;                   // steps unless fixed) serves both
+    dev_step(a, n);
+    A->partner = nullptr;
+    A->partner_sim = nullptr;
+    B->iseg_h = A->iseg_h;
+    B->iseg_e = A->iseg_e;
+    B->tuned = A->tuned;
+    B->st = bst;
+    hipEvent_t ev;
+    HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(ev, A->st));
+    HIP_OK(hipStreamWaitEvent(bst, ev, 0));
+    HIP_OK(hipEventDestroy(ev));
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -8,6 +8,7 @@
 // reference's operation order and is compiled without contraction, so the
 // exported plan (coefficients and every integer list) is bit-identical to the
 // reference's for the same calls.
+#include "courant_amd.h"
 #include "sim.hpp"
 
 #include <cmath>
@@ -15,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 #include <new>
 #include <thread>
 #include <vector>
@@ -678,6 +680,8 @@ extern "C" int cem_finalize(cem_sim *s, double cfl) {
         }
         s->step = 0;
         s->finalized = 1;
+        static std::atomic<long> plans{0};
+        s->plan_id = ++plans;
         if (modeports_finalize(s)) {
             s->finalized = 0;
             return -1;
@@ -687,6 +691,117 @@ extern "C" int cem_finalize(cem_sim *s, double cfl) {
         return -1;
     }
     return 0;
+}
+
+// ---- excitation clone (engine extension; no counterpart in the reference) ----
+// A second simulation with a's plan for another excitation of the same board: every
+// set-up array is copied (no set-up step runs again, mode-port cross-sections are
+// copied with their solved state), the time-dependent state starts from zero at step 0
+// on its own device buffers, and the excited port moves to `port`: lumped port `port`
+// with a's Gaussian, or, when a drives a mode port, mode port `port` with a's waveform.
+extern "C" cem_sim *cem_amd_clone_excitation(const cem_sim *a, int port) {
+    if (!a || !a->finalized) {
+        set_error("simulation not finalized");
+        return nullptr;
+    }
+    const bool modal = a->msrc >= 0;
+    if (port < 0 || port >= (int)(modal ? a->mport.size() : a->port.size())) {
+        set_error(modal ? "bad mode port" : "bad port");
+        return nullptr;
+    }
+    cem_sim *b = nullptr;
+    try {
+        b = new cem_sim();
+        for (int x = 0; x < 3; x++) {
+            b->n[x] = a->n[x];
+            b->node[x] = a->node[x];
+            b->d[x] = a->d[x];
+            b->dd[x] = a->dd[x];
+            b->ca[x] = a->ca[x];
+            b->cb[x] = a->cb[x];
+            b->edge_pec[x] = a->edge_pec[x];
+            b->ide[x] = a->ide[x];
+            b->idh[x] = a->idh[x];
+            b->be[x] = a->be[x];
+            b->ce[x] = a->ce[x];
+            b->bh[x] = a->bh[x];
+            b->ch[x] = a->ch[x];
+            b->pml_sn[x] = a->pml_sn[x];
+            b->pml_kn[x] = a->pml_kn[x];
+            b->pml_an[x] = a->pml_an[x];
+            b->pml_sc[x] = a->pml_sc[x];
+            b->pml_kc[x] = a->pml_kc[x];
+            b->pml_ac[x] = a->pml_ac[x];
+        }
+        b->si = a->si;
+        b->sj = a->sj;
+        b->N = a->N;
+        b->eps_r = a->eps_r;
+        b->sigma = a->sigma;
+        b->cell_pec = a->cell_pec;
+        for (int f = 0; f < 6; f++) {
+            b->bc[f] = a->bc[f];
+            b->npml[f] = a->npml[f];
+        }
+        b->has_pml = a->has_pml;
+        b->dt = a->dt;
+        b->step = 0;
+        b->finalized = 1;
+        b->plan_id = a->plan_id;
+        b->port = a->port;
+        b->src_port = a->src_port;
+        b->src_amp = a->src_amp;
+        b->src_t0 = a->src_t0;
+        b->src_tau = a->src_tau;
+        b->nf = a->nf;
+        b->freq = a->freq;
+        b->rotE = a->rotE;
+        b->vp = a->vp;
+        b->ip = a->ip;
+        b->mport = a->mport;
+        for (auto &mp : b->mport) mp.xs = nullptr;
+        for (size_t k = 0; k < a->mport.size(); k++)
+            if (a->mport[k].xs && !(b->mport[k].xs = mode2d_clone(a->mport[k].xs))) throw std::bad_alloc();
+        b->msrc = a->msrc;
+        b->msrc_t0 = a->msrc_t0;
+        b->msrc_tau = a->msrc_tau;
+        b->npole = a->npole;
+        std::copy(a->pole_f, a->pole_f + kMaxPoles, b->pole_f);
+        b->cell_deps = a->cell_deps;
+        b->ndisp = a->ndisp;
+        b->d_comp = a->d_comp;
+        b->d_idx = a->d_idx;
+        b->d_beta = a->d_beta;
+        b->d_cb = a->d_cb;
+        std::copy(a->d_k, a->d_k + kMaxPoles, b->d_k);
+        b->sibc_on = a->sibc_on;
+        b->sibc_rough = a->sibc_rough;
+        b->sibc_npole = a->sibc_npole;
+        b->sibc_sigma = a->sibc_sigma;
+        b->sibc_p1 = a->sibc_p1;
+        b->sibc_p2 = a->sibc_p2;
+        b->sibc_fit_err = a->sibc_fit_err;
+        std::copy(a->sibc_a, a->sibc_a + kSibcTerms, b->sibc_a);
+        std::copy(a->sibc_c, a->sibc_c + kSibcTerms, b->sibc_c);
+        std::copy(a->sibc_e, a->sibc_e + kSibcTerms, b->sibc_e);
+        b->nsb = a->nsb;
+        b->sb_comp = a->sb_comp;
+        b->sb_nf = a->sb_nf;
+        b->sb_hc = a->sb_hc;
+        b->sb_idx = a->sb_idx;
+        b->sb_hi = a->sb_hi;
+        b->sb_sg = a->sb_sg;
+        b->sb_mask = a->sb_mask;
+    } catch (const std::bad_alloc &) {
+        if (b) cem_destroy(b);
+        set_error("out of memory");
+        return nullptr;
+    }
+    if (modal)
+        b->msrc = port;
+    else
+        b->src_port = port;
+    return b;
 }
 
 // ---- results (sim.c:1016-1022): host copies of the device FP64 accumulators ----

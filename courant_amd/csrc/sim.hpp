@@ -83,6 +83,7 @@ struct cem_sim {
     double dt = 0;
     long step = 0;
     int finalized = 0;
+    long plan_id = 0;              // set by finalize; an excitation clone keeps its source's
 
     std::vector<cem::LumpedPort> port;
     int src_port = -1;

@@ -136,6 +136,8 @@ AMD_SIGNATURES = {
     "cem_amd_s2_renormalize_noalloc": (_I, [_DP, _I, _DP, _DP, _D]),
     "cem_amd_launch_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cem_amd_step_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cem_amd_clone_excitation": (_P, [_P, _I]),
+    "cem_amd_step_pair": (_I, [_P, _P, _L]),
 }
 
 
@@ -474,6 +476,24 @@ class Sim:
 
     def synchronize(self):
         return self._ok(self.lib.cem_amd_synchronize(self.h))
+
+    def clone_excitation(self, port):
+        """A simulation with this one's plan for another excited port (lumped port `port`
+        with this one's Gaussian, or mode port `port` when this one drives a mode port): no
+        set-up step runs again; its fields start from zero at step 0."""
+        h = self.lib.cem_amd_clone_excitation(self.h, port)
+        if not h:
+            raise CourantError(self.cem.last_error())
+        c = Sim(self.cem, self.x, self.y, self.z, handle=h)
+        c.freqs = self.freqs.copy()
+        c.excitation_lumped = self.excitation_lumped
+        c.excitation_mode = self.excitation_mode
+        return c
+
+    def step_pair(self, other, n):
+        """Advance this simulation and its excitation clone `other` (same step) n steps: the H
+        and E updates of both run as one launch each per step."""
+        return self._ok(self.lib.cem_amd_step_pair(self.h, other.h, n))
 
     def tune(self):
         """Run the launch-shape autotuner now; returns the steps it ran."""

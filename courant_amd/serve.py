@@ -10,6 +10,9 @@ releases the interpreter lock for the duration of each call.
 * random_request(seed)      the served workload's board: cells log-normal (median 3 M,
                             sigma 0.8) in [0.5 M, 20 M], lossless or lossy, 1 or 2 ports
 * solve(cem, b, excite)     one right-hand side: build, step, port V/I spectra
+* solve_pair(cem, b)        both right-hand sides of a 2-port board: one set-up, the second
+                            excitation a clone of the first's plan, the two stepped in shared
+                            launches (one H and one E launch per step for both)
 * s_matrix(...)             S = B A^-1 from the excitations' port waves (per frequency)
 * run_batch(...)            many boards at once on a pool of host threads
 * Server / serve(...)       a request loop: boards arrive as a Poisson stream, each
@@ -195,6 +198,30 @@ def solve(cem, b, excite):
     return (v, i), n
 
 
+def solve_pair(cem, b):
+    """Both right-hand sides of a 2-port lumped board: the board is built once (port 1
+    driven), port 2's excitation is a clone of that plan (Sim.clone_excitation: no second
+    set-up), and the two advance together (Sim.step_pair: each step's H and E updates of
+    both in one launch each).  Returns ([(V, I) of each excitation], steps); the spectra
+    equal those of solve(cem, b, 0) and solve(cem, b, 1) bit for bit."""
+    s, ports = build(cem, b, 0)
+    try:
+        c = s.clone_excitation(ports[1])
+        try:
+            n = steps_for(b, s.dt)
+            s.step_pair(c, n)
+            out = [(np.array([x.port_v(p) for p in ports]), np.array([x.port_i(p) for p in ports])) for x in (s, c)]
+        finally:
+            c.close()
+    finally:
+        s.close()
+    return out, n
+
+
+def pairable(b):
+    return b.get("nport", 2) == 2 and b.get("kind") != "board"
+
+
 def excitations(b):
     return (0, 1) if b.get("nport", 2) == 2 else (0,)
 
@@ -228,22 +255,37 @@ def jobs_of(boards):
     return [(k, b, e) for k, b in enumerate(boards) for e in excitations(b)]
 
 
-def run_batch(cem, boards, workers):
+def run_batch(cem, boards, workers, pairs=False):
     """Every excitation of `boards` on `workers` host threads (1 = plain one-at-a-time).
+    pairs: the two excitations of each 2-port board run as one pair (solve_pair).
     Returns (S per board, wall seconds, cell updates)."""
-    jobs = jobs_of(boards)
+    jobs = []
+    for k, b in enumerate(boards):
+        if pairs and pairable(b):
+            jobs.append((k, b, None))
+        else:
+            jobs += [(k, b, e) for e in excitations(b)]
+
+    def one(j):
+        _, b, e = j
+        if e is None:
+            vis, n = solve_pair(cem, b)
+            return vis, n, 2
+        vi, n = solve(cem, b, e)
+        return [vi], n, 1
+
     t0 = time.perf_counter()
     if workers <= 1:
-        res = [solve(cem, b, e) for _, b, e in jobs]
+        res = [one(j) for j in jobs]
     else:
         with ThreadPoolExecutor(max_workers=workers) as ex:
-            res = list(ex.map(lambda j: solve(cem, j[1], j[2]), jobs))
+            res = list(ex.map(one, jobs))
     wall = time.perf_counter() - t0
     per = [[] for _ in boards]
     upd = 0
-    for (k, b, _), (vi, n) in zip(jobs, res):
-        per[k].append(vi)
-        upd += cells(b) * n
+    for (k, b, _), (vis, n, nrhs) in zip(jobs, res):
+        per[k] += vis
+        upd += cells(b) * n * nrhs
     return [assemble(cem, b, p) for b, p in zip(boards, per)], wall, upd
 
 
@@ -254,8 +296,8 @@ class Server:
     running ones finish and give their memory back).  A board's future resolves with its S
     matrix and the time from its arrival to the S matrix."""
 
-    def __init__(self, cem, workers, budget_bytes):
-        self.cem, self.budget = cem, budget_bytes
+    def __init__(self, cem, workers, budget_bytes, pairs=False):
+        self.cem, self.budget, self.pairs = cem, budget_bytes, pairs
         self.pool = ThreadPoolExecutor(max_workers=workers)
         self.cv = threading.Condition()
         self.in_use = 0
@@ -281,14 +323,21 @@ class Server:
             self.cv.notify_all()
 
     def _job(self, b, e):
-        need = estimate_bytes(b)
+        """One excitation (e), or both of a 2-port board as a pair (e None; twice the
+        footprint).  Returns ([(V, I) per excitation], completion time)."""
+        nrhs = 2 if e is None else 1
+        need = nrhs * estimate_bytes(b)
         self._admit(need)
-        vi, n = None, 0
+        vis, n = None, 0
         try:
-            vi, n = solve(self.cem, b, e)
+            if e is None:
+                vis, n = solve_pair(self.cem, b)
+            else:
+                vi, n = solve(self.cem, b, e)
+                vis = [vi]
         finally:
-            self._release(need, cells(b) * n if vi is not None else 0)
-        return vi, time.perf_counter()
+            self._release(need, nrhs * cells(b) * n if vis is not None else 0)
+        return vis, time.perf_counter()
 
     def submit(self, b):
         """Queue every excitation of board b; returns a callable that blocks for
@@ -296,13 +345,16 @@ class Server:
         if estimate_bytes(b) > self.budget:
             raise ValueError("board larger than the device budget")
         t_arr = time.perf_counter()
-        futs = [self.pool.submit(self._job, b, e) for e in excitations(b)]
+        if self.pairs and pairable(b) and 2 * estimate_bytes(b) <= self.budget:
+            futs = [self.pool.submit(self._job, b, None)]
+        else:
+            futs = [self.pool.submit(self._job, b, e) for e in excitations(b)]
         done = {}
 
         def result():
             if not done:
                 res = [f.result() for f in futs]
-                vis = [r[0] for r in res]
+                vis = [v for r in res for v in r[0]]
                 done["r"] = (assemble(self.cem, b, vis), max(r[1] for r in res) - t_arr)
             return done["r"]
 
@@ -312,12 +364,12 @@ class Server:
         self.pool.shutdown(wait=True)
 
 
-def serve(cem, boards, rate, workers, budget_bytes, seed=5000):
-    """Poisson arrivals (`rate` boards per second) of `boards`.  Returns (latencies s, wall s,
-    S per board, cell updates)."""
+def serve(cem, boards, rate, workers, budget_bytes, seed=5000, pairs=False):
+    """Poisson arrivals (`rate` boards per second) of `boards`; pairs: 2-port boards run
+    both excitations as one pair.  Returns (latencies s, wall s, S per board, cell updates)."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(boards))
-    srv = Server(cem, workers, budget_bytes)
+    srv = Server(cem, workers, budget_bytes, pairs=pairs)
     t0 = time.perf_counter()
     handles = []
     t_next = t0

@@ -23,6 +23,27 @@ CEM_API int cem_amd_step_device(cem_sim *s, long n);
  * already fixed), or -1 with cem_last_error() set. */
 CEM_API long cem_amd_tune(cem_sim *s);
 
+/* Both excitations of a 2-port as right-hand sides of shared launches.
+ *
+ * cem_amd_clone_excitation: a new simulation with a's plan for another excited port
+ * of the same board.  Every set-up array is copied (no set-up step runs again; mode-port
+ * cross-sections keep their solved state); its fields start from zero at step 0.  The
+ * excitation moves to lumped port `port` with a's Gaussian (amplitude, t0, tau), or,
+ * when a drives a mode port, to mode port `port` with a's modal waveform.  Destroy it
+ * with cem_destroy.  Returns NULL with cem_last_error() set if a is not finalized or the
+ * port does not exist.
+ *
+ * cem_amd_step_pair: advance a and its clone b (same step) n steps.  The H and E updates
+ * of both run as one launch each per step (b's tiles next to a's, reading a's
+ * coefficient classes, Debye tables and CPML profiles once for both); post-phase and
+ * probe kernels run per simulation; chunks of steps replay as one graph on a's stream.
+ * Each simulation's fields, spectra and states are bit-identical to stepping it alone.
+ * Host mirrors are not refreshed (as cem_amd_step_device).  Returns 0, or -1 with
+ * cem_last_error() set when b is not a clone of a, the two are at different steps, or
+ * either runs the one-pass step. */
+CEM_API cem_sim *cem_amd_clone_excitation(const cem_sim *a, int port);
+CEM_API int cem_amd_step_pair(cem_sim *a, cem_sim *b, long n);
+
 /* Queue the running DFT of every sample buffered since the last flush on the
  * simulation's stream (the engine otherwise flushes every 512 steps and when
  * spectra are read).  Returns 0, or -1 if the simulation has not run on the

@@ -69,6 +69,24 @@ def test_results_identical_alone_and_in_the_mix(cem, gpu_available):
     print("served %d boards in %.2f s, p50 %.3f s, max %.3f s" % (len(lat), wall, np.median(lat), lat.max()))
 
 
+def test_pairs_identical_to_single_excitations(cem, gpu_available):
+    """2-port boards solved as pairs (both excitations as right-hand sides of shared
+    launches, one set-up) give the S matrices of their excitations solved one at a time,
+    bit for bit, batched and served."""
+    boards = _mix()
+    alone = [serve.run_batch(cem, [b], 1)[0][0] for b in boards]
+    print_case("pairs, batched on 3 threads")
+    paired, _, _ = serve.run_batch(cem, boards, 3, pairs=True)
+    for x, y in zip(alone, paired):
+        assert np.array_equal(x, y)
+    print_case("pairs, served, Poisson arrivals")
+    budget = 2 * max(2 * serve.estimate_bytes(b) for b in boards)
+    lat, wall, served, _ = serve.serve(cem, boards + boards[:1], rate=50.0, workers=3, budget_bytes=budget,
+                                       seed=5001, pairs=True)
+    for x, y in zip(alone + alone[:1], served):
+        assert np.array_equal(x, y)
+
+
 def test_full_two_port_touchstone_matches_reference(cem, ref, gpu_available):
     print_case("full 2-port, mode ports, lossy gcpw, 3000 steps per excitation")
     from courant_amd import scenes as S
