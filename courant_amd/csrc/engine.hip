@@ -323,6 +323,23 @@ __device__ __forceinline__ void zerow(float (&x)[W]) {
     for (int e = 0; e < W; e++) x[e] = 0.f;
 }
 
+// The same at a 32-bit byte offset from a base pointer: the load or store takes the
+// base in scalar registers and the offset in one vector register, with no 64-bit
+// address arithmetic per access.
+template <int W>
+__device__ __forceinline__ void ldb(const float *b, unsigned off, float (&x)[W]) {
+    ldw<W>((const float *)((const char *)b + off), x);
+}
+
+template <int W>
+__device__ __forceinline__ void stb(float *b, unsigned off, const float (&x)[W]) {
+    stw<W>((float *)((char *)b + off), x);
+}
+
+__device__ __forceinline__ float lb(const float *b, unsigned off) {
+    return *(const float *)((const char *)b + off);
+}
+
 // One H component at one point (sim.c:579-606) with the CPML recursions of the slabs
 // that hold it (sim.c:680-754), from the point's E differences, with the reference's
 // expression in its order.  Every kernel that updates H calls these, so all of them
@@ -436,55 +453,59 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
         bz[e] = inz[e] ? a.pml.b[2][k] : 0.f;
         cz[e] = inz[e] ? a.pml.c[2][k] : 0.f;
     }
-    const long long kn = k0 + W < g.kp ? W : 0;   // offset of the next group's first point
-    long long q = (long long)ib * g.si + (long long)j * g.sj + k0;
+    // byte offsets of the group within a component array (32 bits: 12 N bytes < 4 GiB, set-up
+    // checks), added to scalar base pointers by the loads themselves
+    const unsigned si4 = 4u * (unsigned)g.si, sj4 = 4u * (unsigned)g.sj;
+    const unsigned kn = k0 + W < g.kp ? 4u * W : 0u;   // on to the next group's first point
+    unsigned q = 4u * ((unsigned)ib * (unsigned)g.si + (unsigned)j * (unsigned)g.sj + (unsigned)k0);
     float ey[W], ez[W];
-    ldw<W>(Ey + q, ey);
-    ldw<W>(Ez + q, ez);
-    for (int i = ib; i < ie; i++, q += g.si) {
+    ldb<W>(Ey, q, ey);
+    ldb<W>(Ez, q, ez);
+    for (int i = ib; i < ie; i++, q += si4) {
         const bool iin = i < g.nx;
         float ex[W], ezj[W], exj[W], eyi[W], ezi[W], hxv[W], hyv[W], hzv[W];
-        ldw<W>(Ex + q, ex);
-        const float eyk = Ey[q + kn], exk = Ex[q + kn];
+        ldb<W>(Ex, q, ex);
+        const float eyk = lb(Ey, q + kn), exk = lb(Ex, q + kn);
         if (jin) {
-            ldw<W>(Ez + q + g.sj, ezj);
-            ldw<W>(Ex + q + g.sj, exj);
+            ldb<W>(Ez, q + sj4, ezj);
+            ldb<W>(Ex, q + sj4, exj);
         } else {
             zerow<W>(ezj);
             zerow<W>(exj);
         }
         if (iin) {
-            ldw<W>(Ey + q + g.si, eyi);
-            ldw<W>(Ez + q + g.si, ezi);
+            ldb<W>(Ey, q + si4, eyi);
+            ldb<W>(Ez, q + si4, ezi);
         } else {
             zerow<W>(eyi);
             zerow<W>(ezi);
         }
         const float hx = iin ? a.hs[0][i] : 0.f;
         const int sx = a.has_pml && iin ? a.pml.slot[0][i] : -1;
-        ldw<W>(Hx + q, hxv);
-        ldw<W>(Hy + q, hyv);
-        ldw<W>(Hz + q, hzv);
+        ldb<W>(Hx, q, hxv);
+        ldb<W>(Hy, q, hyv);
+        ldb<W>(Hz, q, hzv);
         // psi of the slabs that hold this group (x: the plane, y: the row, z: the lane)
         float py0[W], py2[W], px1[W], px2[W], pz0[W], pz1[W];
         float bx = 0.f, cx = 0.f;
-        long long oy = 0, ox = 0, oz = 0;
+        unsigned oy = 0, ox = 0, oz = 0;   // byte offsets in the psi arrays
         if (sy >= 0) {
-            oy = ((long long)sy * (g.nx + 1) + i) * g.kp + k0;
-            ldw<W>(a.psi[0][0] + oy, py0);
-            ldw<W>(a.psi[2][1] + oy, py2);
+            oy = 4u * (((unsigned)sy * (unsigned)(g.nx + 1) + (unsigned)i) * (unsigned)g.kp + (unsigned)k0);
+            ldb<W>(a.psi[0][0], oy, py0);
+            ldb<W>(a.psi[2][1], oy, py2);
         }
         if (sx >= 0) {
-            ox = (long long)sx * g.si + (long long)j * g.sj + k0;
-            ldw<W>(a.psi[1][1] + ox, px1);
-            ldw<W>(a.psi[2][0] + ox, px2);
+            ox = (unsigned)sx * si4 + (unsigned)j * sj4 + 4u * (unsigned)k0;
+            ldb<W>(a.psi[1][1], ox, px1);
+            ldb<W>(a.psi[2][0], ox, px2);
             bx = a.pml.b[0][i];
             cx = a.pml.c[0][i];
         }
         if (zo >= 0) {
-            oz = (((long long)i * (g.ny + 1) + j) * a.pml.zg + zo) * 4 + (k0 & 3);
-            ldw<W>(a.psi[0][1] + oz, pz0);
-            ldw<W>(a.psi[1][0] + oz, pz1);
+            oz = 4u * ((((unsigned)i * (unsigned)(g.ny + 1) + (unsigned)j) * (unsigned)a.pml.zg + (unsigned)zo) * 4u +
+                       (unsigned)(k0 & 3));
+            ldb<W>(a.psi[0][1], oz, pz0);
+            ldb<W>(a.psi[1][0], oz, pz1);
         }
 #pragma unroll
         for (int e = 0; e < W; e++) {
@@ -493,21 +514,21 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
                     ezi[e] - ez[e], eyi[e] - ey[e], hx, hy, hz[e], jin, kin[e], iin, sy, by, cy, sx, bx, cx, inz[e],
                     bz[e], cz[e], a.dth, py0[e], py2[e], px1[e], px2[e], pz0[e], pz1[e]);
         }
-        if (jin) stw<W>(Hx + q, hxv);
-        if (iin) stw<W>(Hy + q, hyv);
-        if (iin && jin) stw<W>(Hz + q, hzv);
+        if (jin) stb<W>(Hx, q, hxv);
+        if (iin) stb<W>(Hy, q, hyv);
+        if (iin && jin) stb<W>(Hz, q, hzv);
         // psi entries of points not updated are stored back unchanged
         if (sy >= 0) {
-            stw<W>(a.psi[0][0] + oy, py0);
-            stw<W>(a.psi[2][1] + oy, py2);
+            stb<W>(a.psi[0][0], oy, py0);
+            stb<W>(a.psi[2][1], oy, py2);
         }
         if (sx >= 0) {
-            stw<W>(a.psi[1][1] + ox, px1);
-            stw<W>(a.psi[2][0] + ox, px2);
+            stb<W>(a.psi[1][1], ox, px1);
+            stb<W>(a.psi[2][0], ox, px2);
         }
         if (zo >= 0) {
-            stw<W>(a.psi[0][1] + oz, pz0);
-            stw<W>(a.psi[1][0] + oz, pz1);
+            stb<W>(a.psi[0][1], oz, pz0);
+            stb<W>(a.psi[1][0], oz, pz1);
         }
 #pragma unroll
         for (int e = 0; e < W; e++) {
@@ -653,46 +674,49 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
         cz[e] = inz[e] ? a.pml.c[2][k] : 0.f;
     }
     const bool box_jk = NP > 0 && j >= a.bj0 && j < a.bj1 && k0 >= 4 * a.bg0 && k0 < 4 * a.bg1;
-    const long long km = k0 ? -1 : 0;   // offset of the previous group's last point
-    long long q = (long long)ib * g.si + (long long)j * g.sj + k0;
+    // byte offsets of the group within a component array (32 bits: 12 N bytes < 4 GiB, set-up
+    // checks), added to scalar base pointers by the loads themselves
+    const unsigned si4 = 4u * (unsigned)g.si, sj4 = 4u * (unsigned)g.sj;
+    const unsigned km = k0 ? 4u : 0u;   // back to the previous group's last point
+    unsigned q = 4u * ((unsigned)ib * (unsigned)g.si + (unsigned)j * (unsigned)g.sj + (unsigned)k0);
     float hym[W], hzm[W];
     if (ib >= 1) {
-        ldw<W>(Hy + q - g.si, hym);
-        ldw<W>(Hz + q - g.si, hzm);
+        ldb<W>(Hy, q - si4, hym);
+        ldb<W>(Hz, q - si4, hzm);
     } else {
         zerow<W>(hym);
         zerow<W>(hzm);
     }
-    for (int i = ib; i < ie; i++, q += g.si) {
+    for (int i = ib; i < ie; i++, q += si4) {
         const bool ib_ = i >= 1 && i < g.nx;
         // Debye currents of the group first, every component and pole at once, so
         // that their loads are in flight together with the field loads
         const bool in_box = kCmp && NP > 0 && box_jk && i >= a.bi0 && i < a.bi1;
         constexpr int NPE = NP > 0 ? NP : 1;          // poles per point
         float jv[3][W * NPE];                         // [comp][point e * NPE + pole]
-        float *Jl = nullptr;
-        long long bkp = 0;                            // pole stride in the box row
+        const unsigned bkp = 4u * (unsigned)(a.bg1 - a.bg0);   // pole stride in the box row (floats)
+        unsigned jo = 0;                              // byte offset of the group in its (i, j) box row
         if (in_box) {
-            bkp = 4 * (a.bg1 - a.bg0);
-            Jl = a.J + ((long long)(i - a.bi0) * (a.bj1 - a.bj0) + (j - a.bj0)) * 3 * NPE * bkp + (k0 - 4 * a.bg0);
+            jo = 4u * (((unsigned)(i - a.bi0) * (unsigned)(a.bj1 - a.bj0) + (unsigned)(j - a.bj0)) * 3u * NPE * bkp +
+                       (unsigned)(k0 - 4 * a.bg0));
 #pragma unroll
             for (int c = 0; c < 3; c++)
 #pragma unroll
                 for (int pp = 0; pp < NPE; pp++) {
                     float t[W];
-                    ldw<W>(Jl + (c * NPE + pp) * bkp, t);
+                    ldb<W>(a.J + (c * NPE + pp) * bkp, jo, t);
 #pragma unroll
                     for (int e = 0; e < W; e++) jv[c][e * NPE + pp] = t[e];
                 }
         }
         float hx[W], hy[W], hz[W], hxj[W], hzj[W];
-        ldw<W>(Hx + q, hx);
-        ldw<W>(Hy + q, hy);
-        ldw<W>(Hz + q, hz);
-        const float hxk = Hx[q + km], hyk = Hy[q + km];
+        ldb<W>(Hx, q, hx);
+        ldb<W>(Hy, q, hy);
+        ldb<W>(Hz, q, hz);
+        const float hxk = lb(Hx, q - km), hyk = lb(Hy, q - km);
         if (j >= 1) {
-            ldw<W>(Hx + q - g.sj, hxj);
-            ldw<W>(Hz + q - g.sj, hzj);
+            ldb<W>(Hx, q - sj4, hxj);
+            ldb<W>(Hz, q - sj4, hzj);
         } else {
             zerow<W>(hxj);
             zerow<W>(hzj);
@@ -703,17 +727,17 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
         float ca[3][W], cb[3][W];
         if (kCmp) {
             if constexpr (W == 4) {
-                const uint4 c4 = *(const uint4 *)(a.cidx + q);
+                const uint4 c4 = *(const uint4 *)((const char *)a.cidx + q);
                 cls[0] = c4.x;
                 cls[1 % W] = c4.y;
                 cls[2 % W] = c4.z;
                 cls[3 % W] = c4.w;
             } else if constexpr (W == 2) {
-                const uint2 c2 = *(const uint2 *)(a.cidx + q);
+                const uint2 c2 = *(const uint2 *)((const char *)a.cidx + q);
                 cls[0] = c2.x;
                 cls[1 % W] = c2.y;
             } else {
-                cls[0] = a.cidx[q];
+                cls[0] = *(const unsigned *)((const char *)a.cidx + q);
             }
 #pragma unroll
             for (int e = 0; e < W; e++)
@@ -726,14 +750,14 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
         } else {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                ldw<W>(a.ca + c * g.N + q, ca[c]);
-                ldw<W>(a.cb + c * g.N + q, cb[c]);
+                ldb<W>(a.ca + c * g.N, q, ca[c]);
+                ldb<W>(a.cb + c * g.N, q, cb[c]);
             }
         }
         float ex0[W], ey0[W], ez0[W];
-        ldw<W>(Ex + q, ex0);
-        ldw<W>(Ey + q, ey0);
-        ldw<W>(Ez + q, ez0);
+        ldb<W>(Ex, q, ex0);
+        ldb<W>(Ey, q, ey0);
+        ldb<W>(Ez, q, ez0);
         float exn[W], eyn[W], ezn[W];
 #pragma unroll
         for (int e = 0; e < W; e++) {
@@ -743,23 +767,24 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
         }
         float py0[W], py2[W], px1[W], px2[W], pz0[W], pz1[W];
         float bx = 0.f, cx = 0.f;
-        long long oy = 0, ox = 0, oz = 0;
+        unsigned oy = 0, ox = 0, oz = 0;   // byte offsets in the psi arrays
         if (sy >= 0) {
-            oy = ((long long)sy * (g.nx + 1) + i) * g.kp + k0;
-            ldw<W>(a.psi[0][0] + oy, py0);
-            ldw<W>(a.psi[2][1] + oy, py2);
+            oy = 4u * (((unsigned)sy * (unsigned)(g.nx + 1) + (unsigned)i) * (unsigned)g.kp + (unsigned)k0);
+            ldb<W>(a.psi[0][0], oy, py0);
+            ldb<W>(a.psi[2][1], oy, py2);
         }
         if (sx >= 0) {
-            ox = (long long)sx * g.si + (long long)j * g.sj + k0;
-            ldw<W>(a.psi[1][1] + ox, px1);
-            ldw<W>(a.psi[2][0] + ox, px2);
+            ox = (unsigned)sx * si4 + (unsigned)j * sj4 + 4u * (unsigned)k0;
+            ldb<W>(a.psi[1][1], ox, px1);
+            ldb<W>(a.psi[2][0], ox, px2);
             bx = a.pml.b[0][i];
             cx = a.pml.c[0][i];
         }
         if (zo >= 0) {
-            oz = (((long long)i * (g.ny + 1) + j) * a.pml.zg + zo) * 4 + (k0 & 3);
-            ldw<W>(a.psi[0][1] + oz, pz0);
-            ldw<W>(a.psi[1][0] + oz, pz1);
+            oz = 4u * ((((unsigned)i * (unsigned)(g.ny + 1) + (unsigned)j) * (unsigned)a.pml.zg + (unsigned)zo) * 4u +
+                       (unsigned)(k0 & 3));
+            ldb<W>(a.psi[0][1], oz, pz0);
+            ldb<W>(a.psi[1][0], oz, pz1);
         }
 #pragma unroll
         for (int e = 0; e < W; e++) {
@@ -784,24 +809,24 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
                     float t[W];
 #pragma unroll
                     for (int e = 0; e < W; e++) t[e] = jv[c][e * NPE + pp];
-                    stw<W>(Jl + (c * NPE + pp) * bkp, t);
+                    stb<W>(a.J + (c * NPE + pp) * bkp, jo, t);
                 }
             }
         }
-        stw<W>(Ex + q, exn);
-        if (j < g.ny) stw<W>(Ey + q, eyn);
-        stw<W>(Ez + q, ezn);
+        stb<W>(Ex, q, exn);
+        if (j < g.ny) stb<W>(Ey, q, eyn);
+        stb<W>(Ez, q, ezn);
         if (sy >= 0) {
-            stw<W>(a.psi[0][0] + oy, py0);
-            stw<W>(a.psi[2][1] + oy, py2);
+            stb<W>(a.psi[0][0], oy, py0);
+            stb<W>(a.psi[2][1], oy, py2);
         }
         if (sx >= 0) {
-            stw<W>(a.psi[1][1] + ox, px1);
-            stw<W>(a.psi[2][0] + ox, px2);
+            stb<W>(a.psi[1][1], ox, px1);
+            stb<W>(a.psi[2][0], ox, px2);
         }
         if (zo >= 0) {
-            stw<W>(a.psi[0][1] + oz, pz0);
-            stw<W>(a.psi[1][0] + oz, pz1);
+            stb<W>(a.psi[0][1], oz, pz0);
+            stb<W>(a.psi[1][0], oz, pz1);
         }
 #pragma unroll
         for (int e = 0; e < W; e++) {
@@ -2197,8 +2222,10 @@ int upload_plan(cem_sim *s, Device *D) {
     const size_t N = s->N;
     const size_t Nd = (size_t)g.N;
     hipStream_t st = D->st;
-    if (3 * Nd > 0xffffffffull) {
-        set_error("grid too large for 32-bit edge indices");
+    // edge indices (3 N) and the stencils' byte offsets within one component array (4 N)
+    // are 32-bit
+    if (4 * Nd > 0xffffffffull) {
+        set_error("grid too large for 32-bit offsets (more than 1.07e9 grid points)");
         return -1;
     }
     // device index of a reference point index (same component)
@@ -2295,6 +2322,7 @@ int upload_plan(cem_sim *s, Device *D) {
         bk1 = (bk1 + 3) / 4 * 4;
         const size_t box = (size_t)(bi1 - bi0) * (bj1 - bj0) * (bk1 - bk0);
         if (3 * box > 8 * (size_t)nd + ((size_t)1 << 22)) fused = false;
+        if (4 * 3 * (size_t)np * box > 0xffffffffull) fused = false;   // 32-bit byte offsets in the box
     }
     // coefficient classes, with the Debye rows of the fused edges
     std::vector<unsigned> cls;
