@@ -189,3 +189,29 @@ def test_debye_material_without_poles_is_the_plain_dielectric(cem, ref):
         for key in ("ca", "cb"):
             for c in range(3):
                 assert np.array_equal(a[key][c], b[key][c])
+
+
+@pytest.mark.parametrize("kind", ["lumped", "mode"])
+def test_excitation_clone_equals_a_fresh_build(cem, kind):
+    """cem_amd_clone_excitation copies a finalized plan for the other port: its plan equals the
+    same scene built and finalized from scratch with that port driven (mode-source profile to
+    1e-10 of its maximum), it starts at step 0, and a clone of a clone drives the first port
+    again."""
+    p = SMALL_LUMPED if kind == "lumped" else SMALL_MODE
+    a = S.gcpw(cem, **p)[0]
+    b = a.clone_excitation(1)
+    c = S.gcpw(cem, **dict(p, excite=2))[0]
+    tol = {("mode_source", "w")}
+    assert_plan_equal(b.plan(), c.plan(), float_tol_keys=tol)
+    assert b.steps == 0 and b.dt == a.dt
+    assert_plan_equal(b.clone_excitation(0).plan(), a.plan(), float_tol_keys=tol)
+
+
+def test_excitation_clone_errors(cem):
+    a = S.gcpw(cem, **SMALL_LUMPED)[0]
+    with pytest.raises(L.CourantError, match="bad port"):
+        a.clone_excitation(2)
+    x = np.linspace(0, 1e-3, 5)
+    s = cem.create_sim(x, x, x)
+    with pytest.raises(L.CourantError, match="not finalized"):
+        s.clone_excitation(0)
