@@ -1477,12 +1477,9 @@ __device__ __forceinline__ int find_edge(const int *list, int n, int t) {
     return lo;
 }
 
-__global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const PostParams &pp = *a.pp;
-    const long long n = a.ctl->step;
-    float vs, gm;
-    waveforms(a.wave, n, &vs, &gm);
+// the post-phase physics of listed edge t (dispersive, then surface-only, then
+// source-only edges) at source waveform values vs (lumped) and gm (modal)
+__device__ __forceinline__ void post_edge(const PostArgs &a, const PostParams &pp, int t, float vs, float gm) {
     if (t < a.ndisp) {                       // listed dispersive edge
         const unsigned q = a.d_idx[t];
         const unsigned m = a.d_mat[t];
@@ -1523,6 +1520,35 @@ __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
         const unsigned q = a.x_idx[u];
         const int x = a.x_src[u];
         a.E[q] += a.wl[x] * vs + a.wm[x] * gm;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
+    const PostParams &pp = *a.pp;
+    float vs, gm;
+    waveforms(a.wave, a.ctl->step, &vs, &gm);
+    post_edge(a, pp, blockIdx.x * kBlock + threadIdx.x, vs, gm);
+}
+
+// Short post-phase lists (a small board, or lumped sources only): one workgroup runs every
+// listed edge and then the step's probe samples and the counter advance (k_gather's work),
+// saving a launch a step.  The workgroup's barrier orders its E writes before its sample
+// reads (one workgroup: no device-wide fence).
+constexpr int kSmallPost = 4 * kGatherBlock;   // most listed edges for the one-workgroup form
+
+__global__ void __launch_bounds__(kGatherBlock) k_post_gather(PostArgs a, GatherArgs g) {
+    const PostParams &pp = *a.pp;
+    float vs, gm;
+    waveforms(a.wave, a.ctl->step, &vs, &gm);
+    const int n = a.ndisp + a.nsbo + a.nxs;
+    for (int t = threadIdx.x; t < n; t += kGatherBlock) post_edge(a, pp, t, vs, gm);
+    __syncthreads();
+    const int row = g.ctl->row;
+    gather_samples(g, row, threadIdx.x >> 6, kGatherBlock / 64);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        g.ctl->row = row + 1;
+        g.ctl->step += 1;
     }
 }
 
@@ -2955,6 +2981,12 @@ int post_blocks(cem_sim *s, Device *D) {
     return (D->nlisted + D->nsbo + D->nxs + kBlock - 1) / kBlock;
 }
 
+// post lists short enough for k_post_gather (post-phase edges and probe samples in one launch)
+bool small_post(const Device *D) {
+    const int n = D->nlisted + D->nsbo + D->nxs;
+    return n > 0 && n <= kSmallPost;
+}
+
 // The device counters follow the host's step and row; after a host-side change
 // of either (resume, sample push, set-up) they are written before the next step.
 void sync_ctl(cem_sim *s, Device *D) {
@@ -2968,12 +3000,13 @@ void sync_ctl(cem_sim *s, Device *D) {
 // bookkeeping and the DFT flush when the series is full.
 void launch_post(cem_sim *s, Device *D) {
     sync_ctl(s, D);
-    if (const int nb = post_blocks(s, D)) {
-        PostArgs a = post_args(s, D, D->par);
-        launch(D, D->tP, k_post, dim3(nb), dim3(kBlock), a);
-    }
     GatherArgs g = gather_args(s, D, 0, D->par);
-    launch(D, D->tG, k_gather, dim3(1), dim3(kGatherBlock), g);
+    if (small_post(D)) {
+        launch(D, D->tP, k_post_gather, dim3(1), dim3(kGatherBlock), post_args(s, D, D->par), g);
+    } else {
+        if (const int nb = post_blocks(s, D)) launch(D, D->tP, k_post, dim3(nb), dim3(kBlock), post_args(s, D, D->par));
+        launch(D, D->tG, k_gather, dim3(1), dim3(kGatherBlock), g);
+    }
     if (D->rows == 0) D->row0 = s->step;
     D->rows++;
     s->step++;
@@ -3139,11 +3172,19 @@ int run_chunk(cem_sim *s, Device *D) {
                 chain.add(hk, dim3(gh * (1 + ha.pair)), dim3(kBlock), 0, ha, 0, nx + 1, D->iseg_h, nbh);
                 chain.add(ek, dim3(ge * (1 + ea.pair)), dim3(kBlock), 0, ea, 0, nx, D->iseg_e, nbe);
             }
-            if (geo[2]) chain.add(k_post, dim3(geo[2]), dim3(kBlock), 0, pa[o]);
-            chain.add(k_gather, dim3(1), dim3(kGatherBlock), 0, ga[o]);
+            if (small_post(D)) {
+                chain.add(k_post_gather, dim3(1), dim3(kGatherBlock), 0, pa[o], ga[o]);
+            } else {
+                if (geo[2]) chain.add(k_post, dim3(geo[2]), dim3(kBlock), 0, pa[o]);
+                chain.add(k_gather, dim3(1), dim3(kGatherBlock), 0, ga[o]);
+            }
             if (P) {
-                if (nbp) chain.add(k_post, dim3(nbp), dim3(kBlock), 0, pb);
-                chain.add(k_gather, dim3(1), dim3(kGatherBlock), 0, gb);
+                if (small_post(P)) {
+                    chain.add(k_post_gather, dim3(1), dim3(kGatherBlock), 0, pb, gb);
+                } else {
+                    if (nbp) chain.add(k_post, dim3(nbp), dim3(kBlock), 0, pb);
+                    chain.add(k_gather, dim3(1), dim3(kGatherBlock), 0, gb);
+                }
             }
         }
         if (!chain.instantiate(&D->gexec)) {
