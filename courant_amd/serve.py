@@ -218,8 +218,20 @@ def solve_pair(cem, b):
     return out, n
 
 
+MALL_BYTES = 256 << 20   # the MI355X's last-level cache (Infinity Cache)
+
+
 def pairable(b):
-    return b.get("nport", 2) == 2 and b.get("kind") != "board"
+    """A 2-port lumped board whose excitations pay off as a pair.  A pair steps two field
+    sets through every launch, so its working set is twice an excitation's: a board whose
+    excitation fits the 256 MiB last-level cache but whose pair does not loses that
+    residency (a 4.7 M-cell board: a pair step took 2.8x an excitation's step, a 38.8
+    M-point board 1.98x).  Pairs are formed when both excitations fit the cache together
+    (with a quarter to spare) or when one alone does not fit it anyway."""
+    if b.get("nport", 2) != 2 or b.get("kind") == "board":
+        return False
+    need = estimate_bytes(b)
+    return 2 * need <= MALL_BYTES * 3 // 4 or need >= MALL_BYTES
 
 
 def excitations(b):
