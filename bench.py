@@ -359,6 +359,7 @@ def main():
     kb = kernel_bytes(pc)
     b8d = step_bytes(pc)
     shape = sim.launch_shape()
+    lanes = sim.lane_widths()
     rows, planes = sim.step_shape()
     shape = (shape[0], shape[1], shape[2], planes)
     copy_gbs = cem.lib.cem_amd_copy_bandwidth(2 << 30, 10)
@@ -406,6 +407,7 @@ def main():
                    "profiled_steps": nprof,
                    "one_pass_step": {"rows_per_workgroup": rows, "xplanes_per_workgroup": planes} if rows else None,
                    "iplanes_per_thread_split": None if rows else {"H": shape[0], "E": shape[1]},
+                   "kpoints_per_thread_split": None if rows else {"H": lanes[0], "E": lanes[1]},
                    "autotuned": shape[2]},
     }
     sim.close()

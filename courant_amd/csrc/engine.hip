@@ -2002,6 +2002,7 @@ struct Device {
     // on the first steps unless fixed by COURANT_ISEG_H / COURANT_ISEG_E / COURANT_ISEG_S
     int iseg_h = kIseg, iseg_e = kIseg;
     int wh = 4, we = 1;          // lane widths (k points per thread) of the H and E updates
+    bool wh_fixed = false;       // H lane width set by COURANT_LANES_H (else chosen by the tuner)
     bool tuned = false;
     // timing
     bool profiling = false;
@@ -3246,6 +3247,7 @@ int cem::dev_ensure(cem_sim *s) {
         if (const char *v = std::getenv(pr.first)) {
             const int w = std::atoi(v);
             *pr.second = w == 1 || w == 2 ? w : 4;
+            if (pr.second == &D->wh) D->wh_fixed = true;
         }
     if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v)), D->tuned = true;
     if (const char *v = std::getenv("COURANT_ISEG_E")) D->iseg_e = std::max(1, std::atoi(v)), D->tuned = true;
@@ -3336,16 +3338,18 @@ constexpr int kTuneCandS[] = {4, 8, 16};   // one-pass step: x planes per workgr
 constexpr int kTuneNs = sizeof kTuneCandS / sizeof kTuneCandS[0];
 constexpr int kTuneSteps = 2;
 constexpr int kTuneRounds = 2;
-constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * kTuneNc * kTuneNc * (1 + kTuneSteps);
+constexpr int kTuneLanesH[] = {4, 2, 1};   // k points per lane of the H update
+constexpr int kTuneNl = sizeof kTuneLanesH / sizeof kTuneLanesH[0];
+constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * (kTuneNl + kTuneNc * kTuneNc) * (1 + kTuneSteps);
 constexpr long kTuneCostStep = 1 + (long)kTuneRounds * kTuneNs * (1 + kTuneSteps);
 constexpr long kTuneCost = kTuneCostSplit > kTuneCostStep ? kTuneCostSplit : kTuneCostStep;
 
 std::mutex g_tune_mu;
-std::map<std::array<long long, 10>, std::array<int, 3>> g_tune;
+std::map<std::array<long long, 10>, std::array<int, 4>> g_tune;
 
 long autotune(cem_sim *s, Device *D, long budget) {
     const std::array<long long, 10> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
-                                           (long long)D->fused, D->wh, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr), (long long)D->tj * 4096 + D->kw};
+                                           (long long)D->fused, D->wh_fixed ? D->wh : 0, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr), (long long)D->tj * 4096 + D->kw};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
@@ -3353,6 +3357,7 @@ long autotune(cem_sim *s, Device *D, long budget) {
             D->iseg_h = it->second[0];
             D->iseg_e = it->second[1];
             D->iseg_s = it->second[2];
+            if (!D->onepass) D->wh = it->second[3];
             D->tuned = true;
             return 0;
         }
@@ -3395,6 +3400,31 @@ long autotune(cem_sim *s, Device *D, long budget) {
             std::fprintf(stderr, " -> %d\n", D->iseg_s);
         }
     } else {
+        // the H update's lane width first, at the default planes per thread: 4 k points per
+        // lane (128-bit accesses) on large grids, fewer (more lanes in flight) on small ones
+        double tl[kTuneNl];
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int c = 0; c < kTuneNl; c++) {
+                D->wh = D->wh_fixed ? D->wh : kTuneLanesH[c];
+                one_step();             // settling step, untimed
+                D->profiling = true;
+                reset({&D->tH});
+                for (int st = 0; st < kTuneSteps; st++) one_step();
+                D->tH.fold();
+                D->profiling = false;
+                tl[c] = r ? std::min(tl[c], D->tH.total_ms) : D->tH.total_ms;
+            }
+        if (!D->wh_fixed) {
+            int b = 0;
+            for (int c = 0; c < kTuneNl; c++)
+                if (tl[c] < tl[b]) b = c;
+            D->wh = kTuneLanesH[b];
+            if (dbg) {
+                std::fprintf(stderr, "courant autotune H update us per step (k points per lane):");
+                for (int c = 0; c < kTuneNl; c++) std::fprintf(stderr, " %d: %.1f", kTuneLanesH[c], tl[c] / kTuneSteps * 1e3);
+                std::fprintf(stderr, " -> %d\n", D->wh);
+            }
+        }
         double t[kTuneNc][kTuneNc];
         for (int r = 0; r < kTuneRounds; r++)
             for (int ch = 0; ch < kTuneNc; ch++)
@@ -3430,7 +3460,7 @@ long autotune(cem_sim *s, Device *D, long budget) {
     reset({&D->tH, &D->tE, &D->tG, &D->tD, &D->tP, &D->tS});
     D->tuned = true;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s};
+    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s, D->wh};
     return cost;
 }
 
@@ -4018,6 +4048,17 @@ extern "C" int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e) {
     }
     if (iseg_h) *iseg_h = s->dev->iseg_h;
     if (iseg_e) *iseg_e = s->dev->iseg_e;
+    return s->dev->tuned ? 1 : 0;
+}
+
+// k points per lane of the H and E updates (the H width chosen by the tuner unless fixed)
+extern "C" int cem_amd_lane_widths(cem_sim *s, int *lanes_h, int *lanes_e) {
+    if (!s || !s->dev) {
+        set_error("simulation has not run on the GPU yet");
+        return -1;
+    }
+    if (lanes_h) *lanes_h = s->dev->wh;
+    if (lanes_e) *lanes_e = s->dev->we;
     return s->dev->tuned ? 1 : 0;
 }
 

@@ -135,6 +135,7 @@ AMD_SIGNATURES = {
     "cem_amd_s2_renormalize": (_I, [_DP, _I, _DP, _DP, _D]),
     "cem_amd_s2_renormalize_noalloc": (_I, [_DP, _I, _DP, _DP, _D]),
     "cem_amd_launch_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cem_amd_lane_widths": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cem_amd_step_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cem_amd_clone_excitation": (_P, [_P, _I]),
     "cem_amd_step_pair": (_I, [_P, _P, _L]),
@@ -510,6 +511,12 @@ class Sim:
         """(i-planes per thread of the H update, of the E update, fixed yet)."""
         h, e = C.c_int(), C.c_int()
         rc = self._ok(self.lib.cem_amd_launch_shape(self.h, C.byref(h), C.byref(e)))
+        return h.value, e.value, rc == 1
+
+    def lane_widths(self):
+        """(k points per thread of the H update, of the E update, fixed yet)."""
+        h, e = C.c_int(), C.c_int()
+        rc = self._ok(self.lib.cem_amd_lane_widths(self.h, C.byref(h), C.byref(e)))
         return h.value, e.value, rc == 1
 
     def step_shape(self):

@@ -87,6 +87,13 @@ CEM_API int cem_amd_s2_renormalize_noalloc(double *S, int nf, const double *z1, 
  * it, 0 before, -1 if the simulation has not run on the GPU. */
 CEM_API int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e);
 
+/* k points per thread of the H and E updates.  The H width is chosen by the
+ * autotuner among 4, 2 and 1 (small grids step faster with more threads)
+ * unless COURANT_LANES_H fixes it; the E width is COURANT_LANES_E (default 1).
+ * Returns 1 once the autotuner has run (or the shape was fixed), 0 before, -1
+ * if the simulation has not run on the GPU. */
+CEM_API int cem_amd_lane_widths(cem_sim *s, int *lanes_h, int *lanes_e);
+
 /* Shape of the one-pass step kernel (H and E updates of a step in one sweep,
  * field sets alternating), which full steps (cem_step, cem_run) use when
  * COURANT_STEP=onepass is in the environment at set-up: rows = grid rows (y)

@@ -462,6 +462,28 @@ def test_autotuned_launch_shape_is_bit_identical(cem, gpu_available, monkeypatch
     assert np.array_equal(a.port_v(1), b.port_v(1))
 
 
+def test_autotuned_lane_width_is_bit_identical(cem, gpu_available, monkeypatch):
+    """The autotuner also picks the k points per thread of the H update (4, 2 or 1); a run
+    with every other width fixed gives bit-identical fields and spectra."""
+    print_case("autotuned H lane width vs fixed widths")
+    a = S.checkpoint_scene(cem)
+    a.step(300)
+    wh, we, fixed = a.lane_widths()
+    assert fixed and wh in (1, 2, 4) and we == 1
+    for w in (1, 2, 4):
+        if w == wh:
+            continue
+        monkeypatch.setenv("COURANT_LANES_H", str(w))
+        b = S.checkpoint_scene(cem)
+        b.step(300)
+        monkeypatch.delenv("COURANT_LANES_H")
+        assert b.lane_widths()[0] == w
+        for fa, fb in zip(fields(a), fields(b)):
+            assert np.array_equal(fa, fb)
+        assert np.array_equal(a.port_v(1), b.port_v(1))
+        b.close()
+
+
 @pytest.mark.parametrize("scene", ["debye_sibc", "mode_lossy"])
 def test_graph_replay_matches_direct_launches(cem, gpu_available, monkeypatch, scene):
     """Runs of steps replayed from the captured graph and the same steps launched one
