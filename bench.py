@@ -408,6 +408,7 @@ def main():
                    "one_pass_step": {"rows_per_workgroup": rows, "xplanes_per_workgroup": planes} if rows else None,
                    "iplanes_per_thread_split": None if rows else {"H": shape[0], "E": shape[1]},
                    "kpoints_per_thread_split": None if rows else {"H": lanes[0], "E": lanes[1]},
+                   "tile_order_split": None if rows else sim.tile_order(),
                    "autotuned": shape[2]},
     }
     sim.close()

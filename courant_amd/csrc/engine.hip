@@ -122,6 +122,7 @@ struct HArgs {
     float *psi[3][2];       // psiH[c][w]
     float dth;
     int has_pml;
+    int xcd;                // 1: XCD-banded tile order (xcd_tile), 0: dispatch order
     // pair: a second right-hand side of the same plan (the other excitation of a 2-port),
     // whose fields and psi_H these are; odd tiles of the launch update it
     int pair;
@@ -151,10 +152,12 @@ struct EArgs {
     const unsigned *cidx;   // N: 8-bit classes of Ex, Ey, Ez (compressed coefficients)
     const float2 *ctab;     // 3 x 256 (ca, cb) per class
     const int *crow;        // 3 x 256 Debye material row of a class; -1: not updated here
+    int ncls;               // classes in use per component (slots 0..ncls-1 of each 256)
     const float *es[3];     // ide per axis
     Pml pml;
     float *psi[3][2];       // psiE[c][w]
     int has_pml;
+    int xcd;                // 1: XCD-banded tile order (xcd_tile), 0: dispatch order
     // Debye ADE of the dispersive edges this kernel updates.  J is held over a box
     // of the grid, [i - bi0][j - bj0][comp][pole][k - 4 bg0]: the lanes of a wave
     // read one pole of consecutive points as one contiguous run, and one (i, j) row
@@ -413,7 +416,7 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, int iseg, int npblk) {
     const Grid g = a.g;
     const int ngw = g.kp / W;
-    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int tile = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     if (a.pair) {   // the two right-hand sides of a tile are neighbouring tiles (one XCD)
         if (tile & 1) {
             a.E = a.E2;
@@ -621,9 +624,10 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     __shared__ int s_row[(kCmp && NP > 0) ? 768 : 1];
     __shared__ float s_bt[(kCmp && NP > 0) ? kDebyeRows * NP : 1];
     if (kCmp) {
-        for (int t = threadIdx.x; t < 768; t += kBlock) {
-            s_tab[t] = a.ctab[t];
-            if (NP > 0) s_row[t] = a.crow[t];
+        for (int t = threadIdx.x; t < 3 * a.ncls; t += kBlock) {   // the classes in use only
+            const int c = t / a.ncls, o = c * 256 + (t - c * a.ncls);
+            s_tab[o] = a.ctab[o];
+            if (NP > 0) s_row[o] = a.crow[o];
         }
         if (NP > 0)
             for (int t = threadIdx.x; t < a.nrow * NP; t += kBlock) s_bt[t] = a.btab[t];
@@ -631,7 +635,7 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     }
     const Grid g = a.g;
     const int ngw = g.kp / W;
-    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int tile = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     if (a.pair) {   // the two right-hand sides of a tile are neighbouring tiles (one XCD)
         if (tile & 1) {
             a.E = a.E2;
@@ -878,9 +882,10 @@ __global__ void __launch_bounds__(1024) k_step(SArgs f, int iseg) {
     float *s_bt = (float *)(s_row + ((kCmp && NP > 0) ? 768 : 0));   // nrow x NP
     const EArgs &a = f.e;
     if (kCmp) {
-        for (int t = threadIdx.x; t < 768; t += blockDim.x) {
-            s_tab[t] = a.ctab[t];
-            if (NP > 0) s_row[t] = a.crow[t];
+        for (int t = threadIdx.x; t < 3 * a.ncls; t += blockDim.x) {   // the classes in use only
+            const int c = t / a.ncls, o = c * 256 + (t - c * a.ncls);
+            s_tab[o] = a.ctab[o];
+            if (NP > 0) s_row[o] = a.crow[o];
         }
         if (NP > 0)
             for (int t = threadIdx.x; t < a.nrow * NP; t += blockDim.x) s_bt[t] = a.btab[t];
@@ -1156,9 +1161,10 @@ __global__ void __launch_bounds__(kBlock) k_step_rows(SArgs f, int iseg, int npb
     __shared__ float s_bt[(kCmp && NP > 0) ? kDebyeRows * NP : 1];
     const EArgs &a = f.e;
     if (kCmp) {
-        for (int t = threadIdx.x; t < 768; t += kBlock) {
-            s_tab[t] = a.ctab[t];
-            if (NP > 0) s_row[t] = a.crow[t];
+        for (int t = threadIdx.x; t < 3 * a.ncls; t += kBlock) {   // the classes in use only
+            const int c = t / a.ncls, o = c * 256 + (t - c * a.ncls);
+            s_tab[o] = a.ctab[o];
+            if (NP > 0) s_row[o] = a.crow[o];
         }
         if (NP > 0)
             for (int t = threadIdx.x; t < a.nrow * NP; t += kBlock) s_bt[t] = a.btab[t];
@@ -1937,6 +1943,7 @@ struct Device {
     DBuf<unsigned> cidx;         // N packed coefficient classes
     DBuf<float2> ctab;           // 3 x 256 (ca, cb)
     DBuf<int> crow;              // 3 x 256 Debye material row of a class (-1: none)
+    int ncls = 256;              // classes in use per component
     bool compressed = false;
     // Debye currents updated inside the E kernel (fused) or by the post kernel
     bool fused = false;
@@ -2003,6 +2010,8 @@ struct Device {
     int iseg_h = kIseg, iseg_e = kIseg;
     int wh = 4, we = 1;          // lane widths (k points per thread) of the H and E updates
     bool wh_fixed = false;       // H lane width set by COURANT_LANES_H (else chosen by the tuner)
+    int xcd = 1;                 // tile order of the H and E updates (1: XCD bands, 0: dispatch order)
+    bool xcd_fixed = false;      // set by COURANT_TILE_ORDER (else chosen by the tuner)
     bool tuned = false;
     // timing
     bool profiling = false;
@@ -2185,11 +2194,12 @@ std::vector<size_t> psi_dev_map(const cem_sim *s, const Device *D, int is_e, int
 // (0, 0, no row): the row padding.  Returns false when a component needs more
 // than 256 classes.
 bool build_classes(const cem_sim *s, const Grid &g, const std::vector<int> &row_of, std::vector<unsigned> &idx,
-                   std::vector<float2> &tab, std::vector<int> &rowtab) {
+                   std::vector<float2> &tab, std::vector<int> &rowtab, int &ncls) {
     const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
     idx.assign((size_t)g.N, 0u);
     tab.assign(3 * 256, make_float2(0.f, 0.f));
     rowtab.assign(3 * 256, -1);
+    ncls = 1;
     size_t t = 0;   // dispersive edges are sorted by (comp, q): walk them with q
     for (int c = 0; c < 3; c++) {
         std::vector<std::pair<std::array<unsigned, 3>, unsigned>> seen;
@@ -2228,6 +2238,7 @@ bool build_classes(const cem_sim *s, const Grid &g, const std::vector<int> &row_
                     idx[d0 + k] |= slot << (8 * c);
                 }
             }
+        ncls = std::max(ncls, (int)seen.size());
     }
     return true;
 }
@@ -2361,11 +2372,11 @@ int upload_plan(cem_sim *s, Device *D) {
         if (fused)
             for (int t = 0; t < nd; t++)
                 if (!linked[t]) row_of[t] = (int)mat[t];
-        compressed = build_classes(s, g, row_of, cls, ctab, crow);
+        compressed = build_classes(s, g, row_of, cls, ctab, crow, D->ncls);
         if (!compressed && fused) {   // retry with every dispersive edge in the post kernel
             fused = false;
             std::fill(row_of.begin(), row_of.end(), -1);
-            compressed = build_classes(s, g, row_of, cls, ctab, crow);
+            compressed = build_classes(s, g, row_of, cls, ctab, crow, D->ncls);
         }
     }
     if (!compressed) fused = false;
@@ -2675,6 +2686,7 @@ HArgs h_args(cem_sim *s, Device *D, int par) {
         for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiHs[par][c][w];
     a.dth = (float)(s->dt / CEM_MU0);
     a.has_pml = s->has_pml;
+    a.xcd = D->xcd;
     return a;
 }
 
@@ -2689,11 +2701,13 @@ EArgs e_args(cem_sim *s, Device *D, int par) {
     a.cidx = D->compressed ? D->cidx.p : nullptr;
     a.ctab = D->compressed ? D->ctab.p : nullptr;
     a.crow = D->compressed ? D->crow.p : nullptr;
+    a.ncls = D->ncls;
     for (int i = 0; i < 3; i++) a.es[i] = D->es[i];
     a.pml = D->pmlE;
     for (int c = 0; c < 3; c++)
         for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiE[c][w];
     a.has_pml = s->has_pml;
+    a.xcd = D->xcd;
     if (D->fused) {
         a.J = D->J.p;
         a.btab = D->btab.p;
@@ -3249,6 +3263,10 @@ int cem::dev_ensure(cem_sim *s) {
             *pr.second = w == 1 || w == 2 ? w : 4;
             if (pr.second == &D->wh) D->wh_fixed = true;
         }
+    if (const char *v = std::getenv("COURANT_TILE_ORDER")) {
+        D->xcd = std::strcmp(v, "plain") != 0;
+        D->xcd_fixed = true;
+    }
     if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v)), D->tuned = true;
     if (const char *v = std::getenv("COURANT_ISEG_E")) D->iseg_e = std::max(1, std::atoi(v)), D->tuned = true;
     if (const char *v = std::getenv("COURANT_ISEG_S")) D->iseg_s = std::max(1, std::atoi(v)), D->tuned = true;
@@ -3340,16 +3358,16 @@ constexpr int kTuneSteps = 2;
 constexpr int kTuneRounds = 2;
 constexpr int kTuneLanesH[] = {4, 2, 1};   // k points per lane of the H update
 constexpr int kTuneNl = sizeof kTuneLanesH / sizeof kTuneLanesH[0];
-constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * (kTuneNl + kTuneNc * kTuneNc) * (1 + kTuneSteps);
+constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * (kTuneNl + kTuneNc * kTuneNc + 2) * (1 + kTuneSteps);
 constexpr long kTuneCostStep = 1 + (long)kTuneRounds * kTuneNs * (1 + kTuneSteps);
 constexpr long kTuneCost = kTuneCostSplit > kTuneCostStep ? kTuneCostSplit : kTuneCostStep;
 
 std::mutex g_tune_mu;
-std::map<std::array<long long, 10>, std::array<int, 4>> g_tune;
+std::map<std::array<long long, 10>, std::array<int, 5>> g_tune;
 
 long autotune(cem_sim *s, Device *D, long budget) {
     const std::array<long long, 10> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
-                                           (long long)D->fused, D->wh_fixed ? D->wh : 0, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr), (long long)D->tj * 4096 + D->kw};
+                                           (long long)D->fused, D->wh_fixed ? D->wh : 0, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr) + 8 * (D->xcd_fixed ? 1 + D->xcd : 0), (long long)D->tj * 4096 + D->kw};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
@@ -3358,6 +3376,7 @@ long autotune(cem_sim *s, Device *D, long budget) {
             D->iseg_e = it->second[1];
             D->iseg_s = it->second[2];
             if (!D->onepass) D->wh = it->second[3];
+            if (!D->onepass) D->xcd = it->second[4];
             D->tuned = true;
             return 0;
         }
@@ -3456,11 +3475,34 @@ long autotune(cem_sim *s, Device *D, long budget) {
             }
             std::fprintf(stderr, "  -> H %d, E %d\n", D->iseg_h, D->iseg_e);
         }
+        // then the tile order at that shape: XCD bands (tiles that share halo rows on one L2)
+        // or dispatch order (consecutive tiles on consecutive XCDs, every XCD streaming the
+        // same region of memory); which one wins depends on the grid
+        double to[2];
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int c = 0; c < 2; c++) {
+                D->xcd = D->xcd_fixed ? D->xcd : 1 - c;
+                one_step();             // settling step, untimed
+                D->profiling = true;
+                reset({&D->tH, &D->tE});
+                for (int st = 0; st < kTuneSteps; st++) one_step();
+                D->tH.fold();
+                D->tE.fold();
+                D->profiling = false;
+                const double v = D->tH.total_ms + D->tE.total_ms;
+                to[c] = r ? std::min(to[c], v) : v;
+            }
+        if (!D->xcd_fixed) {
+            D->xcd = to[1] < to[0] ? 0 : 1;
+            if (dbg)
+                std::fprintf(stderr, "courant autotune H+E us per step: XCD bands %.1f, dispatch order %.1f -> %s\n",
+                             to[0] / kTuneSteps * 1e3, to[1] / kTuneSteps * 1e3, D->xcd ? "XCD bands" : "dispatch order");
+        }
     }
     reset({&D->tH, &D->tE, &D->tG, &D->tD, &D->tP, &D->tS});
     D->tuned = true;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s, D->wh};
+    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s, D->wh, D->xcd};
     return cost;
 }
 
@@ -4049,6 +4091,15 @@ extern "C" int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e) {
     if (iseg_h) *iseg_h = s->dev->iseg_h;
     if (iseg_e) *iseg_e = s->dev->iseg_e;
     return s->dev->tuned ? 1 : 0;
+}
+
+// tile order of the H and E updates: 1 XCD bands, 0 dispatch order (chosen by the tuner unless fixed)
+extern "C" int cem_amd_tile_order(cem_sim *s) {
+    if (!s || !s->dev) {
+        set_error("simulation has not run on the GPU yet");
+        return -1;
+    }
+    return s->dev->xcd;
 }
 
 // k points per lane of the H and E updates (the H width chosen by the tuner unless fixed)

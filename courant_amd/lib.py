@@ -136,6 +136,7 @@ AMD_SIGNATURES = {
     "cem_amd_s2_renormalize_noalloc": (_I, [_DP, _I, _DP, _DP, _D]),
     "cem_amd_launch_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cem_amd_lane_widths": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cem_amd_tile_order": (_I, [_P]),
     "cem_amd_step_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cem_amd_clone_excitation": (_P, [_P, _I]),
     "cem_amd_step_pair": (_I, [_P, _P, _L]),
@@ -518,6 +519,10 @@ class Sim:
         h, e = C.c_int(), C.c_int()
         rc = self._ok(self.lib.cem_amd_lane_widths(self.h, C.byref(h), C.byref(e)))
         return h.value, e.value, rc == 1
+
+    def tile_order(self):
+        """"xcd" (consecutive tiles on one XCD) or "plain" (dispatch order)."""
+        return "xcd" if self._ok(self.lib.cem_amd_tile_order(self.h)) else "plain"
 
     def step_shape(self):
         """(grid rows per workgroup, x planes per workgroup) of the one-pass step kernel;

@@ -94,6 +94,13 @@ CEM_API int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e);
  * if the simulation has not run on the GPU. */
 CEM_API int cem_amd_lane_widths(cem_sim *s, int *lanes_h, int *lanes_e);
 
+/* Tile order of the H and E updates: 1 when consecutive tiles run on one XCD
+ * (tiles that share halo rows share its L2), 0 in dispatch order (consecutive
+ * tiles on consecutive XCDs).  Chosen by the autotuner per grid shape unless
+ * COURANT_TILE_ORDER=xcd|plain fixes it.  -1 if the simulation has not run on
+ * the GPU. */
+CEM_API int cem_amd_tile_order(cem_sim *s);
+
 /* Shape of the one-pass step kernel (H and E updates of a step in one sweep,
  * field sets alternating), which full steps (cem_step, cem_run) use when
  * COURANT_STEP=onepass is in the environment at set-up: rows = grid rows (y)
