@@ -69,11 +69,12 @@ def step_bytes(pc):
             + (8 * pc["nterm"] + 36) * pc["nsibc"])
 
 
-def kernel_bytes(pc):
+def kernel_bytes(pc, cbytes=4):
     """Bytes each kernel must move in this engine's storage, per launch:
       H    : E read (12 B/pt) + H read/write (24 B/pt) + psi_H read/write (8 B/entry)
-      E    : E read/write (24) + H read (12) + coefficient classes (4: three 8-bit class
-             numbers instead of 24 B of ca, cb) per point + psi_E (8 B/entry) + the Debye
+      E    : E read/write (24) + H read (12) + coefficient classes (cbytes: a class byte
+             when the grid has at most 256 distinct class triples, else a word of three 8-bit
+             class numbers, instead of 24 B of ca, cb) per point + psi_E (8 B/entry) + the Debye
              currents it updates, read/write (8 B per pole per dispersive edge; beta / dt
              comes from a per-material table)
       step : the one-pass step (H and E of a step in one sweep): E and H read once and
@@ -84,7 +85,7 @@ def kernel_bytes(pc):
              source edges, not counted)
     """
     h = 36 * pc["N"] + 8 * pc["psiH"]
-    e = 40 * pc["N"] + 8 * pc["psiE"] + 8 * pc["npole"] * pc["ndisp"]
+    e = (36 + cbytes) * pc["N"] + 8 * pc["psiE"] + 8 * pc["npole"] * pc["ndisp"]
     step = 52 * pc["N"] + 8 * (pc["psiH"] + pc["psiE"]) + 8 * pc["npole"] * pc["ndisp"]
     post = (8 * pc["nterm"] + 36) * pc["nsibc"]
     return {"H": h, "E": e, "step": step, "post": post}
@@ -146,12 +147,13 @@ def _pmc_pass(counter, args_tail, tmp, tag, env=None):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def pmc_traffic(workload, shape):
+def pmc_traffic(workload, shape, lanes, order):
     """HBM bytes per dispatch of each stepper kernel: FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch, one counter per rocprofv3 run), each scaled by what the same counter reads for
     the plain copy kernel moving 5e8 B in and 5e8 B out per dispatch.  The counted child runs
-    use the launch shape the timed run was tuned to."""
-    env = dict(os.environ, COURANT_ISEG_H=str(shape[0]), COURANT_ISEG_E=str(shape[1]))
+    use the launch shape the timed run was tuned to (planes and lanes per thread, tile order)."""
+    env = dict(os.environ, COURANT_ISEG_H=str(shape[0]), COURANT_ISEG_E=str(shape[1]),
+               COURANT_LANES_H=str(lanes[0]), COURANT_TILE_ORDER=order)
     if shape[3]:
         env["COURANT_ISEG_S"] = str(shape[3])
     tmp = tempfile.mkdtemp(prefix="courant_pmc_", dir="/tmp")
@@ -356,10 +358,11 @@ def main():
     stats = {k: sim.kernel_stats(i) for i, k in enumerate(("H", "E", "gather", "dft", "post", "step"))}
     sim.set_profiling(False)
     pc = plan_counts(sim)
-    kb = kernel_bytes(pc)
+    kb = kernel_bytes(pc, sim.class_bytes())
     b8d = step_bytes(pc)
     shape = sim.launch_shape()
     lanes = sim.lane_widths()
+    order = sim.tile_order()
     rows, planes = sim.step_shape()
     shape = (shape[0], shape[1], shape[2], planes)
     copy_gbs = cem.lib.cem_amd_copy_bandwidth(2 << 30, 10)
@@ -408,12 +411,13 @@ def main():
                    "one_pass_step": {"rows_per_workgroup": rows, "xplanes_per_workgroup": planes} if rows else None,
                    "iplanes_per_thread_split": None if rows else {"H": shape[0], "E": shape[1]},
                    "kpoints_per_thread_split": None if rows else {"H": lanes[0], "E": lanes[1]},
-                   "tile_order_split": None if rows else sim.tile_order(),
+                   "tile_order_split": None if rows else order,
+                   "class_bytes_per_point": sim.class_bytes(),
                    "autotuned": shape[2]},
     }
     sim.close()
     if not args.no_pmc:
-        tr = pmc_traffic(args.workload, shape)
+        tr = pmc_traffic(args.workload, shape, lanes, order)
         if tr:
             kname = "cem::" + out["roofline"]["kernel"]
             out["roofline"]["traffic"] = round(tr["kernels"].get(kname, 0.0))

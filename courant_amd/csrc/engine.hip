@@ -37,6 +37,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "courant_amd.h"
@@ -153,6 +154,11 @@ struct EArgs {
     const float2 *ctab;     // 3 x 256 (ca, cb) per class
     const int *crow;        // 3 x 256 Debye material row of a class; -1: not updated here
     int ncls;               // classes in use per component (slots 0..ncls-1 of each 256)
+    // one byte per point instead of cidx where at most 256 distinct class triples occur:
+    // pidx[point] indexes ptab, whose entry is the point's packed cidx word
+    const unsigned char *pidx;
+    const unsigned *ptab;
+    int nptab;
     const float *es[3];     // ide per axis
     Pml pml;
     float *psi[3][2];       // psiE[c][w]
@@ -618,8 +624,11 @@ __device__ __forceinline__ void debye_point(float &en, float e0, float cb, float
 // reference's order and with the reference's expressions.  Same lane shape as
 // the H update; the H of plane i-1 is carried in registers.  The (ca, cb) class
 // table, the classes' Debye rows and the material rows sit in LDS.
-template <int NP, bool kCmp, int W>
+// CMP: 0 coefficient arrays, 1 class words (cidx), 2 class bytes (pidx -> ptab -> class word)
+template <int NP, int CMP, int W>
 __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, int iseg, int npblk) {
+    constexpr bool kCmp = CMP > 0;
+    __shared__ unsigned s_ptab[CMP == 2 ? 256 : 1];
     __shared__ float2 s_tab[kCmp ? 768 : 1];
     __shared__ int s_row[(kCmp && NP > 0) ? 768 : 1];
     __shared__ float s_bt[(kCmp && NP > 0) ? kDebyeRows * NP : 1];
@@ -629,6 +638,8 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
             s_tab[o] = a.ctab[o];
             if (NP > 0) s_row[o] = a.crow[o];
         }
+        if (CMP == 2)
+            for (int t = threadIdx.x; t < a.nptab; t += kBlock) s_ptab[t] = a.ptab[t];
         if (NP > 0)
             for (int t = threadIdx.x; t < a.nrow * NP; t += kBlock) s_bt[t] = a.btab[t];
         __syncthreads();
@@ -729,7 +740,22 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
         const int sx = a.has_pml ? a.pml.slot[0][i] : -1;
         unsigned cls[W];
         float ca[3][W], cb[3][W];
-        if (kCmp) {
+        if constexpr (CMP == 2) {
+            const unsigned char *pb = a.pidx + (q >> 2);   // q / 4: the point index
+            if constexpr (W == 4) {
+                const uchar4 b4 = *(const uchar4 *)pb;
+                cls[0] = s_ptab[b4.x];
+                cls[1 % W] = s_ptab[b4.y];
+                cls[2 % W] = s_ptab[b4.z];
+                cls[3 % W] = s_ptab[b4.w];
+            } else if constexpr (W == 2) {
+                const uchar2 b2 = *(const uchar2 *)pb;
+                cls[0] = s_ptab[b2.x];
+                cls[1 % W] = s_ptab[b2.y];
+            } else {
+                cls[0] = s_ptab[*pb];
+            }
+        } else if constexpr (CMP == 1) {
             if constexpr (W == 4) {
                 const uint4 c4 = *(const uint4 *)((const char *)a.cidx + q);
                 cls[0] = c4.x;
@@ -743,6 +769,8 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
             } else {
                 cls[0] = *(const unsigned *)((const char *)a.cidx + q);
             }
+        }
+        if constexpr (kCmp) {
 #pragma unroll
             for (int e = 0; e < W; e++)
 #pragma unroll
@@ -1944,6 +1972,9 @@ struct Device {
     DBuf<float2> ctab;           // 3 x 256 (ca, cb)
     DBuf<int> crow;              // 3 x 256 Debye material row of a class (-1: none)
     int ncls = 256;              // classes in use per component
+    DBuf<unsigned char> pidx;    // N class bytes (at most 256 distinct class words), else empty
+    DBuf<unsigned> ptab;         // 256 class words indexed by pidx
+    int nptab = 0;
     bool compressed = false;
     // Debye currents updated inside the E kernel (fused) or by the post kernel
     bool fused = false;
@@ -2243,6 +2274,37 @@ bool build_classes(const cem_sim *s, const Grid &g, const std::vector<int> &row_
     return true;
 }
 
+// Class bytes: when the grid holds at most 256 distinct class words (the three
+// components' classes of a point), the E update reads one byte per point and looks
+// the word up in a 256-entry table instead of reading the 4-byte word.  Returns -1
+// on a device error; leaves pidx empty when there are too many distinct words.
+int build_class_bytes(const std::vector<unsigned> &cls, Device *D, hipStream_t st) {
+    std::unordered_map<unsigned, unsigned char> slot;
+    std::vector<unsigned> tab;
+    std::vector<unsigned char> b(cls.size());
+    unsigned last = 0;
+    unsigned char lastb = 0;
+    for (size_t q = 0; q < cls.size(); q++) {
+        const unsigned v = cls[q];
+        if (q == 0 || v != last) {
+            auto it = slot.find(v);
+            if (it == slot.end()) {
+                if (tab.size() == 256) return 0;
+                it = slot.emplace(v, (unsigned char)tab.size()).first;
+                tab.push_back(v);
+            }
+            last = v;
+            lastb = it->second;
+        }
+        b[q] = lastb;
+    }
+    D->nptab = (int)tab.size();
+    tab.resize(256, 0u);
+    if (D->ptab.alloc(256, D->st) || D->ptab.upload(tab.data(), 256, st)) return -1;
+    if (D->pidx.alloc(b.size(), D->st) || D->pidx.upload(b.data(), b.size(), st)) return -1;
+    return 0;
+}
+
 int upload_plan(cem_sim *s, Device *D) {
     const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
     Grid &g = D->g;
@@ -2386,6 +2448,7 @@ int upload_plan(cem_sim *s, Device *D) {
         if (D->cidx.alloc(Nd, D->st) || D->cidx.upload(cls.data(), Nd, st)) return -1;
         if (D->ctab.alloc(3 * 256, D->st) || D->ctab.upload(ctab.data(), 3 * 256, st)) return -1;
         if (D->crow.alloc(3 * 256, D->st) || D->crow.upload(crow.data(), 3 * 256, st)) return -1;
+        if (!std::getenv("COURANT_CLASS_WORDS") && build_class_bytes(cls, D, st)) return -1;
         std::vector<unsigned>().swap(cls);
     } else {
         if (D->ca.alloc(3 * Nd, D->st) || D->cb.alloc(3 * Nd, D->st)) return -1;
@@ -2702,6 +2765,9 @@ EArgs e_args(cem_sim *s, Device *D, int par) {
     a.ctab = D->compressed ? D->ctab.p : nullptr;
     a.crow = D->compressed ? D->crow.p : nullptr;
     a.ncls = D->ncls;
+    a.pidx = D->pidx.p;
+    a.ptab = D->ptab.p;
+    a.nptab = D->nptab;
     for (int i = 0; i < 3; i++) a.es[i] = D->es[i];
     a.pml = D->pmlE;
     for (int c = 0; c < 3; c++)
@@ -2731,20 +2797,25 @@ typedef void (*HKernel)(HArgs, int, int, int, int);
 
 // E kernel of this simulation: coefficient classes or arrays, the number of
 // Debye poles it updates itself (0 when the post kernel takes them), lane width
+template <int W, int C>
+EKernel e_kernel_c(const Device *D) {
+    switch (D->fused ? D->npole : 0) {
+    case 1: return k_update_e<1, C, W>;
+    case 2: return k_update_e<2, C, W>;
+    case 3: return k_update_e<3, C, W>;
+    case 4: return k_update_e<4, C, W>;
+    case 5: return k_update_e<5, C, W>;
+    case 6: return k_update_e<6, C, W>;
+    case 7: return k_update_e<7, C, W>;
+    case 8: return k_update_e<8, C, W>;
+    default: return k_update_e<0, C, W>;
+    }
+}
+
 template <int W>
 EKernel e_kernel_w(const Device *D) {
-    if (!D->compressed) return k_update_e<0, false, W>;
-    switch (D->fused ? D->npole : 0) {
-    case 1: return k_update_e<1, true, W>;
-    case 2: return k_update_e<2, true, W>;
-    case 3: return k_update_e<3, true, W>;
-    case 4: return k_update_e<4, true, W>;
-    case 5: return k_update_e<5, true, W>;
-    case 6: return k_update_e<6, true, W>;
-    case 7: return k_update_e<7, true, W>;
-    case 8: return k_update_e<8, true, W>;
-    default: return k_update_e<0, true, W>;
-    }
+    if (!D->compressed) return k_update_e<0, 0, W>;
+    return D->pidx.p ? e_kernel_c<W, 2>(D) : e_kernel_c<W, 1>(D);
 }
 
 EKernel e_kernel(const Device *D) {
@@ -3294,6 +3365,8 @@ void cem::dev_destroy(cem_sim *s) {
     D->ca.release();
     D->cb.release();
     D->cidx.release();
+    D->pidx.release();
+    D->ptab.release();
     D->ctab.release();
     D->crow.release();
     D->btab.release();
@@ -4091,6 +4164,17 @@ extern "C" int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e) {
     if (iseg_h) *iseg_h = s->dev->iseg_h;
     if (iseg_e) *iseg_e = s->dev->iseg_e;
     return s->dev->tuned ? 1 : 0;
+}
+
+// bytes per point of the coefficient data the E update reads: 1 (class bytes), 4 (class
+// words) or 24 (ca and cb arrays)
+extern "C" int cem_amd_class_bytes(cem_sim *s) {
+    if (!s || !s->dev) {
+        set_error("simulation has not run on the GPU yet");
+        return -1;
+    }
+    const Device *D = s->dev;
+    return !D->compressed ? 24 : D->pidx.p ? 1 : 4;
 }
 
 // tile order of the H and E updates: 1 XCD bands, 0 dispatch order (chosen by the tuner unless fixed)

@@ -137,6 +137,7 @@ AMD_SIGNATURES = {
     "cem_amd_launch_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cem_amd_lane_widths": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cem_amd_tile_order": (_I, [_P]),
+    "cem_amd_class_bytes": (_I, [_P]),
     "cem_amd_step_shape": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cem_amd_clone_excitation": (_P, [_P, _I]),
     "cem_amd_step_pair": (_I, [_P, _P, _L]),
@@ -523,6 +524,10 @@ class Sim:
     def tile_order(self):
         """"xcd" (consecutive tiles on one XCD) or "plain" (dispatch order)."""
         return "xcd" if self._ok(self.lib.cem_amd_tile_order(self.h)) else "plain"
+
+    def class_bytes(self):
+        """Bytes per grid point of the coefficient data the E update reads (1, 4 or 24)."""
+        return self._ok(self.lib.cem_amd_class_bytes(self.h))
 
     def step_shape(self):
         """(grid rows per workgroup, x planes per workgroup) of the one-pass step kernel;

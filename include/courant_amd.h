@@ -101,6 +101,14 @@ CEM_API int cem_amd_lane_widths(cem_sim *s, int *lanes_h, int *lanes_e);
  * the GPU. */
 CEM_API int cem_amd_tile_order(cem_sim *s);
 
+/* Bytes per grid point of the coefficient data the E update reads: 1 when the
+ * grid holds at most 256 distinct (Ex, Ey, Ez) class triples (a class byte per
+ * point and a 256-entry table; COURANT_CLASS_WORDS keeps the words), 4 for a
+ * class word per point (three 8-bit classes), 24 for the ca and cb arrays
+ * (more than 256 classes per component, or COURANT_COEF_ARRAYS).  -1 if the
+ * simulation has not run on the GPU. */
+CEM_API int cem_amd_class_bytes(cem_sim *s);
+
 /* Shape of the one-pass step kernel (H and E updates of a step in one sweep,
  * field sets alternating), which full steps (cem_step, cem_run) use when
  * COURANT_STEP=onepass is in the environment at set-up: rows = grid rows (y)

@@ -442,6 +442,28 @@ def test_coefficient_table_matches_coefficient_arrays(cem, gpu_available, monkey
     assert a.energy() == b.energy()
 
 
+@pytest.mark.parametrize("lanes", ["1", "4"])
+def test_class_bytes_match_class_words(cem, gpu_available, monkeypatch, lanes):
+    """The E update reading one class byte per point (a 256-entry table of class words)
+    and reading the class word per point give bit-identical fields and spectra, at one and
+    at four k points per lane."""
+    print_case(f"class bytes vs class words, E lanes {lanes}")
+    monkeypatch.setenv("COURANT_LANES_E", lanes)
+    a = S.checkpoint_scene(cem)
+    a.step(300)
+    assert a.class_bytes() == 1
+    monkeypatch.setenv("COURANT_CLASS_WORDS", "1")
+    b = S.checkpoint_scene(cem)
+    b.step(300)
+    monkeypatch.delenv("COURANT_CLASS_WORDS")
+    assert b.class_bytes() == 4
+    for fa, fb in zip(fields(a), fields(b)):
+        assert np.array_equal(fa, fb)
+    assert np.array_equal(a.port_v(1), b.port_v(1))
+    assert a.energy() == b.energy()
+    b.close()
+
+
 def test_autotuned_launch_shape_is_bit_identical(cem, gpu_available, monkeypatch):
     """The autotuner picks the i-planes per thread on the first steps; a run with a fixed,
     different shape gives bit-identical fields and spectra."""
