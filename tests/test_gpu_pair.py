@@ -33,8 +33,11 @@ def same(a, b, ports, tag):
     assert np.array_equal(a.source_spectrum(), b.source_spectrum()), tag
 
 
-def test_pair_matches_each_excitation_alone_lumped(cem, gpu_available):
-    print_case("pair step vs alone: lossy lumped-port board (Debye + SIBC + CPML)")
+@pytest.mark.parametrize("order", [None, "plain", "xcd"])
+def test_pair_matches_each_excitation_alone_lumped(cem, gpu_available, monkeypatch, order):
+    print_case(f"pair step vs alone: lossy lumped-port board (Debye + SIBC + CPML), tile order {order}")
+    if order:   # the tile orders number the two right-hand sides' tiles differently
+        monkeypatch.setenv("COURANT_TILE_ORDER", order)
     a, ports = serve.build(cem, BOARD, 0)
     b = a.clone_excitation(ports[1])
     a1, _ = serve.build(cem, BOARD, 0)
