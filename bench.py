@@ -147,13 +147,13 @@ def _pmc_pass(counter, args_tail, tmp, tag, env=None):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def pmc_traffic(workload, shape, lanes, order):
+def pmc_traffic(workload, shape, lanes, order, hrev):
     """HBM bytes per dispatch of each stepper kernel: FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch, one counter per rocprofv3 run), each scaled by what the same counter reads for
     the plain copy kernel moving 5e8 B in and 5e8 B out per dispatch.  The counted child runs
     use the launch shape the timed run was tuned to (planes and lanes per thread, tile order)."""
     env = dict(os.environ, COURANT_ISEG_H=str(shape[0]), COURANT_ISEG_E=str(shape[1]),
-               COURANT_LANES_H=str(lanes[0]), COURANT_TILE_ORDER=order)
+               COURANT_LANES_H=str(lanes[0]), COURANT_TILE_ORDER=order, COURANT_H_REVERSE=str(int(hrev)))
     if shape[3]:
         env["COURANT_ISEG_S"] = str(shape[3])
     tmp = tempfile.mkdtemp(prefix="courant_pmc_", dir="/tmp")
@@ -363,6 +363,7 @@ def main():
     shape = sim.launch_shape()
     lanes = sim.lane_widths()
     order = sim.tile_order()
+    sim_hrev = sim.h_reversed()
     rows, planes = sim.step_shape()
     shape = (shape[0], shape[1], shape[2], planes)
     copy_gbs = cem.lib.cem_amd_copy_bandwidth(2 << 30, 10)
@@ -412,12 +413,13 @@ def main():
                    "iplanes_per_thread_split": None if rows else {"H": shape[0], "E": shape[1]},
                    "kpoints_per_thread_split": None if rows else {"H": lanes[0], "E": lanes[1]},
                    "tile_order_split": None if rows else order,
+                   "h_sweep_reversed": None if rows else sim_hrev,
                    "class_bytes_per_point": sim.class_bytes(),
                    "autotuned": shape[2]},
     }
     sim.close()
     if not args.no_pmc:
-        tr = pmc_traffic(args.workload, shape, lanes, order)
+        tr = pmc_traffic(args.workload, shape, lanes, order, sim_hrev)
         if tr:
             kname = "cem::" + out["roofline"]["kernel"]
             out["roofline"]["traffic"] = round(tr["kernels"].get(kname, 0.0))

@@ -124,6 +124,7 @@ struct HArgs {
     float dth;
     int has_pml;
     int xcd;                // 1: XCD-banded tile order (xcd_tile), 0: dispatch order
+    int rev;                // 1: tiles from the last (the sweep starts where the E sweep ended)
     // pair: a second right-hand side of the same plan (the other excitation of a 2-port),
     // whose fields and psi_H these are; odd tiles of the launch update it
     int pair;
@@ -423,6 +424,9 @@ __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, in
     const Grid g = a.g;
     const int ngw = g.kp / W;
     int tile = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    // reversed, the H sweep starts on the planes the E sweep before it touched last (still
+    // in the MALL) and ends on the planes the E sweep after it starts with
+    if (a.rev) tile = (int)gridDim.x - 1 - tile;
     if (a.pair) {   // the two right-hand sides of a tile are neighbouring tiles (one XCD)
         if (tile & 1) {
             a.E = a.E2;
@@ -2043,6 +2047,8 @@ struct Device {
     bool wh_fixed = false;       // H lane width set by COURANT_LANES_H (else chosen by the tuner)
     int xcd = 1;                 // tile order of the H and E updates (1: XCD bands, 0: dispatch order)
     bool xcd_fixed = false;      // set by COURANT_TILE_ORDER (else chosen by the tuner)
+    int hrev = 1;                // H sweep over its tiles from the last (zig-zag with the E sweep)
+    bool hrev_fixed = false;     // set by COURANT_H_REVERSE (else chosen by the tuner)
     bool tuned = false;
     // timing
     bool profiling = false;
@@ -2750,6 +2756,7 @@ HArgs h_args(cem_sim *s, Device *D, int par) {
     a.dth = (float)(s->dt / CEM_MU0);
     a.has_pml = s->has_pml;
     a.xcd = D->xcd;
+    a.rev = D->hrev;
     return a;
 }
 
@@ -3334,6 +3341,10 @@ int cem::dev_ensure(cem_sim *s) {
             *pr.second = w == 1 || w == 2 ? w : 4;
             if (pr.second == &D->wh) D->wh_fixed = true;
         }
+    if (const char *v = std::getenv("COURANT_H_REVERSE")) {
+        D->hrev = std::atoi(v) != 0;
+        D->hrev_fixed = true;
+    }
     if (const char *v = std::getenv("COURANT_TILE_ORDER")) {
         D->xcd = std::strcmp(v, "plain") != 0;
         D->xcd_fixed = true;
@@ -3431,16 +3442,16 @@ constexpr int kTuneSteps = 2;
 constexpr int kTuneRounds = 2;
 constexpr int kTuneLanesH[] = {4, 2, 1};   // k points per lane of the H update
 constexpr int kTuneNl = sizeof kTuneLanesH / sizeof kTuneLanesH[0];
-constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * (kTuneNl + kTuneNc * kTuneNc + 2) * (1 + kTuneSteps);
+constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * (kTuneNl + kTuneNc * kTuneNc + 4) * (1 + kTuneSteps);
 constexpr long kTuneCostStep = 1 + (long)kTuneRounds * kTuneNs * (1 + kTuneSteps);
 constexpr long kTuneCost = kTuneCostSplit > kTuneCostStep ? kTuneCostSplit : kTuneCostStep;
 
 std::mutex g_tune_mu;
-std::map<std::array<long long, 10>, std::array<int, 5>> g_tune;
+std::map<std::array<long long, 10>, std::array<int, 6>> g_tune;
 
 long autotune(cem_sim *s, Device *D, long budget) {
     const std::array<long long, 10> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
-                                           (long long)D->fused, D->wh_fixed ? D->wh : 0, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr) + 8 * (D->xcd_fixed ? 1 + D->xcd : 0), (long long)D->tj * 4096 + D->kw};
+                                           (long long)D->fused, D->wh_fixed ? D->wh : 0, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr) + 8 * (D->xcd_fixed ? 1 + D->xcd : 0) + 32 * (D->hrev_fixed ? 1 + D->hrev : 0), (long long)D->tj * 4096 + D->kw};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
@@ -3450,6 +3461,7 @@ long autotune(cem_sim *s, Device *D, long budget) {
             D->iseg_s = it->second[2];
             if (!D->onepass) D->wh = it->second[3];
             if (!D->onepass) D->xcd = it->second[4];
+            if (!D->onepass) D->hrev = it->second[5];
             D->tuned = true;
             return 0;
         }
@@ -3550,11 +3562,14 @@ long autotune(cem_sim *s, Device *D, long budget) {
         }
         // then the tile order at that shape: XCD bands (tiles that share halo rows on one L2)
         // or dispatch order (consecutive tiles on consecutive XCDs, every XCD streaming the
-        // same region of memory); which one wins depends on the grid
-        double to[2];
+        // same region of memory), each with the H sweep forward or reversed (zig-zag: each
+        // sweep starts on the planes the one before it touched last); which one wins
+        // depends on the grid
+        double to[4];
         for (int r = 0; r < kTuneRounds; r++)
-            for (int c = 0; c < 2; c++) {
-                D->xcd = D->xcd_fixed ? D->xcd : 1 - c;
+            for (int c = 0; c < 4; c++) {
+                D->xcd = D->xcd_fixed ? D->xcd : 1 - (c & 1);
+                D->hrev = D->hrev_fixed ? D->hrev : c >> 1;
                 one_step();             // settling step, untimed
                 D->profiling = true;
                 reset({&D->tH, &D->tE});
@@ -3565,17 +3580,23 @@ long autotune(cem_sim *s, Device *D, long budget) {
                 const double v = D->tH.total_ms + D->tE.total_ms;
                 to[c] = r ? std::min(to[c], v) : v;
             }
-        if (!D->xcd_fixed) {
-            D->xcd = to[1] < to[0] ? 0 : 1;
-            if (dbg)
-                std::fprintf(stderr, "courant autotune H+E us per step: XCD bands %.1f, dispatch order %.1f -> %s\n",
-                             to[0] / kTuneSteps * 1e3, to[1] / kTuneSteps * 1e3, D->xcd ? "XCD bands" : "dispatch order");
-        }
+        int b = 0;
+        for (int c = 0; c < 4; c++)
+            if (to[c] < to[b]) b = c;
+        if (!D->xcd_fixed) D->xcd = 1 - (b & 1);
+        if (!D->hrev_fixed) D->hrev = b >> 1;
+        if (dbg)
+            std::fprintf(stderr,
+                         "courant autotune H+E us per step: XCD bands %.1f, dispatch order %.1f; H reversed: %.1f, %.1f"
+                         " -> %s%s\n",
+                         to[0] / kTuneSteps * 1e3, to[1] / kTuneSteps * 1e3, to[2] / kTuneSteps * 1e3,
+                         to[3] / kTuneSteps * 1e3, D->xcd ? "XCD bands" : "dispatch order",
+                         D->hrev ? ", H reversed" : "");
     }
     reset({&D->tH, &D->tE, &D->tG, &D->tD, &D->tP, &D->tS});
     D->tuned = true;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s, D->wh, D->xcd};
+    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s, D->wh, D->xcd, D->hrev};
     return cost;
 }
 
@@ -4177,13 +4198,14 @@ extern "C" int cem_amd_class_bytes(cem_sim *s) {
     return !D->compressed ? 24 : D->pidx.p ? 1 : 4;
 }
 
-// tile order of the H and E updates: 1 XCD bands, 0 dispatch order (chosen by the tuner unless fixed)
+// tile order of the H and E updates: bit 0 XCD bands (else dispatch order), bit 1 H sweep
+// reversed (chosen by the tuner unless fixed)
 extern "C" int cem_amd_tile_order(cem_sim *s) {
     if (!s || !s->dev) {
         set_error("simulation has not run on the GPU yet");
         return -1;
     }
-    return s->dev->xcd;
+    return s->dev->xcd + 2 * s->dev->hrev;
 }
 
 // k points per lane of the H and E updates (the H width chosen by the tuner unless fixed)

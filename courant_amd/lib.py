@@ -523,7 +523,11 @@ class Sim:
 
     def tile_order(self):
         """"xcd" (consecutive tiles on one XCD) or "plain" (dispatch order)."""
-        return "xcd" if self._ok(self.lib.cem_amd_tile_order(self.h)) else "plain"
+        return "xcd" if self._ok(self.lib.cem_amd_tile_order(self.h)) & 1 else "plain"
+
+    def h_reversed(self):
+        """True when the H update runs its tiles from the last (zig-zag with the E update)."""
+        return bool(self._ok(self.lib.cem_amd_tile_order(self.h)) & 2)
 
     def class_bytes(self):
         """Bytes per grid point of the coefficient data the E update reads (1, 4 or 24)."""

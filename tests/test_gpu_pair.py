@@ -33,10 +33,12 @@ def same(a, b, ports, tag):
     assert np.array_equal(a.source_spectrum(), b.source_spectrum()), tag
 
 
-@pytest.mark.parametrize("order", [None, "plain", "xcd"])
+@pytest.mark.parametrize("order", [None, "plain", "xcd", "reversed"])
 def test_pair_matches_each_excitation_alone_lumped(cem, gpu_available, monkeypatch, order):
     print_case(f"pair step vs alone: lossy lumped-port board (Debye + SIBC + CPML), tile order {order}")
-    if order:   # the tile orders number the two right-hand sides' tiles differently
+    if order == "reversed":   # the H sweep from its last tile: the right-hand sides swap parity
+        monkeypatch.setenv("COURANT_H_REVERSE", "1")
+    elif order:   # the tile orders number the two right-hand sides' tiles differently
         monkeypatch.setenv("COURANT_TILE_ORDER", order)
     a, ports = serve.build(cem, BOARD, 0)
     b = a.clone_excitation(ports[1])

@@ -442,6 +442,25 @@ def test_coefficient_table_matches_coefficient_arrays(cem, gpu_available, monkey
     assert a.energy() == b.energy()
 
 
+def test_autotuned_h_direction_is_bit_identical(cem, gpu_available, monkeypatch):
+    """The autotuner also picks whether the H update runs its tiles from the last (a
+    zig-zag with the E update over the planes); a run with the other direction fixed gives
+    bit-identical fields and spectra."""
+    print_case("autotuned H sweep direction vs the other direction")
+    a = S.checkpoint_scene(cem)
+    a.step(300)
+    rev = a.h_reversed()
+    monkeypatch.setenv("COURANT_H_REVERSE", "0" if rev else "1")
+    b = S.checkpoint_scene(cem)
+    b.step(300)
+    monkeypatch.delenv("COURANT_H_REVERSE")
+    assert b.h_reversed() != rev
+    for fa, fb in zip(fields(a), fields(b)):
+        assert np.array_equal(fa, fb)
+    assert np.array_equal(a.port_v(1), b.port_v(1))
+    b.close()
+
+
 @pytest.mark.parametrize("lanes", ["1", "4"])
 def test_class_bytes_match_class_words(cem, gpu_available, monkeypatch, lanes):
     """The E update reading one class byte per point (a 256-entry table of class words)
