@@ -123,7 +123,8 @@ struct HArgs {
     float *psi[3][2];       // psiH[c][w]
     float dth;
     int has_pml;
-    int xcd;                // 1: XCD-banded tile order (xcd_tile), 0: dispatch order
+    int xcd;                // 1: XCD-banded tile order (xcd_tile), 0: dispatch order in chunks
+    int chunk;              // log2 of the tiles per chunk in dispatch order (chunk_tile)
     int rev;                // 1: tiles from the last (the sweep starts where the E sweep ended)
     // pair: a second right-hand side of the same plan (the other excitation of a 2-port),
     // whose fields and psi_H these are; odd tiles of the launch update it
@@ -164,7 +165,8 @@ struct EArgs {
     Pml pml;
     float *psi[3][2];       // psiE[c][w]
     int has_pml;
-    int xcd;                // 1: XCD-banded tile order (xcd_tile), 0: dispatch order
+    int xcd;                // 1: XCD-banded tile order (xcd_tile), 0: dispatch order in chunks
+    int chunk;              // log2 of the tiles per chunk in dispatch order (chunk_tile)
     // Debye ADE of the dispersive edges this kernel updates.  J is held over a box
     // of the grid, [i - bi0][j - bj0][comp][pole][k - 4 bg0]: the lanes of a wave
     // read one pole of consecutive points as one contiguous run, and one (i, j) row
@@ -292,6 +294,17 @@ struct DftArgs {
 __device__ __forceinline__ int xcd_tile(int wg, int G) {
     const int q = G >> 3, r = G & 7, x = wg & 7;
     return x * q + min(x, r) + (wg >> 3);
+}
+
+// Dispatch order in chunks of 2^s consecutive tiles: the n-th workgroup dealt to XCD x
+// runs tile n mod 2^s of chunk 8 (n >> s) + x, so each XCD sweeps runs of 2^s tiles
+// (the halo rows of a run's inner tile boundaries stay in its L2) while the eight XCDs
+// stream one window of 8 chunks; tiles past the last whole round of 8 chunks run in
+// dispatch order.  s = 0 is plain dispatch order.  A bijection on [0, G).
+__device__ __forceinline__ int chunk_tile(int wg, int G, int s) {
+    if (s == 0 || wg >= ((G >> (3 + s)) << (3 + s))) return wg;
+    const int x = wg & 7, n = wg >> 3;
+    return ((((n >> s) << 3) + x) << s) + (n & ((1 << s) - 1));
 }
 
 // Lane width W: each thread of the stencil kernels owns W consecutive k points
@@ -423,7 +436,7 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) k_update_h(HArgs a, int i0, int i1, int iseg, int npblk) {
     const Grid g = a.g;
     const int ngw = g.kp / W;
-    int tile = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    int tile = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : chunk_tile(blockIdx.x, gridDim.x, a.chunk);
     // reversed, the H sweep starts on the planes the E sweep before it touched last (still
     // in the MALL) and ends on the planes the E sweep after it starts with
     if (a.rev) tile = (int)gridDim.x - 1 - tile;
@@ -650,7 +663,7 @@ __global__ void __launch_bounds__(kBlock) k_update_e(EArgs a, int i0, int i1, in
     }
     const Grid g = a.g;
     const int ngw = g.kp / W;
-    int tile = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    int tile = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : chunk_tile(blockIdx.x, gridDim.x, a.chunk);
     if (a.pair) {   // the two right-hand sides of a tile are neighbouring tiles (one XCD)
         if (tile & 1) {
             a.E = a.E2;
@@ -2047,6 +2060,7 @@ struct Device {
     bool wh_fixed = false;       // H lane width set by COURANT_LANES_H (else chosen by the tuner)
     int xcd = 1;                 // tile order of the H and E updates (1: XCD bands, 0: dispatch order)
     bool xcd_fixed = false;      // set by COURANT_TILE_ORDER (else chosen by the tuner)
+    int chunk = 3;               // dispatch order: log2 of the tiles per chunk (chunk_tile)
     int hrev = 1;                // H sweep over its tiles from the last (zig-zag with the E sweep)
     bool hrev_fixed = false;     // set by COURANT_H_REVERSE (else chosen by the tuner)
     bool tuned = false;
@@ -2756,6 +2770,7 @@ HArgs h_args(cem_sim *s, Device *D, int par) {
     a.dth = (float)(s->dt / CEM_MU0);
     a.has_pml = s->has_pml;
     a.xcd = D->xcd;
+    a.chunk = D->chunk;
     a.rev = D->hrev;
     return a;
 }
@@ -2781,6 +2796,7 @@ EArgs e_args(cem_sim *s, Device *D, int par) {
         for (int w = 0; w < 2; w++) a.psi[c][w] = D->psiE[c][w];
     a.has_pml = s->has_pml;
     a.xcd = D->xcd;
+    a.chunk = D->chunk;
     if (D->fused) {
         a.J = D->J.p;
         a.btab = D->btab.p;
@@ -3346,7 +3362,11 @@ int cem::dev_ensure(cem_sim *s) {
         D->hrev_fixed = true;
     }
     if (const char *v = std::getenv("COURANT_TILE_ORDER")) {
-        D->xcd = std::strcmp(v, "plain") != 0;
+        // xcd: XCD bands; plain: dispatch order; plain<T>: dispatch order in chunks of T tiles
+        D->xcd = std::strncmp(v, "plain", 5) != 0;
+        D->chunk = 0;
+        if (!D->xcd && v[5])
+            for (int t = std::atoi(v + 5); t > 1 && D->chunk < 6; t >>= 1) D->chunk++;
         D->xcd_fixed = true;
     }
     if (const char *v = std::getenv("COURANT_ISEG_H")) D->iseg_h = std::max(1, std::atoi(v)), D->tuned = true;
@@ -3442,16 +3462,20 @@ constexpr int kTuneSteps = 2;
 constexpr int kTuneRounds = 2;
 constexpr int kTuneLanesH[] = {4, 2, 1};   // k points per lane of the H update
 constexpr int kTuneNl = sizeof kTuneLanesH / sizeof kTuneLanesH[0];
-constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * (kTuneNl + kTuneNc * kTuneNc + 4) * (1 + kTuneSteps);
+// tile orders: {XCD bands, log2 chunk} (chunk_tile), each with the H sweep forward or reversed
+constexpr int kTuneOrders[][2] = {{1, 0}, {0, 0}, {0, 2}, {0, 3}, {0, 4}};
+constexpr int kTuneNo = sizeof kTuneOrders / sizeof kTuneOrders[0];
+constexpr long kTuneCostSplit = 1 + (long)kTuneRounds * (kTuneNl + kTuneNc * kTuneNc + 2 * kTuneNo) * (1 + kTuneSteps);
 constexpr long kTuneCostStep = 1 + (long)kTuneRounds * kTuneNs * (1 + kTuneSteps);
 constexpr long kTuneCost = kTuneCostSplit > kTuneCostStep ? kTuneCostSplit : kTuneCostStep;
 
 std::mutex g_tune_mu;
-std::map<std::array<long long, 10>, std::array<int, 6>> g_tune;
+std::map<std::array<long long, 10>, std::array<int, 7>> g_tune;
 
 long autotune(cem_sim *s, Device *D, long budget) {
     const std::array<long long, 10> key = {s->n[0], s->n[1], s->n[2], s->has_pml, (long long)D->compressed,
-                                           (long long)D->fused, D->wh_fixed ? D->wh : 0, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr) + 8 * (D->xcd_fixed ? 1 + D->xcd : 0) + 32 * (D->hrev_fixed ? 1 + D->hrev : 0), (long long)D->tj * 4096 + D->kw};
+                                           (long long)D->fused, D->wh_fixed ? D->wh : 0, D->we, (long long)D->onepass + 2 * D->steprows + 4 * (D->partner != nullptr) + 8 * (D->xcd_fixed ? 1 + D->xcd + 2 * D->chunk : 0) + 256 * (D->hrev_fixed ? 1 + D->hrev : 0),
+                                           (long long)D->tj * 4096 + D->kw};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
@@ -3462,6 +3486,7 @@ long autotune(cem_sim *s, Device *D, long budget) {
             if (!D->onepass) D->wh = it->second[3];
             if (!D->onepass) D->xcd = it->second[4];
             if (!D->onepass) D->hrev = it->second[5];
+            if (!D->onepass) D->chunk = it->second[6];
             D->tuned = true;
             return 0;
         }
@@ -3565,11 +3590,13 @@ long autotune(cem_sim *s, Device *D, long budget) {
         // same region of memory), each with the H sweep forward or reversed (zig-zag: each
         // sweep starts on the planes the one before it touched last); which one wins
         // depends on the grid
-        double to[4];
+        double to[2 * kTuneNo];
+        const int xcd0 = D->xcd, chunk0 = D->chunk;
         for (int r = 0; r < kTuneRounds; r++)
-            for (int c = 0; c < 4; c++) {
-                D->xcd = D->xcd_fixed ? D->xcd : 1 - (c & 1);
-                D->hrev = D->hrev_fixed ? D->hrev : c >> 1;
+            for (int c = 0; c < 2 * kTuneNo; c++) {
+                D->xcd = D->xcd_fixed ? xcd0 : kTuneOrders[c % kTuneNo][0];
+                D->chunk = D->xcd_fixed ? chunk0 : kTuneOrders[c % kTuneNo][1];
+                D->hrev = D->hrev_fixed ? D->hrev : c / kTuneNo;
                 one_step();             // settling step, untimed
                 D->profiling = true;
                 reset({&D->tH, &D->tE});
@@ -3581,22 +3608,23 @@ long autotune(cem_sim *s, Device *D, long budget) {
                 to[c] = r ? std::min(to[c], v) : v;
             }
         int b = 0;
-        for (int c = 0; c < 4; c++)
+        for (int c = 0; c < 2 * kTuneNo; c++)
             if (to[c] < to[b]) b = c;
-        if (!D->xcd_fixed) D->xcd = 1 - (b & 1);
-        if (!D->hrev_fixed) D->hrev = b >> 1;
-        if (dbg)
-            std::fprintf(stderr,
-                         "courant autotune H+E us per step: XCD bands %.1f, dispatch order %.1f; H reversed: %.1f, %.1f"
-                         " -> %s%s\n",
-                         to[0] / kTuneSteps * 1e3, to[1] / kTuneSteps * 1e3, to[2] / kTuneSteps * 1e3,
-                         to[3] / kTuneSteps * 1e3, D->xcd ? "XCD bands" : "dispatch order",
+        D->xcd = D->xcd_fixed ? xcd0 : kTuneOrders[b % kTuneNo][0];
+        D->chunk = D->xcd_fixed ? chunk0 : kTuneOrders[b % kTuneNo][1];
+        if (!D->hrev_fixed) D->hrev = b / kTuneNo;
+        if (dbg) {
+            std::fprintf(stderr, "courant autotune H+E us per step (XCD bands, dispatch order in chunks of 1 4 8 16"
+                                 " tiles; H forward, reversed):");
+            for (int c = 0; c < 2 * kTuneNo; c++) std::fprintf(stderr, " %.1f", to[c] / kTuneSteps * 1e3);
+            std::fprintf(stderr, " -> %s, chunk %d%s\n", D->xcd ? "XCD bands" : "dispatch order", 1 << D->chunk,
                          D->hrev ? ", H reversed" : "");
+        }
     }
     reset({&D->tH, &D->tE, &D->tG, &D->tD, &D->tP, &D->tS});
     D->tuned = true;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s, D->wh, D->xcd, D->hrev};
+    g_tune[key] = {D->iseg_h, D->iseg_e, D->iseg_s, D->wh, D->xcd, D->hrev, D->chunk};
     return cost;
 }
 
@@ -4199,13 +4227,13 @@ extern "C" int cem_amd_class_bytes(cem_sim *s) {
 }
 
 // tile order of the H and E updates: bit 0 XCD bands (else dispatch order), bit 1 H sweep
-// reversed (chosen by the tuner unless fixed)
+// reversed, bits 2.. log2 of the dispatch-order chunk (chosen by the tuner unless fixed)
 extern "C" int cem_amd_tile_order(cem_sim *s) {
     if (!s || !s->dev) {
         set_error("simulation has not run on the GPU yet");
         return -1;
     }
-    return s->dev->xcd + 2 * s->dev->hrev;
+    return s->dev->xcd + 2 * s->dev->hrev + 4 * s->dev->chunk;
 }
 
 // k points per lane of the H and E updates (the H width chosen by the tuner unless fixed)

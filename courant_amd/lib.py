@@ -522,8 +522,12 @@ class Sim:
         return h.value, e.value, rc == 1
 
     def tile_order(self):
-        """"xcd" (consecutive tiles on one XCD) or "plain" (dispatch order)."""
-        return "xcd" if self._ok(self.lib.cem_amd_tile_order(self.h)) & 1 else "plain"
+        """"xcd" (one band of tiles per XCD), "plain" (dispatch order) or "plain<T>" (dispatch
+        order in chunks of T tiles); the value COURANT_TILE_ORDER takes."""
+        v = self._ok(self.lib.cem_amd_tile_order(self.h))
+        if v & 1:
+            return "xcd"
+        return "plain" if v < 4 else f"plain{1 << (v >> 2)}"
 
     def h_reversed(self):
         """True when the H update runs its tiles from the last (zig-zag with the E update)."""

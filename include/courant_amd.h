@@ -94,13 +94,16 @@ CEM_API int cem_amd_launch_shape(cem_sim *s, int *iseg_h, int *iseg_e);
  * if the simulation has not run on the GPU. */
 CEM_API int cem_amd_lane_widths(cem_sim *s, int *lanes_h, int *lanes_e);
 
-/* Tile order of the H and E updates: bit 0 set when consecutive tiles run on
- * one XCD (tiles that share halo rows share its L2), clear in dispatch order
- * (consecutive tiles on consecutive XCDs); bit 1 set when the H update runs its
- * tiles from the last (each sweep then starts on the planes the one before it
- * touched last).  Chosen by the autotuner per grid shape unless
- * COURANT_TILE_ORDER=xcd|plain and COURANT_H_REVERSE=0|1 fix them.  -1 if the
- * simulation has not run on the GPU. */
+/* Tile order of the H and E updates: bit 0 set when each XCD runs one
+ * contiguous band of tiles (tiles that share halo rows share its L2), clear in
+ * dispatch order; bits 2 and up, in dispatch order, log2 of the tiles per
+ * chunk (0: consecutive tiles on consecutive XCDs; s: runs of 2^s consecutive
+ * tiles per XCD, the eight XCDs streaming one window); bit 1 set when the H
+ * update runs its tiles from the last (each sweep then starts on the planes the
+ * one before it touched last).  Chosen by the autotuner per grid shape unless
+ * COURANT_TILE_ORDER=xcd|plain|plain<T> (T tiles per chunk) and
+ * COURANT_H_REVERSE=0|1 fix them.  -1 if the simulation has not run on the
+ * GPU. */
 CEM_API int cem_amd_tile_order(cem_sim *s);
 
 /* Bytes per grid point of the coefficient data the E update reads: 1 when the

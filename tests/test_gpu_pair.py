@@ -33,7 +33,7 @@ def same(a, b, ports, tag):
     assert np.array_equal(a.source_spectrum(), b.source_spectrum()), tag
 
 
-@pytest.mark.parametrize("order", [None, "plain", "xcd", "reversed"])
+@pytest.mark.parametrize("order", [None, "plain", "xcd", "plain4", "reversed"])
 def test_pair_matches_each_excitation_alone_lumped(cem, gpu_available, monkeypatch, order):
     print_case(f"pair step vs alone: lossy lumped-port board (Debye + SIBC + CPML), tile order {order}")
     if order == "reversed":   # the H sweep from its last tile: the right-hand sides swap parity

@@ -526,13 +526,14 @@ def test_autotuned_lane_width_is_bit_identical(cem, gpu_available, monkeypatch):
 
 
 def test_autotuned_tile_order_is_bit_identical(cem, gpu_available, monkeypatch):
-    """The autotuner also picks the tile order of the H and E updates (dispatch order or XCD
-    bands); a run with the other order fixed gives bit-identical fields and spectra."""
+    """The autotuner also picks the tile order of the H and E updates (XCD bands, or dispatch
+    order in chunks of 1-16 tiles); runs with other orders fixed give bit-identical fields
+    and spectra."""
     print_case("autotuned tile order vs the other orders")
     a = S.checkpoint_scene(cem)
     a.step(300)
     order = a.tile_order()
-    for other in ("plain", "xcd"):
+    for other in ("plain", "xcd", "plain8"):
         if other == order:
             continue
         monkeypatch.setenv("COURANT_TILE_ORDER", other)
