@@ -1461,6 +1461,11 @@ __global__ void __launch_bounds__(kBlock) k_step_rows(SArgs f, int iseg, int npb
 // surface-impedance overwrite of the E edge of SIBC slot u (sibc.c:256-269):
 // J from up to four H faces, then the recursive-convolution terms
 __device__ __forceinline__ float sibc_edge(const PostArgs &a, const PostParams &pp, int u) {
+    // every term's state loaded at once, ahead of the face values (a loop of load, update,
+    // store would wait for each load in turn: the stores may alias the next term's load)
+    float yv[kSibcTerms];
+#pragma unroll
+    for (int k = 0; k < kSibcTerms; k++) yv[k] = k < pp.nterm ? a.Y[(long long)k * a.nsb + u] : 0.f;
     const unsigned q = a.sb_idx[u];
     const int c = (int)(q / a.N);
     const long long qq = (long long)q - c * a.N;
@@ -1476,13 +1481,15 @@ __device__ __forceinline__ float sibc_edge(const PostArgs &a, const PostParams &
     if (m & 4u) Js += inv * Hv[qq];
     if (m & 8u) Js += -inv * Hv[qq - su];
     float es = 0.f;
-    for (int k = 0; k < pp.nterm; k++) {
-        float *y = a.Y + (long long)k * a.nsb + u;
-        float yv = *y;
-        yv += pp.om[k] * (Js - yv);
-        *y = yv;
-        es += pp.zc[k] * (Js - yv);
-    }
+#pragma unroll
+    for (int k = 0; k < kSibcTerms; k++)
+        if (k < pp.nterm) {
+            yv[k] += pp.om[k] * (Js - yv[k]);
+            es += pp.zc[k] * (Js - yv[k]);
+        }
+#pragma unroll
+    for (int k = 0; k < kSibcTerms; k++)
+        if (k < pp.nterm) a.Y[(long long)k * a.nsb + u] = yv[k];
     return es;
 }
 
