@@ -309,15 +309,16 @@ def main():
                          "mode-port GCPW, 4.6 M points; cfg1: lumped GCPW, 120x80x60")
     ap.add_argument("--copy-only", type=float, default=0,
                     help="only run the stream-copy kernel on this many GB (byte-counter calibration)")
-    ap.add_argument("--no-extra", action="store_true", help="skip the cfg2 line")
+    ap.add_argument("--no-extra", action="store_true", help="skip the cfg2 line and the mode sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--batch-boards", type=int, default=8, help="boards of the batched set (of 64; 0: none)")
     ap.add_argument("--serve-requests", type=int, default=10, help="boards of the served stream (0: none)")
     ap.add_argument("--serve-rates", type=str, default="1,5,20", help="Poisson arrival rates, boards/s")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solves (host threads, one stream each)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="device footprint admitted at once")
-    ap.add_argument("--mode-sweep", type=float, default=0.0, metavar="DF",
-                    help="also time the lossy GCPW's port-mode sweep at output frequencies spaced DF Hz")
+    ap.add_argument("--mode-sweep", type=float, default=1e8, metavar="DF",
+                    help="time the lossy GCPW's port-mode sweep at output frequencies spaced DF Hz "
+                         "(default 1e8: 400 frequencies; 0 or --no-extra: skip)")
     args = ap.parse_args()
     if args.copy_only:
         from courant_amd import lib
@@ -437,7 +438,7 @@ def main():
             out["batched"] = b
         if v:
             out["served"] = v
-    if args.mode_sweep > 0:
+    if args.mode_sweep > 0 and not args.no_extra:
         out["mode_sweep"] = mode_sweep(cem, args.mode_sweep)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
