@@ -1969,6 +1969,15 @@ struct KernelTimer {
         count += (long)used;
         used = 0;
     }
+    // summed time of the dispatches stamped on pairs [a, b) (their stream synchronised)
+    double span(size_t a, size_t b) const {
+        double ms = 0;
+        for (size_t k = a; k < b; k++) {
+            float x = 0;
+            if (hipEventElapsedTime(&x, pool[k].first, pool[k].second) == hipSuccess) ms += x;
+        }
+        return ms;
+    }
     void destroy() {
         for (auto &e : pool) {
             (void)hipEventDestroy(e.first);
@@ -3511,21 +3520,42 @@ long autotune(cem_sim *s, Device *D, long budget) {
             k->count = 0;
         }
     };
+    // One candidate: a settling step, then kTuneSteps steps whose dispatches stamp event
+    // pairs; a stage's candidates are timed from their pairs after one synchronisation at
+    // the end of the stage (no host round trip per candidate)
+    struct Mark {
+        size_t h0, h1, e0, e1, s0, s1;
+    };
+    auto measure = [&] {
+        one_step();                     // settling step, untimed
+        Mark m{D->tH.used, 0, D->tE.used, 0, D->tS.used, 0};
+        D->profiling = true;
+        for (int st = 0; st < kTuneSteps; st++) one_step();
+        D->profiling = false;
+        m.h1 = D->tH.used;
+        m.e1 = D->tE.used;
+        m.s1 = D->tS.used;
+        return m;
+    };
+    auto settle = [&] { (void)hipStreamSynchronize(D->st); };
+    reset({&D->tH, &D->tE, &D->tS});
     one_step();                         // warm-up step (first launches, caches)
     const bool dbg = std::getenv("COURANT_TUNE_DEBUG") != nullptr;
     if (D->onepass) {
         double t[kTuneNs];
+        Mark mk[kTuneRounds][kTuneNs];
         for (int r = 0; r < kTuneRounds; r++)
             for (int c = 0; c < kTuneNs; c++) {
                 D->iseg_s = kTuneCandS[c];
-                one_step();             // settling step, untimed
-                D->profiling = true;
-                reset({&D->tS});
-                for (int st = 0; st < kTuneSteps; st++) one_step();
-                D->tS.fold();
-                D->profiling = false;
-                t[c] = r ? std::min(t[c], D->tS.total_ms) : D->tS.total_ms;
+                mk[r][c] = measure();
             }
+        settle();
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int c = 0; c < kTuneNs; c++) {
+                const double v = D->tS.span(mk[r][c].s0, mk[r][c].s1);
+                t[c] = r ? std::min(t[c], v) : v;
+            }
+        reset({&D->tH, &D->tE, &D->tS});
         int b = 0;
         for (int c = 0; c < kTuneNs; c++)
             if (t[c] < t[b]) b = c;
@@ -3539,17 +3569,19 @@ long autotune(cem_sim *s, Device *D, long budget) {
         // the H update's lane width first, at the default planes per thread: 4 k points per
         // lane (128-bit accesses) on large grids, fewer (more lanes in flight) on small ones
         double tl[kTuneNl];
+        Mark ml[kTuneRounds][kTuneNl];
         for (int r = 0; r < kTuneRounds; r++)
             for (int c = 0; c < kTuneNl; c++) {
                 D->wh = D->wh_fixed ? D->wh : kTuneLanesH[c];
-                one_step();             // settling step, untimed
-                D->profiling = true;
-                reset({&D->tH});
-                for (int st = 0; st < kTuneSteps; st++) one_step();
-                D->tH.fold();
-                D->profiling = false;
-                tl[c] = r ? std::min(tl[c], D->tH.total_ms) : D->tH.total_ms;
+                ml[r][c] = measure();
             }
+        settle();
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int c = 0; c < kTuneNl; c++) {
+                const double v = D->tH.span(ml[r][c].h0, ml[r][c].h1);
+                tl[c] = r ? std::min(tl[c], v) : v;
+            }
+        reset({&D->tH, &D->tE, &D->tS});
         if (!D->wh_fixed) {
             int b = 0;
             for (int c = 0; c < kTuneNl; c++)
@@ -3562,21 +3594,23 @@ long autotune(cem_sim *s, Device *D, long budget) {
             }
         }
         double t[kTuneNc][kTuneNc];
+        Mark mi[kTuneRounds][kTuneNc][kTuneNc];
         for (int r = 0; r < kTuneRounds; r++)
             for (int ch = 0; ch < kTuneNc; ch++)
                 for (int ce = 0; ce < kTuneNc; ce++) {
                     D->iseg_h = kTuneCand[ch];
                     D->iseg_e = kTuneCand[ce];
-                    one_step();         // settling step, untimed
-                    D->profiling = true;
-                    reset({&D->tH, &D->tE});
-                    for (int st = 0; st < kTuneSteps; st++) one_step();
-                    D->tH.fold();
-                    D->tE.fold();
-                    D->profiling = false;
-                    const double v = D->tH.total_ms + D->tE.total_ms;
+                    mi[r][ch][ce] = measure();
+                }
+        settle();
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int ch = 0; ch < kTuneNc; ch++)
+                for (int ce = 0; ce < kTuneNc; ce++) {
+                    const Mark &m = mi[r][ch][ce];
+                    const double v = D->tH.span(m.h0, m.h1) + D->tE.span(m.e0, m.e1);
                     t[ch][ce] = r ? std::min(t[ch][ce], v) : v;
                 }
+        reset({&D->tH, &D->tE, &D->tS});
         int bh = 0, be = 0;
         for (int ch = 0; ch < kTuneNc; ch++)
             for (int ce = 0; ce < kTuneNc; ce++)
@@ -3599,19 +3633,20 @@ long autotune(cem_sim *s, Device *D, long budget) {
         // depends on the grid
         double to[2 * kTuneNo];
         const int xcd0 = D->xcd, chunk0 = D->chunk;
+        const int hrev0 = D->hrev;
+        Mark mo[kTuneRounds][2 * kTuneNo];
         for (int r = 0; r < kTuneRounds; r++)
             for (int c = 0; c < 2 * kTuneNo; c++) {
                 D->xcd = D->xcd_fixed ? xcd0 : kTuneOrders[c % kTuneNo][0];
                 D->chunk = D->xcd_fixed ? chunk0 : kTuneOrders[c % kTuneNo][1];
-                D->hrev = D->hrev_fixed ? D->hrev : c / kTuneNo;
-                one_step();             // settling step, untimed
-                D->profiling = true;
-                reset({&D->tH, &D->tE});
-                for (int st = 0; st < kTuneSteps; st++) one_step();
-                D->tH.fold();
-                D->tE.fold();
-                D->profiling = false;
-                const double v = D->tH.total_ms + D->tE.total_ms;
+                D->hrev = D->hrev_fixed ? hrev0 : c / kTuneNo;
+                mo[r][c] = measure();
+            }
+        settle();
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int c = 0; c < 2 * kTuneNo; c++) {
+                const Mark &m = mo[r][c];
+                const double v = D->tH.span(m.h0, m.h1) + D->tE.span(m.e0, m.e1);
                 to[c] = r ? std::min(to[c], v) : v;
             }
         // XCD bands with the H sweep forward unless another order is clearly faster: on
@@ -3622,7 +3657,7 @@ long autotune(cem_sim *s, Device *D, long budget) {
             if (to[c] < to[b] && (b || to[c] < 0.98 * to[0])) b = c;
         D->xcd = D->xcd_fixed ? xcd0 : kTuneOrders[b % kTuneNo][0];
         D->chunk = D->xcd_fixed ? chunk0 : kTuneOrders[b % kTuneNo][1];
-        if (!D->hrev_fixed) D->hrev = b / kTuneNo;
+        D->hrev = D->hrev_fixed ? hrev0 : b / kTuneNo;
         if (dbg) {
             std::fprintf(stderr, "courant autotune H+E us per step (XCD bands, dispatch order in chunks of 1 4 8 16"
                                  " tiles; H forward, reversed):");
