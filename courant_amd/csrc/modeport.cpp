@@ -101,6 +101,7 @@ int cem::modeports_finalize(cem_sim *s) {
             const int rc = cem_mode2d_solve(p.xs, warp(s, p.def.f0), p.def.neff0 * tries[t], nullptr, 300, 1e-13);
             if (rc == 0) {
                 p.def.neff0 *= tries[t];
+                p.xs_f0 = warp(s, p.def.f0);
                 return 1;
             }
             if (rc < 0) {
@@ -261,6 +262,7 @@ static void reset_port(cem_sim *s, ModePort &p) {
 // f0, upwards then downwards (modeport.c:146-201).
 static int solve_port_modes(cem_sim *s, ModePort &p) {
     reset_port(s, p);
+    p.xs_f0 = 0;   // the sweeps move xs
     const int q0 = nearest_freq(s, p.def.f0);
     for (int pass = 0; pass < 2; pass++)
         if (sweep_port(s, p, p.xs, q0, pass == 0 ? 1 : -1, true)) return -1;
@@ -283,7 +285,8 @@ static int sweep_batch(const ModePort &p) {
 }
 
 // The port's sweep on the GPU in batches.  q0 (the output frequency nearest f0) is
-// solved as the sequential sweep solves it (a cold solve, then one seeded by it).  Then
+// solved as the sequential sweep solves it (a cold solve, then one seeded by it; the
+// set-up's solve stands for the cold one when q0 is f0 and xs has not moved).  Then
 // each batch takes the next frequencies of each direction (upwards, downwards; at most
 // K, within kSpan of the seed's frequency) and solves them at once, each seeded with
 // the mode of the last frequency solved in its direction: its start vector, and as the
@@ -362,7 +365,11 @@ static int sweep_port_batched(cem_sim *s, ModePort &p) {
         return nu > 0 && nv > 0 ? std::hypot(pr, pi) / std::sqrt(nu * nv) : 0.0;
     };
     const double f_q0 = warp(s, s->freq[q0]);
-    cem_mode2d_solve(xs, f_q0, d.neff0, nullptr, 300, 1e-13);
+    // the cold solve at q0 is the set-up's last solve when q0 is f0 itself (same cross-
+    // section, frequency, shift and fixed start vector; the solver is deterministic): xs
+    // then still holds its result
+    if (p.xs_f0 != f_q0) cem_mode2d_solve(xs, f_q0, d.neff0, nullptr, 300, 1e-13);
+    p.xs_f0 = 0;
     {
         std::vector<double> st;
         vec_of(xs, st);
@@ -532,6 +539,7 @@ static int solve_all_ports(cem_sim *s) {
     for (auto &p : s->mport)
         if (!p.solved && p.xs) {
             reset_port(s, p);
+            p.xs_f0 = 0;   // the sweeps move xs
             jobs.push_back({&p, p.xs, 1, 0, {}});
             jobs.push_back({&p, mode2d_clone(p.xs), -1, 0, {}});
         }

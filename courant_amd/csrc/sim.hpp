@@ -54,6 +54,7 @@ struct ModePort {                  // sim_internal.h:45-61
     std::vector<size_t> src_idx;
     dvec src_w;
     cem_mode2d *xs = nullptr;
+    double xs_f0 = 0;   // warped frequency of the set-up's cold solve while xs still holds it, else 0
     dvec z0, beta, kappa, alpha, alpha_c;   // per output frequency
     bool solved = false;
 };

@@ -676,6 +676,34 @@ def test_mode2d_lu_graph_matches_direct_launches(cem, gpu_available, monkeypatch
         cem.lib.cem_mode2d_destroy(m)
 
 
+def test_mode2d_cold_solve_is_repeatable(cem, gpu_available):
+    """A port sweep starts from the set-up's solve at f0 instead of solving there again, which
+    holds only if a solve from the fixed start vector gives the same result bit for bit every
+    time: on one cross-section after its state has moved (a seeded solve elsewhere), and on a
+    fresh cross-section with its own device context."""
+    print_case("mode2d: repeated cold solves")
+    s, info = S.gcpw(cem, **MODE_MINI)
+    nx, ny, nz = info["cells"]
+    m = cem.lib.cem_cross_section_x(s.h, 3, 0, ny, 0, nz)
+    m2 = cem.lib.cem_cross_section_x(s.h, 3, 0, ny, 0, nz)
+    assert m and m2
+    try:
+        for x in (m, m2):
+            _solve_mode.sizes[x] = (ny * (nz + 1), (ny + 1) * nz)
+        a = _solve_mode(cem, m, 20e9, 1.8)
+        seed = a[4].copy()
+        assert a[0] == 0
+        assert _solve_mode(cem, m, 23e9, 1.8, seed.ctypes.data_as(L._DP))[0] == 0
+        b = _solve_mode(cem, m, 20e9, 1.8)
+        c = _solve_mode(cem, m2, 20e9, 1.8)
+        for r in (b, c):
+            assert r[:4] == a[:4], (r[:4], a[:4])
+            assert np.array_equal(r[4], a[4])
+    finally:
+        cem.lib.cem_mode2d_destroy(m)
+        cem.lib.cem_mode2d_destroy(m2)
+
+
 def test_four_layer_board_matches_reference(cem, ref, gpu_available):
     """The four-layer board generator at a reduced size (meanders on every copper layer, vias,
     6-pole Debye dielectrics, Huray SIBC, lumped ports): fields after 800 steps within 1e-4 of the
