@@ -11,7 +11,7 @@
 //                 pivoting over the kl rows below the diagonal, row
 //                 interchanges within the panel, scaling, rank-1 updates
 //   k_lu_trail  : the panel's interchanges and rank-1 updates applied to the
-//                 trailing columns (one wavefront per column, column in
+//                 trailing columns (one or two columns per wavefront, in
 //                 registers) in the order of the unblocked column loop, so
 //                 every entry sees the host's update sequence
 //   k_inverse   : the whole inverse iteration in one workgroup: normalisation,
@@ -41,8 +41,9 @@ namespace cem {
 namespace {
 
 constexpr double kMu0 = CEM_MU0;
-constexpr int kLuThreads = 1024;       // trailing update: one wavefront per column, 16 columns per workgroup share one LDS copy of the panel
+constexpr int kLuThreads = 1024;       // trailing update: 16 wavefronts share one LDS copy of the panel
 constexpr int kTrailRows = 8;          // rows per lane of a trailing column (panel rows <= 512)
+constexpr int kCUs = 256;              // compute units of an MI355X
 constexpr int kInvThreads = 512;
 constexpr size_t kRingBytes = 150 * 1024;  // LDS ring of band columns for the triangular solves (gfx950: 160 KiB per workgroup)
 constexpr int kProbeBatch = 64;        // colours per probing launch
@@ -415,9 +416,15 @@ __global__ void __launch_bounds__(512) k_lu_panel(Geo g, zc *band, int *ipiv, in
     }
 }
 
-// Trailing columns c in [j1, min(j1 - 1 + kv, n - 1)], one wavefront each; the
-// panel's multipliers are shared through LDS.
+// Trailing columns c in [j1, min(j1 - 1 + kv, n - 1)]: each wavefront holds CPW of them,
+// 16 apart (the workgroup's 16 wavefronts cover 16 CPW consecutive columns), so that one
+// LDS read of a multiplier serves CPW columns and one LDS copy of the panel's multipliers
+// serves 16 CPW columns.  TR: rows per lane, 64 TR >= the panel's rows R (kl + nb): a
+// narrower band walks fewer register rows per rank-1 update.  Each column sees the same
+// interchanges and rank-1 updates in the same order whatever TR and CPW are.
+template <int TR, int CPW>
 __global__ void __launch_bounds__(kLuThreads) k_lu_trail(Geo g, zc *band, const int *ipiv, int j0, int j1) {
+    constexpr int W = kLuThreads / 64;
     extern __shared__ zc L[];
     band += blockIdx.y * g.sband;
     ipiv += blockIdx.y * g.svec;
@@ -429,43 +436,62 @@ __global__ void __launch_bounds__(kLuThreads) k_lu_trail(Geo g, zc *band, const 
     });
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int c = j1 + (int)(blockIdx.x * (kLuThreads / 64) + (threadIdx.x >> 6));
+    const int c0 = j1 + (int)blockIdx.x * W * CPW + (int)(threadIdx.x >> 6);
     const int cend = min(j1 - 1 + kv, g.n - 1);
-    if (c > cend) return;
-    const int rlo = max(j0, c - kv);
-    zc v[kTrailRows];
+    if (c0 > cend) return;
+    zc v[CPW][TR];
 #pragma unroll
-    for (int m = 0; m < kTrailRows; m++) {
-        const int row = j0 + lane + 64 * m;
-        v[m] = row < j0 + R && row >= rlo ? band[bidx(g, row, c)] : mk(0.0, 0.0);
+    for (int u = 0; u < CPW; u++) {
+        const int c = c0 + u * W, rlo = max(j0, c - kv);
+#pragma unroll
+        for (int m = 0; m < TR; m++) {
+            const int row = j0 + lane + 64 * m;
+            v[u][m] = c <= cend && row < j0 + R && row >= rlo ? band[bidx(g, row, c)] : mk(0.0, 0.0);
+        }
     }
     for (int j = j0; j < j1; j++) {
-        if (c > j + kv) continue;
+        if (c0 > j + kv) continue;   // later columns of the wavefront lie further right
         const int p = ipiv[j], last = min(j + kl, g.n - 1);
-        // value of rows j and p
-        zc aj = mk(0.0, 0.0), ap = mk(0.0, 0.0);
+        zc aj[CPW], x[CPW];   // rows j and p of each column (x: row j after the interchange)
 #pragma unroll
-        for (int m = 0; m < kTrailRows; m++) {
-            if (m == (j - j0) / 64) aj = v[m];
-            if (m == (p - j0) / 64) ap = v[m];
+        for (int u = 0; u < CPW; u++) {
+            zc a = mk(0.0, 0.0), b = mk(0.0, 0.0);
+#pragma unroll
+            for (int m = 0; m < TR; m++) {
+                if (m == (j - j0) / 64) a = v[u][m];
+                if (m == (p - j0) / 64) b = v[u][m];
+            }
+            aj[u] = mk(bcast(a.r, (j - j0) % 64), bcast(a.i, (j - j0) % 64));
+            x[u] = mk(bcast(b.r, (p - j0) % 64), bcast(b.i, (p - j0) % 64));
         }
-        aj = shfl(aj, (j - j0) % 64);
-        ap = shfl(ap, (p - j0) % 64);
-        const zc x = ap;   // row j after the interchange
 #pragma unroll
-        for (int m = 0; m < kTrailRows; m++) {
+        for (int m = 0; m < TR; m++) {
             const int row = j0 + lane + 64 * m;
             if (p != j) {
-                if (row == j) v[m] = ap;
-                else if (row == p) v[m] = aj;
+#pragma unroll
+                for (int u = 0; u < CPW; u++) {
+                    if (c0 + u * W > j + kv) continue;
+                    if (row == j) v[u][m] = x[u];
+                    else if (row == p) v[u][m] = aj[u];
+                }
             }
-            if (row > j && row <= last) v[m] = zsub(v[m], zmul(L[(size_t)(j - j0) * R + (row - j0)], x));
+            if (row > j && row <= last) {
+                const zc l = L[(size_t)(j - j0) * R + (row - j0)];
+#pragma unroll
+                for (int u = 0; u < CPW; u++)
+                    if (c0 + u * W <= j + kv) v[u][m] = zsub(v[u][m], zmul(l, x[u]));
+            }
         }
     }
 #pragma unroll
-    for (int m = 0; m < kTrailRows; m++) {
-        const int row = j0 + lane + 64 * m;
-        if (row < j0 + R && row >= rlo) band[bidx(g, row, c)] = v[m];
+    for (int u = 0; u < CPW; u++) {
+        const int c = c0 + u * W, rlo = max(j0, c - kv);
+        if (c > cend) break;
+#pragma unroll
+        for (int m = 0; m < TR; m++) {
+            const int row = j0 + lane + 64 * m;
+            if (row < j0 + R && row >= rlo) band[bidx(g, row, c)] = v[u][m];
+        }
     }
 }
 
@@ -875,9 +901,10 @@ int set_lds_attributes() {
                          (const void *)k_inverse<8, 14>, (const void *)k_inverse<8, 15>};
     for (const void *f : inv)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInvLdsMax) != hipSuccess) return -1;
-    if (hipFuncSetAttribute((const void *)k_lu_trail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTrailLdsMax) !=
-        hipSuccess)
-        return -1;
+    for (const void *f : {(const void *)k_lu_trail<2, 1>, (const void *)k_lu_trail<4, 1>, (const void *)k_lu_trail<8, 1>,
+                          (const void *)k_lu_trail<2, 2>, (const void *)k_lu_trail<4, 2>, (const void *)k_lu_trail<8, 2>})
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTrailLdsMax) != hipSuccess)
+            return -1;
     done[dev] = 1;
     return 0;
 }
@@ -1118,9 +1145,24 @@ int mode_gpu_solve_batch(ModeGpu **ctx, const ModeGpuProblem *P, int nbq, std::c
                 put(k_lu_panel<8>, dim3(1, by), pb, 0u, g, M.band.p, M.ipiv.p, M.info.p, j0, j1);
             const int ncol = std::min(j1 - 1 + kv, n - 1) - j1 + 1;
             if (ncol > 0) {
-                const int wpb = kLuThreads / 64;
-                put(k_lu_trail, dim3((ncol + wpb - 1) / wpb, by), dim3(kLuThreads), lds, g, M.band.p, M.ipiv.p, j0,
-                    j1);
+                // two columns per wavefront once one per wavefront would take more than
+                // two workgroups per CU (their LDS copies of the panel bound it there): half
+                // the LDS copies and multiplier reads; fewer problems keep one column each
+                // for parallelism
+                const int w1 = kLuThreads / 64;
+                const bool two = (long long)by * ((ncol + w1 - 1) / w1) > 2 * kCUs;
+                const int wpb = two ? 2 * w1 : w1;   // columns per workgroup
+                const dim3 tg((ncol + wpb - 1) / wpb, by), tb(kLuThreads);
+                if (R <= 128) {
+                    if (two) put(k_lu_trail<2, 2>, tg, tb, lds, g, M.band.p, M.ipiv.p, j0, j1);
+                    else put(k_lu_trail<2, 1>, tg, tb, lds, g, M.band.p, M.ipiv.p, j0, j1);
+                } else if (R <= 256) {
+                    if (two) put(k_lu_trail<4, 2>, tg, tb, lds, g, M.band.p, M.ipiv.p, j0, j1);
+                    else put(k_lu_trail<4, 1>, tg, tb, lds, g, M.band.p, M.ipiv.p, j0, j1);
+                } else {
+                    if (two) put(k_lu_trail<8, 2>, tg, tb, lds, g, M.band.p, M.ipiv.p, j0, j1);
+                    else put(k_lu_trail<8, 1>, tg, tb, lds, g, M.band.p, M.ipiv.p, j0, j1);
+                }
             }
         }
     };

@@ -618,11 +618,14 @@ def _compare_gpu_host(cem, monkeypatch, m, nu, nv, f, neff):
     return hs, gs
 
 
-def test_mode2d_gpu_matches_host_waveguide(cem, gpu_available, monkeypatch):
+@pytest.mark.parametrize("NZ", [12, 120])
+def test_mode2d_gpu_matches_host_waveguide(cem, gpu_available, monkeypatch, NZ):
     """WR-28 cross-section (test_mode2d.c geometry): the GPU band LU + inverse iteration gives
-    the host solver's TE10 eigenpair, from the fixed start vector and from a seeded start."""
-    print_case("mode2d GPU vs host: WR-28")
-    a, b, NY, NZ = 7.112e-3, 3.556e-3, 24, 12
+    the host solver's TE10 eigenpair, from the fixed start vector and from a seeded start.
+    NZ = 120 makes the band (half-width 244) wide enough for the LU's widest trailing-update
+    kernel (panels of more than 256 rows)."""
+    print_case("mode2d GPU vs host: WR-28, %d z cells" % NZ)
+    a, b, NY = 7.112e-3, 3.556e-3, 24
     y = np.array([a * j / NY for j in range(NY + 1)])
     z = np.array([b * k / NZ for k in range(NZ + 1)])
     m = cem.lib.cem_mode2d_create(NY, NZ, y.ctypes.data_as(L._DP), z.ctypes.data_as(L._DP))
