@@ -186,6 +186,41 @@ def test_batched_port_sweep_matches_sequential_sweep(cem, gpu_available, monkeyp
             assert err <= 1e-9
 
 
+def test_wide_batches_match_narrow_batches(cem, gpu_available, monkeypatch):
+    """400 output frequencies swept in batches of up to 64 frequencies per direction (128
+    problems per launch: the LU's trailing update then runs two columns per wavefront) against
+    batches of three per direction (one column per wavefront).  The two batchings seed each
+    frequency differently and inverse iteration stops at |d lambda| < 1e-13 |lambda| per
+    iteration, which at its slowest-converging frequencies, the lowest (0.1-0.2 GHz), leaves
+    seed-dependent differences (measured: up to 2.0e-9 relative in Z0, 3.3e-10 in beta, 5.1e-9
+    in alpha, a small imaginary part of the propagation constant; 2.2e-9 of max |a| in the
+    waves), against ~5e-12 from 2 GHz up: Z0, beta and alpha within 1e-8 relative at every
+    frequency, the waves within 1e-8 of max |a|.  A wrong factorisation gives differences of
+    order one."""
+    print_case("port-mode sweep, 64 vs 3 frequencies per direction per batch")
+    mini = dict(MODE_MINI, df=0.1e9)
+    res = {}
+    for k in ("64", "3"):
+        monkeypatch.setenv("COURANT_MODE_BATCH", k)
+        b, ib = S.gcpw(cem, **mini)
+        b.step(400)
+        res[k] = [(b.mode_port_waves(p), b.mode_port_alpha(p)) for p in ib["ports"]]
+        b.close()
+    monkeypatch.delenv("COURANT_MODE_BATCH", raising=False)
+    for (ws, als), (wb, alb) in zip(res["3"], res["64"]):
+        assert len(ws["z0"]) == 400
+        for k, x, y, tol in (("z0", ws["z0"], wb["z0"], 1e-8), ("beta", ws["beta"], wb["beta"], 1e-8),
+                             ("alpha", als, alb, 1e-8)):
+            err = np.max(np.abs(x - y) / np.abs(x))
+            print("  %s: max rel diff %.2e at %.2f GHz" % (k, err, mini["df"] * (1 + np.argmax(np.abs(x - y) / np.abs(x))) / 1e9))
+            assert err <= tol
+        scale = np.max(np.abs(ws["a"]))
+        for k in ("a", "b"):
+            err = np.max(np.abs(ws[k] - wb[k])) / scale
+            print("  %s: max diff %.2e of max |a|" % (k, err))
+            assert err <= 1e-8
+
+
 def test_device_wave_separation_matches_host_formula(cem, gpu_available, monkeypatch):
     """The FP64 wave separation on the GPU against the same formula on the host, on the
     same simulation (same spectra, same solved port modes): 1e-12 of max |a|."""
